@@ -1,0 +1,271 @@
+/*
+ * wv_hip.h -- C ABI of libwv_hip.so, the MI355X (gfx950) implementation of
+ * the Wavelets compression hot path: multi-level CDF 9/7 / Haar DWT over a
+ * cubelet, threshold selection, uniform / log / Lloyd quantization with a
+ * fused bin histogram, and the host Huffman + .cube writer that consume the
+ * bins and counts.
+ *
+ * Conventions
+ *  - plain pointers and sizes, no C++ or torch types; every call returns an
+ *    int status (WV_OK or a negative WV_ERR_*) and never exits; the message
+ *    of the last failure on the calling thread is wv_last_error().
+ *  - device buffers are owned by the caller; every device call takes an
+ *    explicit hipStream_t (passed as void*, 0 = the null stream) and is
+ *    asynchronous unless its comment says it synchronises.
+ *  - "geometry" sizes are (x, y, z) with x fastest, as Cube::size
+ *    (wavelet.h:305-435).
+ *
+ * Each entry point cites the reference interface it replaces.
+ */
+#ifndef WV_HIP_H
+#define WV_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WV_OK 0
+#define WV_ERR_ARG -1          /* invalid argument / shape */
+#define WV_ERR_HIP -2          /* HIP runtime error */
+#define WV_ERR_UNSUPPORTED -3  /* shape or option outside what is built */
+#define WV_ERR_NOMEM -4        /* workspace too small / budget exceeded */
+#define WV_ERR_STATE -5        /* call order / busy */
+
+/* ids follow wavelet.h:25-46 */
+#define WV_QUANT_UNIFORM 1
+#define WV_QUANT_LOG 2
+#define WV_QUANT_LLOYD 3
+#define WV_WAVELET_HAAR 1
+#define WV_WAVELET_CDF97 2
+#define WV_DTYPE_UINT8 1
+#define WV_DTYPE_INT32 2
+#define WV_DTYPE_FLOAT32 3
+
+const char *wv_last_error(void);
+/* library version string, e.g. "wv_hip 1.0 gfx950" */
+const char *wv_version(void);
+
+/* ------------------------------------------------------------------ *
+ * Geometry of one cubelet.  Replaces readData's size handling:
+ * setWaveletSteps + padDataSize (test_compress_common.cc:362-382).
+ * ------------------------------------------------------------------ */
+typedef struct {
+  int orig[3];       /* input size before padding */
+  int steps[3];      /* transform steps per axis after clamping */
+  int padded[3];     /* padded size (multiple of 2^steps per axis) */
+  int out_size[3];   /* size of the transformed cube (rotated: z,x,y) */
+  int wavelet;       /* WV_WAVELET_* */
+  int dtype;         /* WV_DTYPE_* of the input */
+  int64_t count;     /* padded element count */
+  int64_t ws_floats; /* workspace floats wv_dwt3d needs */
+} wv_geom;
+
+/* steps < 0 or > ceil(log2(len)) become ceil(log2(len)). */
+int wv_geom_init(wv_geom *g, const int orig[3], const int steps[3],
+                 int wavelet, int dtype);
+
+/* ------------------------------------------------------------------ *
+ * Device statistics of a coefficient array, produced as a side effect of
+ * the last DWT pass (or by wv_stats_compute): a 2048-bin histogram of bits
+ * 30..20 of |x|, the signed extremes and max |x|.  One WV_STATS_BYTES
+ * device block; clear it with wv_stats_clear before a forward transform.
+ * ------------------------------------------------------------------ */
+#define WV_STATS_BYTES 8256
+
+int wv_stats_clear(void *stats_d, void *stream);
+/* standalone statistics pass for coefficients that did not come out of
+   wv_dwt3d_forward */
+int wv_stats_compute(const float *coef_d, int64_t n, void *stats_d,
+                     void *stream);
+
+/* ------------------------------------------------------------------ *
+ * Forward 3-D transform.  Replaces waveletTransformGPU(..., inverse=false)
+ * (test_compress_gpu.cu:45-49,504-593; the math of cdf97_3d / haar_3d,
+ * dwt_cpu.cc:624-648,943-965).  Reads the unpadded input in its own dtype
+ * (u8 / i32 / f32, converted and REPEAT-padded on load as
+ * translateCubeDataToFloat + padData do, test_compress_common.cc:386-488),
+ * runs every level of every axis, and writes out_d in the rotated layout
+ * (size g->out_size).  ws_d must hold g->ws_floats floats.  stats_d (may
+ * be NULL) receives the fused statistics and must be cleared first.
+ * Coefficients are bit-identical to the reference CPU transform.
+ * ------------------------------------------------------------------ */
+int wv_dwt3d_forward(const wv_geom *g, const void *in_d, float *out_d,
+                     float *ws_d, void *stats_d, void *stream);
+
+/* Same, recording a hipEvent_t (passed as void*) before the first pass and
+   after every pass: events must hold wv_dwt3d_num_passes(g)+1 events.
+   Lets a caller time each fused axis pass on its own stream. */
+int wv_dwt3d_forward_ev(const wv_geom *g, const void *in_d, float *out_d,
+                        float *ws_d, void *stats_d, void **events,
+                        void *stream);
+int wv_dwt3d_num_passes(const wv_geom *g);
+
+/* Inverse 3-D transform (waveletTransformGPU(..., inverse=true); the math
+   of cdf97_inverse / haar_inv, dwt_cpu.cc:174-203,757-799,966-984).
+   coef_d is in g->out_size layout; out_d receives the padded cube in x,y,z
+   layout.  ws_d holds g->ws_floats floats.  coef_d is not modified. */
+int wv_dwt3d_inverse(const wv_geom *g, const float *coef_d, float *out_d,
+                     float *ws_d, void *stream);
+
+/* ------------------------------------------------------------------ *
+ * Threshold.  Replaces sortAbsAndFindLimitsGPU + the selection at
+ * test_compress_gpu.cu:295-308,632-662 (thrust::sort of |x|) with a radix
+ * select; semantics of thresh_cpu (thresh_cpu.cc:80-109): index
+ * (int)(frac*(float)n), threshold = the idx-th smallest |x| (0 -> 1e-16),
+ * nonzero = n-idx-1, min/max with FLT_MAX / FLT_MIN starts.
+ * ------------------------------------------------------------------ */
+typedef struct {
+  float threshold;
+  float min_val, max_val, max_abs;
+  int32_t thresh_idx, nonzero_count;
+  uint32_t threshold_key;   /* bit pattern of the threshold before 0->1e-16 */
+  int32_t n_candidates;     /* internal: size of the last candidate set */
+} wv_thresh_result;
+
+/* workspace bytes for wv_threshold on n coefficients */
+int64_t wv_threshold_ws_bytes(int64_t n);
+/* stats_d from the forward transform (already complete); result_d is a
+   device wv_thresh_result.  Does not synchronise. */
+int wv_threshold(const float *coef_d, int64_t n, float frac,
+                 const void *stats_d, void *ws_d, wv_thresh_result *result_d,
+                 void *stream);
+
+/* ------------------------------------------------------------------ *
+ * Quantizer parameters, computed on the host with the host libm exactly
+ * as QuantUniform::init / QuantLog::init / QuantCodebook::init
+ * (quant.h:196-242,337-372,487-492).  For the log quantizer the host also
+ * tabulates the exact bin edges (the smallest float of every bin), so the
+ * device assigns bins by comparison and reproduces the host logf bit for
+ * bit without evaluating a logarithm.
+ * ------------------------------------------------------------------ */
+#define WV_MAX_BINS 65536
+typedef struct {
+  int alg, bin_count, zero_bin;
+  float threshold, max_val;
+  float neg_scale, neg_inv_scale, neg_offset;
+  float pos_scale, pos_inv_scale, pos_offset;
+  int n_pos_edges, n_neg_edges;  /* log: edge counts */
+  float pos_clamp;               /* log: first x whose bin is clamped */
+  int n_boundaries;              /* codebook: bin_count-1 */
+  const float *pos_edges;        /* host arrays owned by the params */
+  const float *neg_edges;
+  const float *boundaries;       /* codebook boundaries (bin_count-1) */
+  const float *values;           /* codebook values (bin_count) */
+  const float *table_d;          /* device copy of the tables, set by
+                                    wv_quant_params_upload */
+  void *priv;
+} wv_quant_params;
+
+int wv_quant_params_init(wv_quant_params *p, int alg, int bin_count,
+                         float threshold, float max_val,
+                         const float *boundaries, const float *values);
+void wv_quant_params_free(wv_quant_params *p);
+/* device bytes of the parameter tables (edges or boundaries, then the
+   bin_count dequantization values) */
+int64_t wv_quant_table_bytes(const wv_quant_params *p);
+/* copies the tables into table_d (wv_quant_table_bytes bytes, caller-owned
+   device memory) on the stream and records it in p->table_d */
+int wv_quant_params_upload(wv_quant_params *p, void *table_d, void *stream);
+/* host quantization of one value (Quantizer::quant, quant.h:33-37) */
+int wv_quant_host(const wv_quant_params *p, float x);
+float wv_dequant_host(const wv_quant_params *p, int bin);
+
+/* ------------------------------------------------------------------ *
+ * Quantize + histogram.  Replaces quantizeGPU (quant_gpu.cu:122-182, the
+ * thrust::transform with Quant*Functor) fused with computeFrequenciesGPU
+ * (histogram_gpu.cu:117-179): bins_d[i] = quant(coef_d[i]) and
+ * counts_d[b] = #{i : bins_d[i] == b}.  counts_d (bin_count + 1 uint32;
+ * the extra entry counts bins outside [0, bin_count), which the reference
+ * log quantizer can produce for |x| above its max) is cleared by the call.
+ * bins_d may alias coef_d.
+ * ------------------------------------------------------------------ */
+int wv_quantize_hist(const float *coef_d, int64_t n, const wv_quant_params *p,
+                     int32_t *bins_d, uint32_t *counts_d, void *stream);
+
+/* Device-parameter uniform quantization: the threshold and max |x| are
+   read from result_d on the device (no host round trip).  The parameter
+   arithmetic is the same float expressions as QuantUniform::init. */
+int wv_quantize_hist_uniform_dev(const float *coef_d, int64_t n,
+                                 int bin_count,
+                                 const wv_thresh_result *result_d,
+                                 int32_t *bins_d, uint32_t *counts_d,
+                                 void *stream);
+
+/* Dequantize (dequantizeGPU, quant_gpu.cu:237-300) through the uploaded
+   dequantization table. */
+int wv_dequantize(const int32_t *bins_d, int64_t n, const wv_quant_params *p,
+                  float *out_d, void *stream);
+
+/* ------------------------------------------------------------------ *
+ * Lloyd-Max codebook on the device.  Replaces computeLloydQuantizationGPU
+ * / cudaLloyd (quant_gpu.cu:185-233, cudalloyds.cu:85-194) with the CPU
+ * algorithm's semantics (computeLloydQuantization test_compress_cpu.cc:
+ * 404-459, lloyd lloyds.cpp:30-141): points are the sorted |x| above rank
+ * thresh_idx without the largest, sums in sequential float order, stop at
+ * reldist <= 1e-6 or max_iters (<0: no cap).  Synchronises the stream.
+ * bin_boundaries / bin_values are host arrays of bin_count-1 / bin_count.
+ * ------------------------------------------------------------------ */
+int64_t wv_lloyd_ws_bytes(int64_t n, int bin_count);
+int wv_lloyd(const float *coef_d, int64_t n, const wv_thresh_result *res_host,
+             int bin_count, int max_iters, void *ws_d, float *bin_boundaries,
+             float *bin_values, int *iters_out, void *stream);
+
+/* ------------------------------------------------------------------ *
+ * Host Huffman coder + .cube writer (writeQuantData,
+ * test_compress_common.cc:198-237,897-1022; huffman.h/.cc; bit_stream.h;
+ * cubelet_file.cc:18-84,153-327).  counts are the device histogram; the
+ * zero-bin run statistics are recomputed from bins like huffScanDupsOnly.
+ * ------------------------------------------------------------------ */
+typedef struct {
+  int orig[3], steps[3], out_size[3];
+  int wavelet, quant_alg, bin_count, orig_dtype;
+  float thresh_frac, threshold, max_val;
+  const float *boundaries, *values;   /* Lloyd only */
+  int zero_bin;                       /* -1: dup encoding off (-noz) */
+  int with_footer;                    /* test_compress_gpu writes it */
+} wv_cube_desc;
+
+/* Encodes bins (host array, n values) and writes a complete .cube file
+   image into file_buf (cap bytes).  Returns bytes written, or a negative
+   status; *payload_bytes (optional) receives the Huffman payload size. */
+int64_t wv_write_cube(const wv_cube_desc *d, const int32_t *bins_host,
+                      int64_t n, const uint32_t *counts_host,
+                      uint8_t *file_buf, int64_t cap, int64_t *payload_bytes);
+
+/* ------------------------------------------------------------------ *
+ * One-call drop-in: the whole of compressFile (test_compress_gpu.cu:
+ * 150-436) on a device-resident input.  Synchronises.  bins_host (may be
+ * NULL) receives the bins; file_buf the .cube image.
+ * ------------------------------------------------------------------ */
+typedef struct {
+  int wavelet, quant_alg, bin_count;
+  int steps[3];
+  float thresh_frac;
+  int compress_zeros;     /* 0 = -noz */
+  int lloyd_max_iters;    /* <0: no cap */
+  int with_footer;
+} wv_options;
+
+typedef struct {
+  wv_thresh_result thresh;
+  int out_size[3], steps[3];
+  int zero_bin, lloyd_iters;
+  int64_t payload_bytes, file_bytes;
+  double ms_device;       /* device pipeline time (events) */
+  double ms_total;        /* including D2H and host Huffman */
+} wv_compress_info;
+
+int wv_compress(const void *in_d, int dtype, const int orig[3],
+                const wv_options *opt, void *work_d, int64_t work_bytes,
+                int32_t *bins_host, uint8_t *file_buf, int64_t file_cap,
+                wv_compress_info *info, void *stream);
+/* device bytes wv_compress needs in work_d */
+int64_t wv_compress_work_bytes(const int orig[3], const wv_options *opt);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

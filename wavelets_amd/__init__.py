@@ -1,0 +1,313 @@
+"""wavelets_amd -- MI355X (gfx950) implementation of the Wavelets compression
+hot path (oshkosher/Wavelets, test_compress_gpu): 3-D CDF 9/7 / Haar DWT,
+threshold selection, uniform / log / Lloyd quantization with a fused bin
+histogram, and the Huffman + .cube writer, behind the C ABI of
+libwv_hip.so (include/wv_hip.h).
+
+This module is the Python mirror of that ABI (ctypes).  torch is used only
+for device memory and streams.  There is no CPU fallback: every call runs
+the HIP library and fails loudly if it is not built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libwv_hip.so")
+
+HAAR, CDF97 = 1, 2
+UNIFORM, LOG, LLOYD = 1, 2, 3
+UINT8, INT32, FLOAT32 = 1, 2, 3
+STATS_BYTES = 8256
+
+_lib = None
+
+
+class Geom(C.Structure):
+    _fields_ = [("orig", C.c_int * 3), ("steps", C.c_int * 3),
+                ("padded", C.c_int * 3), ("out_size", C.c_int * 3),
+                ("wavelet", C.c_int), ("dtype", C.c_int),
+                ("count", C.c_int64), ("ws_floats", C.c_int64)]
+
+
+class ThreshResult(C.Structure):
+    _fields_ = [("threshold", C.c_float), ("min_val", C.c_float),
+                ("max_val", C.c_float), ("max_abs", C.c_float),
+                ("thresh_idx", C.c_int32), ("nonzero_count", C.c_int32),
+                ("threshold_key", C.c_uint32), ("n_candidates", C.c_int32)]
+
+
+class QuantParams(C.Structure):
+    _fields_ = [("alg", C.c_int), ("bin_count", C.c_int), ("zero_bin", C.c_int),
+                ("threshold", C.c_float), ("max_val", C.c_float),
+                ("neg_scale", C.c_float), ("neg_inv_scale", C.c_float),
+                ("neg_offset", C.c_float), ("pos_scale", C.c_float),
+                ("pos_inv_scale", C.c_float), ("pos_offset", C.c_float),
+                ("n_pos_edges", C.c_int), ("n_neg_edges", C.c_int),
+                ("pos_clamp", C.c_float), ("n_boundaries", C.c_int),
+                ("pos_edges", C.POINTER(C.c_float)),
+                ("neg_edges", C.POINTER(C.c_float)),
+                ("boundaries", C.POINTER(C.c_float)),
+                ("values", C.POINTER(C.c_float)),
+                ("table_d", C.c_void_p), ("priv", C.c_void_p)]
+
+
+class CubeDesc(C.Structure):
+    _fields_ = [("orig", C.c_int * 3), ("steps", C.c_int * 3),
+                ("out_size", C.c_int * 3), ("wavelet", C.c_int),
+                ("quant_alg", C.c_int), ("bin_count", C.c_int),
+                ("orig_dtype", C.c_int), ("thresh_frac", C.c_float),
+                ("threshold", C.c_float), ("max_val", C.c_float),
+                ("boundaries", C.c_void_p), ("values", C.c_void_p),
+                ("zero_bin", C.c_int), ("with_footer", C.c_int)]
+
+
+class Options(C.Structure):
+    _fields_ = [("wavelet", C.c_int), ("quant_alg", C.c_int),
+                ("bin_count", C.c_int), ("steps", C.c_int * 3),
+                ("thresh_frac", C.c_float), ("compress_zeros", C.c_int),
+                ("lloyd_max_iters", C.c_int), ("with_footer", C.c_int)]
+
+
+class CompressInfo(C.Structure):
+    _fields_ = [("thresh", ThreshResult), ("out_size", C.c_int * 3),
+                ("steps", C.c_int * 3), ("zero_bin", C.c_int),
+                ("lloyd_iters", C.c_int), ("payload_bytes", C.c_int64),
+                ("file_bytes", C.c_int64), ("ms_device", C.c_double),
+                ("ms_total", C.c_double)]
+
+
+P = C.c_void_p
+I64 = C.c_int64
+
+
+def _declare(L):
+    L.wv_last_error.restype = C.c_char_p
+    L.wv_version.restype = C.c_char_p
+    L.wv_geom_init.argtypes = [C.POINTER(Geom), P, P, C.c_int, C.c_int]
+    L.wv_stats_clear.argtypes = [P, P]
+    L.wv_stats_compute.argtypes = [P, I64, P, P]
+    L.wv_dwt3d_forward.argtypes = [C.POINTER(Geom), P, P, P, P, P]
+    L.wv_dwt3d_inverse.argtypes = [C.POINTER(Geom), P, P, P, P]
+    L.wv_dwt3d_forward_ev.argtypes = [C.POINTER(Geom), P, P, P, P, P, P]
+    L.wv_dwt3d_num_passes.argtypes = [C.POINTER(Geom)]
+    L.wv_threshold_ws_bytes.restype = I64
+    L.wv_threshold_ws_bytes.argtypes = [I64]
+    L.wv_threshold.argtypes = [P, I64, C.c_float, P, P, P, P]
+    L.wv_quant_params_init.argtypes = [C.POINTER(QuantParams), C.c_int, C.c_int,
+                                       C.c_float, C.c_float, P, P]
+    L.wv_quant_params_free.argtypes = [C.POINTER(QuantParams)]
+    L.wv_quant_table_bytes.restype = I64
+    L.wv_quant_table_bytes.argtypes = [C.POINTER(QuantParams)]
+    L.wv_quant_params_upload.argtypes = [C.POINTER(QuantParams), P, P]
+    L.wv_quant_host.argtypes = [C.POINTER(QuantParams), C.c_float]
+    L.wv_dequant_host.restype = C.c_float
+    L.wv_dequant_host.argtypes = [C.POINTER(QuantParams), C.c_int]
+    L.wv_quantize_hist.argtypes = [P, I64, C.POINTER(QuantParams), P, P, P]
+    L.wv_quantize_hist_uniform_dev.argtypes = [P, I64, C.c_int, P, P, P, P]
+    L.wv_dequantize.argtypes = [P, I64, C.POINTER(QuantParams), P, P]
+    L.wv_write_cube.restype = I64
+    L.wv_write_cube.argtypes = [C.POINTER(CubeDesc), P, I64, P, P, I64, P]
+    if hasattr(L, "wv_compress"):
+        L.wv_compress.argtypes = [P, C.c_int, P, C.POINTER(Options), P, I64, P,
+                                  P, I64, C.POINTER(CompressInfo), P]
+        L.wv_compress_work_bytes.restype = I64
+        L.wv_compress_work_bytes.argtypes = [P, C.POINTER(Options)]
+    if hasattr(L, "wv_lloyd"):
+        L.wv_lloyd_ws_bytes.restype = I64
+        L.wv_lloyd_ws_bytes.argtypes = [I64, C.c_int]
+        L.wv_lloyd.argtypes = [P, I64, C.POINTER(ThreshResult), C.c_int, C.c_int,
+                               P, P, P, P, P]
+
+
+def lib():
+    """The loaded libwv_hip.so.  torch is imported first so the HIP runtime
+    it carries is the one the library binds to."""
+    global _lib
+    if _lib is None:
+        import torch  # noqa: F401
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libwv_hip.so is not built: run python -m wavelets_amd.build")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+class WvError(RuntimeError):
+    pass
+
+
+def check(rc):
+    if rc < 0:
+        raise WvError("wv_hip error %d: %s" % (rc, lib().wv_last_error().decode()))
+    return rc
+
+
+def _i3(v):
+    return (C.c_int * 3)(*[int(a) for a in v])
+
+
+def _stream(stream=None):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return C.c_void_p(s.cuda_stream)
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+DTYPE_ID = {"uint8": UINT8, "int32": INT32, "float32": FLOAT32}
+
+
+def dtype_id(t):
+    name = str(t.dtype).replace("torch.", "")
+    if name not in DTYPE_ID:
+        raise ValueError("unsupported input dtype %s (uint8, int32, float32)" % name)
+    return DTYPE_ID[name]
+
+
+def geom(orig, steps, wavelet, dtype):
+    g = Geom()
+    check(lib().wv_geom_init(C.byref(g), _i3(orig), _i3(steps), wavelet, dtype))
+    return g
+
+
+def dwt3d_forward(vol, steps, wavelet, with_stats=True, stream=None):
+    """vol: cuda tensor [z, y, x] of uint8 / int32 / float32.
+    Returns (coef, geom, stats) with coef flat float32 in the rotated
+    layout of geom.out_size."""
+    import torch
+    orig = [vol.shape[2], vol.shape[1], vol.shape[0]]
+    g = geom(orig, steps, wavelet, dtype_id(vol))
+    dev = vol.device
+    coef = torch.empty(g.count, dtype=torch.float32, device=dev)
+    ws = torch.empty(max(g.ws_floats, 1), dtype=torch.float32, device=dev)
+    stats = None
+    s = _stream(stream)
+    if with_stats:
+        stats = torch.empty(STATS_BYTES, dtype=torch.uint8, device=dev)
+        check(lib().wv_stats_clear(_ptr(stats), s))
+    check(lib().wv_dwt3d_forward(C.byref(g), _ptr(vol.contiguous()), _ptr(coef), _ptr(ws),
+                                 _ptr(stats), s))
+    return coef, g, stats
+
+
+def dwt3d_inverse(coef, g, stream=None):
+    import torch
+    out = torch.empty(g.count, dtype=torch.float32, device=coef.device)
+    ws = torch.empty(max(g.ws_floats, 1), dtype=torch.float32, device=coef.device)
+    check(lib().wv_dwt3d_inverse(C.byref(g), _ptr(coef), _ptr(out), _ptr(ws), _stream(stream)))
+    return out
+
+
+def stats_compute(coef, stream=None):
+    import torch
+    stats = torch.empty(STATS_BYTES, dtype=torch.uint8, device=coef.device)
+    check(lib().wv_stats_compute(_ptr(coef), coef.numel(), _ptr(stats), _stream(stream)))
+    return stats
+
+
+def threshold(coef, frac, stats, stream=None, sync=True):
+    """Returns a device tensor holding a wv_thresh_result and, with sync,
+    the host ThreshResult."""
+    import torch
+    n = coef.numel()
+    ws = torch.empty(lib().wv_threshold_ws_bytes(n), dtype=torch.uint8, device=coef.device)
+    res = torch.zeros(C.sizeof(ThreshResult), dtype=torch.uint8, device=coef.device)
+    check(lib().wv_threshold(_ptr(coef), n, C.c_float(frac), _ptr(stats), _ptr(ws), _ptr(res),
+                             _stream(stream)))
+    if not sync:
+        return res, None
+    host = ThreshResult.from_buffer_copy(res.cpu().numpy().tobytes())
+    return res, host
+
+
+class Quantizer:
+    """Host parameters + device tables of one quantizer."""
+
+    def __init__(self, alg, bin_count, threshold, max_val, boundaries=None, values=None,
+                 device="cuda", stream=None):
+        import torch
+        self.p = QuantParams()
+        b = None if boundaries is None else np.ascontiguousarray(boundaries, np.float32)
+        v = None if values is None else np.ascontiguousarray(values, np.float32)
+        self._keep = (b, v)
+        check(lib().wv_quant_params_init(
+            C.byref(self.p), alg, bin_count, C.c_float(threshold), C.c_float(max_val),
+            b.ctypes.data_as(P) if b is not None else None,
+            v.ctypes.data_as(P) if v is not None else None))
+        nbytes = lib().wv_quant_table_bytes(C.byref(self.p))
+        self.table = torch.empty(max(nbytes, 4), dtype=torch.uint8, device=device)
+        check(lib().wv_quant_params_upload(C.byref(self.p), _ptr(self.table), _stream(stream)))
+
+    def __del__(self):
+        try:
+            lib().wv_quant_params_free(C.byref(self.p))
+        except Exception:
+            pass
+
+    @property
+    def zero_bin(self):
+        return self.p.zero_bin
+
+    def host_quant(self, x):
+        return lib().wv_quant_host(C.byref(self.p), C.c_float(x))
+
+    def edges(self):
+        pe = np.ctypeslib.as_array(self.p.pos_edges, (self.p.n_pos_edges,)).copy() \
+            if self.p.n_pos_edges else np.zeros(0, np.float32)
+        ne = np.ctypeslib.as_array(self.p.neg_edges, (self.p.n_neg_edges,)).copy() \
+            if self.p.n_neg_edges else np.zeros(0, np.float32)
+        return pe, ne
+
+    def quantize(self, coef, out=None, stream=None):
+        import torch
+        n = coef.numel()
+        bins = out if out is not None else torch.empty(n, dtype=torch.int32, device=coef.device)
+        counts = torch.empty(self.p.bin_count + 64, dtype=torch.int32, device=coef.device)
+        check(lib().wv_quantize_hist(_ptr(coef), n, C.byref(self.p), _ptr(bins), _ptr(counts),
+                                     _stream(stream)))
+        return bins, counts[:self.p.bin_count]
+
+    def dequantize(self, bins, stream=None):
+        import torch
+        out = torch.empty(bins.numel(), dtype=torch.float32, device=bins.device)
+        check(lib().wv_dequantize(_ptr(bins), bins.numel(), C.byref(self.p), _ptr(out),
+                                  _stream(stream)))
+        return out
+
+
+def quantize_uniform_dev(coef, bin_count, res_dev, stream=None):
+    import torch
+    n = coef.numel()
+    bins = torch.empty(n, dtype=torch.int32, device=coef.device)
+    counts = torch.empty(((bin_count + 63) // 64) * 64 + 64, dtype=torch.int32,
+                         device=coef.device)
+    check(lib().wv_quantize_hist_uniform_dev(_ptr(coef), n, bin_count, _ptr(res_dev),
+                                             _ptr(bins), _ptr(counts), _stream(stream)))
+    return bins, counts[:bin_count]
+
+
+def write_cube(desc_kwargs, bins_host, counts_host):
+    """bins_host / counts_host: numpy int32 arrays.  Returns the .cube bytes."""
+    d = CubeDesc()
+    keep = []
+    for k, v in desc_kwargs.items():
+        if k in ("orig", "steps", "out_size"):
+            setattr(d, k, _i3(v))
+        elif k in ("boundaries", "values"):
+            arr = np.ascontiguousarray(v, np.float32)
+            keep.append(arr)
+            setattr(d, k, arr.ctypes.data_as(P))
+        else:
+            setattr(d, k, v)
+    b = np.ascontiguousarray(bins_host, np.int32)
+    c = np.ascontiguousarray(counts_host, np.uint32)
+    cap = b.size * 8 + (1 << 20)
+    buf = np.empty(cap, np.uint8)
+    pay = C.c_int64()
+    n = check(lib().wv_write_cube(C.byref(d), b.ctypes.data_as(P), b.size, c.ctypes.data_as(P),
+                                  buf.ctypes.data_as(P), cap, C.byref(pay)))
+    return buf[:n].tobytes()

@@ -1,0 +1,42 @@
+// wv_device.h -- device-side helpers shared by the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define WV_BLOCK 256
+#define WV_MAXP 32          // max (low,high) pairs a thread owns per level
+#define WV_H1_BITS 11       // threshold select digit 1: bits 30..20 of |x|
+#define WV_H1_BINS 2048
+#define WV_H1_SHIFT 20
+#define WV_H2_BINS 2048     // digit 2: bits 19..9
+#define WV_H2_SHIFT 9
+#define WV_H3_BINS 512      // digit 3: bits 8..0
+
+// Fused statistics block (device).  Layout is part of the ABI size
+// WV_STATS_BYTES.
+struct WvStats {
+  uint32_t hist[WV_H1_BINS];
+  uint32_t min_ord, max_ord, max_key, count_lo;
+  uint32_t pad[12];
+};
+
+// Total order on floats through their bit patterns (no NaNs occur).
+__device__ __host__ inline uint32_t wv_f2ord(float f) {
+  uint32_t u = __builtin_bit_cast(uint32_t, f);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __host__ inline float wv_ord2f(uint32_t o) {
+  uint32_t u = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+  return __builtin_bit_cast(float, u);
+}
+
+// MirroredArray::getOffset (dwt_cpu.h:162-176): whole-sample symmetric
+// extension, folded as many times as needed (lines of length 2 and 4).
+__device__ inline int wv_mirror(int o, int n) {
+  if (o < 0) o = -o;
+  while (o >= n) {
+    o = n * 2 - o - 2;
+    if (o < 0) o = -o;
+  }
+  return o;
+}

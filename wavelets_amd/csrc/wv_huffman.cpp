@@ -1,0 +1,319 @@
+// wv_huffman.cpp -- the host half of the compress path: Huffman coding of
+// the quantized bins with the zero-bin duplicate-string code, the LSB-first
+// word bit stream, and the .cube container writer.
+//
+// Semantics follow the reference host code the GPU driver calls after the
+// device stages: initHuffman(freqCounts, zeroBin) + huffScanDupsOnly
+// (test_compress_common.cc:927-973), Huffman::computeHuffmanCoding
+// (huffman.cc:42-97, ties broken toward the later node), encodeToStream /
+// dupEncodeString (huffman.h:241-296), BitStreamWriter (bit_stream.h:60-162),
+// writeQuantData (test_compress_common.cc:198-237), and the container of
+// cubelet_file.cc:18-84,259-327 with the metadata of wavelet.cc:283-343.
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <queue>
+#include <string>
+#include <vector>
+#include "../../include/wv_hip.h"
+
+namespace wvh {
+
+// run of n copies of the duplicate value: how many 3-bit "dup" codes (for
+// 2^(e+3) values, greedy, e <= 7) and how many single codes it needs
+static void run_codes(int64_t n, int64_t &dups, int64_t &singles) {
+  dups = 0;
+  while (n > 8) {
+    int e = 7;
+    while (e > 0 && (int64_t(1) << (e + 3)) > n) e--;
+    n -= int64_t(1) << (e + 3);
+    dups++;
+  }
+  singles = n;
+}
+
+struct Code {
+  int len = 0;
+  std::vector<uint32_t> words;   // LSB-first
+};
+
+class Coder {
+ public:
+  // counts: device histogram of bins (bin_count entries); dup_value < 0
+  // disables the duplicate-string code
+  Coder(const uint32_t *counts, int bin_count, int dup_value,
+        const int32_t *bins, int64_t n)
+      : nb_(bin_count), dup_(dup_value >= 0 && dup_value < bin_count ? dup_value : -1) {
+    freq_.assign(counts, counts + bin_count);
+    if (dup_ >= 0) {
+      freq_[dup_] = 0;
+      int64_t run = 0, d, s;
+      for (int64_t i = 0; i < n; i++) {
+        if (bins[i] == dup_) { run++; continue; }
+        if (run) { run_codes(run, d, s); freq_[dup_] += s; dupcount_ += d; run = 0; }
+      }
+      if (run) { run_codes(run, d, s); freq_[dup_] += s; dupcount_ += d; }
+      freq_.push_back(dupcount_);
+    }
+    build();
+  }
+
+  int key() const { return nb_; }
+  const std::vector<int32_t> &decode_table() const { return table_; }
+  const Code &code(int v) const { return codes_[v]; }
+  int longest() const { return longest_; }
+
+  // encodes n bins; returns the number of bits
+  int64_t encode(const int32_t *bins, int64_t n, std::vector<uint32_t> &out) const {
+    Writer w(out);
+    for (int64_t i = 0; i < n; i++) {
+      const int v = bins[i];
+      if (v == dup_) {
+        int64_t len = 1;
+        while (i + len < n && bins[i + len] == dup_) len++;
+        i += len - 1;
+        while (len > 8) {
+          int e = 7;
+          while (e > 0 && (int64_t(1) << (e + 3)) > len) e--;
+          len -= int64_t(1) << (e + 3);
+          w.put(codes_[nb_]);
+          w.put_bits((uint32_t)e, 3);
+        }
+        while (len-- > 0) w.put(codes_[v]);
+      } else {
+        w.put(codes_[v]);
+      }
+    }
+    return w.finish();
+  }
+
+ private:
+  struct Writer {
+    std::vector<uint32_t> &out;
+    uint64_t acc = 0;
+    int fill = 0;
+    int64_t bits = 0;
+    explicit Writer(std::vector<uint32_t> &o) : out(o) {}
+    void put_bits(uint32_t v, int k) {
+      if (k <= 0) return;
+      if (k < 32) v &= (1u << k) - 1;
+      acc |= (uint64_t)v << fill;
+      fill += k;
+      bits += k;
+      if (fill >= 32) {
+        out.push_back((uint32_t)acc);
+        acc >>= 32;
+        fill -= 32;
+      }
+    }
+    void put(const Code &c) {
+      int left = c.len;
+      for (size_t w = 0; left > 0; w++) {
+        const int k = left > 32 ? 32 : left;
+        put_bits(c.words[w], k);
+        left -= k;
+      }
+    }
+    int64_t finish() {
+      if (fill > 0) out.push_back((uint32_t)acc);
+      acc = 0;
+      fill = 0;
+      return bits;
+    }
+  };
+
+  struct Node {
+    int64_t count;
+    int left = -1, right = -1, value = -1;
+  };
+
+  void build() {
+    const int leaves = (int)freq_.size();
+    nodes_.clear();
+    nodes_.reserve(2 * leaves);
+    // the queue pops the smallest count; equal counts pop the node created
+    // later first (the reference compares node addresses in one array)
+    auto later_first = [this](int a, int b) {
+      if (nodes_[a].count != nodes_[b].count) return nodes_[a].count > nodes_[b].count;
+      return a < b;
+    };
+    std::priority_queue<int, std::vector<int>, decltype(later_first)> pq(later_first);
+    for (int i = 0; i < leaves; i++) {
+      Node nd;
+      nd.count = freq_[i];
+      nd.value = i;
+      nodes_.push_back(nd);
+      if (freq_[i] > 0) pq.push(i);
+    }
+    while (pq.size() > 1) {
+      const int a = pq.top(); pq.pop();
+      const int b = pq.top(); pq.pop();
+      Node nd;
+      nd.count = nodes_[a].count + nodes_[b].count;
+      nd.left = a;
+      nd.right = b;
+      nodes_.push_back(nd);
+      pq.push((int)nodes_.size() - 1);
+    }
+    const int root = pq.top();
+    codes_.assign(leaves, Code());
+    longest_ = 0;
+    std::vector<int> path;   // 0/1 per depth
+    assign(root, path);
+    // decode table from a breadth-first numbering (huffman.cc:177-228)
+    std::vector<int> order{root};
+    std::vector<int> id(nodes_.size(), -1);
+    for (size_t h = 0; h < order.size(); h++) {
+      const int v = order[h];
+      id[v] = (int)h;
+      if (nodes_[v].value < 0) {
+        order.push_back(nodes_[v].left);
+        order.push_back(nodes_[v].right);
+      }
+    }
+    table_.resize(2 * order.size());
+    for (size_t h = 0; h < order.size(); h++) {
+      const Node &nd = nodes_[order[h]];
+      if (nd.value >= 0) { table_[2 * h] = 0; table_[2 * h + 1] = nd.value; }
+      else { table_[2 * h] = id[nd.left]; table_[2 * h + 1] = id[nd.right]; }
+    }
+  }
+
+  void assign(int v, std::vector<int> &path) {
+    const Node &nd = nodes_[v];
+    if (nd.value >= 0) {
+      Code &c = codes_[nd.value];
+      c.len = (int)path.size();
+      c.words.assign(path.size() / 32 + 1, 0u);
+      for (size_t i = 0; i < path.size(); i++)
+        if (path[i]) c.words[i / 32] |= 1u << (i % 32);
+      longest_ = std::max(longest_, c.len);
+      return;
+    }
+    path.push_back(0);
+    assign(nd.left, path);
+    path.back() = 1;
+    assign(nd.right, path);
+    path.pop_back();
+  }
+
+  int nb_, dup_;
+  int64_t dupcount_ = 0;
+  std::vector<int64_t> freq_;
+  std::vector<Node> nodes_;
+  std::vector<Code> codes_;
+  std::vector<int32_t> table_;
+  int longest_ = 0;
+};
+
+// ---- protobuf 2 wire format (only what the .cube metadata uses) ----
+class Proto {
+ public:
+  std::string b;
+  void varint(uint64_t v) {
+    while (v >= 0x80) { b.push_back((char)(v | 0x80)); v >>= 7; }
+    b.push_back((char)v);
+  }
+  void tag(int field, int wire) { varint(((uint64_t)field << 3) | (uint64_t)wire); }
+  void u32(int field, uint32_t v) { tag(field, 0); varint(v); }
+  void i32(int field, int32_t v) { tag(field, 0); varint((uint64_t)(int64_t)v); }
+  void f32(int field, float v) { tag(field, 5); raw(&v, 4); }
+  void raw(const void *p, size_t n) { b.append((const char *)p, n); }
+  void bytes(int field, const std::string &s) { tag(field, 2); varint(s.size()); b += s; }
+};
+
+static int proto_quant_id(int alg) {   // wavelet.cc:21-35 -> proto enum
+  return alg == WV_QUANT_LOG ? 1 : alg == WV_QUANT_UNIFORM ? 2 : 3;
+}
+
+static std::string params_msg(const wv_cube_desc *d, const Coder &c,
+                              int dup_key, int dup_value) {
+  Proto p;
+  p.i32(1, d->steps[0]);
+  p.i32(2, d->steps[1]);
+  p.i32(3, d->steps[2]);
+  p.i32(4, d->wavelet);
+  p.f32(6, d->thresh_frac);
+  p.f32(7, d->threshold);
+  p.i32(8, d->bin_count);
+  p.i32(9, proto_quant_id(d->quant_alg));
+  if (d->quant_alg == WV_QUANT_LLOYD) {
+    Proto pb, pv;
+    for (int i = 0; i < d->bin_count - 1; i++) pb.raw(&d->boundaries[i], 4);
+    for (int i = 0; i < d->bin_count; i++) pv.raw(&d->values[i], 4);
+    if (!pb.b.empty()) p.bytes(11, pb.b);
+    if (!pv.b.empty()) p.bytes(12, pv.b);
+  } else {
+    p.f32(10, d->max_val);
+  }
+  Proto t;
+  for (int32_t v : c.decode_table()) t.varint((uint64_t)(int64_t)v);
+  if (!t.b.empty()) p.bytes(13, t.b);
+  p.u32(14, (uint32_t)d->orig[0]);
+  p.u32(15, (uint32_t)d->orig[1]);
+  p.u32(16, (uint32_t)d->orig[2]);
+  if (d->orig_dtype > 0) p.i32(17, d->orig_dtype);
+  p.u32(18, 0);   // transform_2d = false
+  if (dup_key >= 0) p.i32(19, dup_key);
+  if (dup_value >= 0) p.i32(20, dup_value);
+  return p.b;
+}
+
+static std::string cubelet_msg(const wv_cube_desc *d, const std::string &params,
+                               uint32_t byte_count, uint64_t data_offset) {
+  Proto m;
+  m.u32(1, (uint32_t)d->out_size[0]);
+  m.u32(2, (uint32_t)d->out_size[1]);
+  m.u32(3, (uint32_t)d->out_size[2]);
+  m.u32(4, 0);
+  m.u32(5, 0);
+  m.u32(6, 0);
+  m.u32(7, 2);                  // INT32
+  m.u32(8, byte_count);
+  m.u32(9, 2);                  // WAVELET
+  if (data_offset > 0) { m.tag(13, 0); m.varint(data_offset); }
+  m.bytes(14, params);
+  return m.b;
+}
+
+}  // namespace wvh
+
+extern "C" int64_t wv_write_cube(const wv_cube_desc *d, const int32_t *bins,
+                                 int64_t n, const uint32_t *counts,
+                                 uint8_t *file_buf, int64_t cap,
+                                 int64_t *payload_bytes) {
+  using namespace wvh;
+  if (!d || !bins || !counts || n <= 0 || d->bin_count < 1) return WV_ERR_ARG;
+  const int dup = d->zero_bin;
+  Coder coder(counts, d->bin_count, dup, bins, n);
+  std::vector<uint32_t> words;
+  words.reserve((size_t)(n / 8 + 16));
+  coder.encode(bins, n, words);
+  const uint32_t nbytes = (uint32_t)(words.size() * 4);
+  const int dup_key = dup >= 0 ? d->bin_count : -1;
+  const std::string params = params_msg(d, coder, dup_key, dup);
+  const std::string meta = cubelet_msg(d, params, nbytes, 0);
+  std::string out;
+  char header[64] = {0};
+  std::memcpy(header, "SCU cubelets 1.0\n", 17);
+  out.append(header, 64);
+  const uint32_t mlen = (uint32_t)meta.size();
+  out.append((const char *)&mlen, 4);
+  out += meta;
+  const uint64_t data_offset = out.size();
+  out.append((const char *)words.data(), nbytes);
+  if (d->with_footer) {
+    const int32_t eof = -1;
+    out.append((const char *)&eof, 4);
+    Proto index;
+    index.bytes(5, cubelet_msg(d, params, nbytes, data_offset));
+    out += index.b;
+    const int32_t ilen = (int32_t)index.b.size();
+    out.append((const char *)&ilen, 4);
+    out += "cube";
+  }
+  if (payload_bytes) *payload_bytes = nbytes;
+  if ((int64_t)out.size() > cap) return WV_ERR_NOMEM;
+  std::memcpy(file_buf, out.data(), out.size());
+  return (int64_t)out.size();
+}

@@ -1,0 +1,529 @@
+// wv_lloyd.hip -- Lloyd-Max codebook training on the device with the
+// semantics of the reference CPU path (computeLloydQuantization,
+// test_compress_cpu.cc:404-459; lloyd, lloyds.cpp:30-141):
+//
+//  * points: the |x| ranked above the threshold index, ascending, without
+//    the largest (sorted[idx+1 .. n-2]).  They are gathered as the keys
+//    > threshold plus the threshold's own duplicates ranked after idx, then
+//    sorted with an LSD radix sort (4 x 8-bit digits, stable block ranks).
+//  * every assignment (nearest codebook with fabsf and strict '<', the
+//    'between' partition scan) is computed per point in parallel, exactly;
+//  * every float sum the reference accumulates sequentially (per-cell sums
+//    and the signed distortion) is accumulated sequentially here too, in
+//    point order, by one wavefront per chain: the chains are independent,
+//    so the cells and the distortion run concurrently.  When a cell's
+//    members are not contiguous in point order (only possible with an
+//    unsorted codebook) the chain scans all points and adds its own.
+//  * the loop, its stop rule (reldist > 1e-6f) and the codebook / partition
+//    updates run on the device; the host only enqueues iterations and polls
+//    a done flag every few iterations.
+#include "wv_device.h"
+#include "wv_internal.h"
+#include <vector>
+#include <cmath>
+
+namespace wv {
+
+#define LL_SORT_ITEMS 16            // keys per thread per sort block
+#define LL_SORT_CHUNK (WV_BLOCK * LL_SORT_ITEMS)
+#define LL_MAX_CS 16383           // codebook held in LDS by the assignment
+#define LL_HARD_CAP 100000        // uncapped runs stop with an error here
+
+struct LloydState {
+  int32_t psize, cs;
+  int32_t iters, done, max_iters, nonmono_b, nonmono_n;
+  float stop;
+  float dist, reldist, pmin, pmax;
+  uint32_t n_gt;                    // keys > threshold gathered
+  int32_t pad[3];
+};
+
+// ---------------------------------------------------------------- gather
+__global__ void __launch_bounds__(WV_BLOCK)
+ll_gather(const float *coef, int64_t n, uint32_t tkey, uint32_t *out,
+          LloydState *S) {
+  const int lane = threadIdx.x & 63;
+  const int64_t stride = (int64_t)gridDim.x * WV_BLOCK;
+  const int64_t nround = (n + stride - 1) / stride;
+  for (int64_t r = 0; r < nround; r++) {
+    const int64_t i = r * stride + (int64_t)blockIdx.x * WV_BLOCK + threadIdx.x;
+    uint32_t key = 0;
+    bool hit = false;
+    if (i < n) {
+      key = __builtin_bit_cast(uint32_t, coef[i]) & 0x7fffffffu;
+      hit = key > tkey;
+    }
+    const uint64_t m = __ballot(hit);
+    if (m) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(&S->n_gt, (uint32_t)__popcll(m));
+      base = __shfl(base, 0);
+      if (hit) out[base + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = key;
+    }
+  }
+}
+
+// ---------------------------------------------------------------- sort
+// per-block digit histograms, block-major [block][256]
+__global__ void __launch_bounds__(WV_BLOCK)
+ll_sort_hist(const uint32_t *keys, const LloydState *S, int shift,
+             uint32_t *hist) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t n = S->n_gt;
+  const uint32_t base = blockIdx.x * LL_SORT_CHUNK;
+  for (int k = 0; k < LL_SORT_ITEMS; k++) {
+    const uint32_t i = base + k * WV_BLOCK + threadIdx.x;
+    if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255], 1u);
+  }
+  __syncthreads();
+  hist[blockIdx.x * 256 + threadIdx.x] = h[threadIdx.x];
+}
+
+// exclusive offsets: off[b][d] = sum_{d'<d} total[d'] + sum_{b'<b} hist[b'][d]
+__global__ void __launch_bounds__(256)
+ll_sort_scan(uint32_t *hist, int nblocks) {
+  __shared__ uint32_t tot[256];
+  const int d = threadIdx.x;
+  uint32_t run = 0;
+  for (int b = 0; b < nblocks; b++) run += hist[b * 256 + d];
+  tot[d] = run;
+  __syncthreads();
+  // exclusive scan over digits
+  if (d == 0) {
+    uint32_t acc = 0;
+    for (int k = 0; k < 256; k++) { uint32_t t = tot[k]; tot[k] = acc; acc += t; }
+  }
+  __syncthreads();
+  uint32_t acc = tot[d];
+  int b = 0;
+  for (; b + 8 <= nblocks; b += 8) {
+    uint32_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = hist[(b + k) * 256 + d];
+#pragma unroll
+    for (int k = 0; k < 8; k++) { hist[(b + k) * 256 + d] = acc; acc += v[k]; }
+  }
+  for (; b < nblocks; b++) {
+    const uint32_t v = hist[b * 256 + d];
+    hist[b * 256 + d] = acc;
+    acc += v;
+  }
+}
+
+// stable scatter: keys of a block are ranked in index order
+__global__ void __launch_bounds__(WV_BLOCK)
+ll_sort_scatter(const uint32_t *keys, uint32_t *out, const LloydState *S,
+                int shift, const uint32_t *offs) {
+  __shared__ uint32_t base[256];
+  __shared__ uint32_t wave_cnt[WV_BLOCK / 64][256];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  base[tid] = offs[blockIdx.x * 256 + tid];
+  const uint32_t n = S->n_gt;
+  const uint32_t start = blockIdx.x * LL_SORT_CHUNK;
+  __syncthreads();
+  for (int k = 0; k < LL_SORT_ITEMS; k++) {
+    const uint32_t i = start + k * WV_BLOCK + tid;
+    const bool valid = i < n;
+    const uint32_t key = valid ? keys[i] : 0;
+    const uint32_t dig = (key >> shift) & 255;
+    // lanes of this wave holding the same digit
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int bit = 0; bit < 8; bit++) {
+      const uint64_t b = __ballot((dig >> bit) & 1);
+      peers &= ((dig >> bit) & 1) ? b : ~b;
+    }
+    const uint32_t rank = (uint32_t)__popcll(peers & ((1ull << lane) - 1));
+    for (int j = lane; j < 256; j += 64) wave_cnt[wave][j] = 0;
+    __syncthreads();
+    if (valid && rank == 0) wave_cnt[wave][dig] = (uint32_t)__popcll(peers);
+    __syncthreads();
+    if (valid) {
+      uint32_t before = 0;
+      for (int w = 0; w < wave; w++) before += wave_cnt[w][dig];
+      out[base[dig] + before + rank] = key;
+    }
+    __syncthreads();
+    // advance the per-digit bases by this round's totals
+    {
+      uint32_t t = 0;
+      for (int w = 0; w < WV_BLOCK / 64; w++) t += wave_cnt[w][tid];
+      base[tid] += t;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- Lloyd
+// nearest codebook entry: first minimum of fabsf(p - c[j]) (lloyds.cpp:50-60)
+__device__ inline int nearest(const float *cb, int cs, float p) {
+  float mn = fabsf(p - cb[0]);
+  int idx = 0;
+  for (int j = 1; j < cs; j++) {
+    const float d = fabsf(p - cb[j]);
+    if (d < mn) { idx = j; mn = d; }
+  }
+  return idx;
+}
+
+// first j with c[j] <= p < c[j+1], or -1 (lloyds.cpp:90-99)
+__device__ inline int between(const float *cb, int cs, float p) {
+  for (int j = 0; j < cs - 1; j++)
+    if (p >= cb[j] && p < cb[j + 1]) return j;
+  return -1;
+}
+
+// assignment of every point; records the first index of every group run
+// and flags a group that appears in more than one run
+template <bool NEAREST>
+__global__ void __launch_bounds__(WV_BLOCK)
+ll_assign(const float *pts, const float *cb_g, LloydState *S, int32_t *grp,
+          int32_t *first, int32_t *last) {
+  extern __shared__ float cb[];
+  if (S->done) return;
+  const int cs = S->cs;
+  for (int j = threadIdx.x; j < cs; j += WV_BLOCK) cb[j] = cb_g[j];
+  __syncthreads();
+  const int psize = S->psize;
+  for (int i = blockIdx.x * WV_BLOCK + threadIdx.x; i < psize; i += gridDim.x * WV_BLOCK)
+    grp[i] = NEAREST ? nearest(cb, cs, pts[i]) : between(cb, cs, pts[i]);
+}
+
+// run boundaries: a group whose members are contiguous has exactly one run
+__global__ void __launch_bounds__(WV_BLOCK)
+ll_runs(LloydState *S, const int32_t *grp, int32_t *first, int32_t *last,
+        int32_t *nruns) {
+  if (S->done) return;
+  const int psize = S->psize;
+  for (int i = blockIdx.x * WV_BLOCK + threadIdx.x; i < psize; i += gridDim.x * WV_BLOCK) {
+    const int g = grp[i];
+    if (g < 0) continue;
+    if (i == 0 || grp[i - 1] != g) { first[g] = i; atomicAdd(&nruns[g], 1); }
+    if (i == psize - 1 || grp[i + 1] != g) last[g] = i;
+  }
+}
+
+__global__ void ll_clear_runs(LloydState *S, int32_t *nruns, int32_t *first,
+                              int32_t *last) {
+  if (S->done) return;
+  for (int j = threadIdx.x; j < S->cs; j += blockDim.x) {
+    nruns[j] = 0;
+    first[j] = 0;
+    last[j] = -1;
+  }
+}
+
+// one wavefront per chain: chains 0..cs-1 are the cells, chain cs the
+// signed distortion (only when DIST).  Sequential float order, lane-wise
+// loads of 64 points then 64 dependent adds in lane order.
+template <bool DIST>
+__global__ void __launch_bounds__(64)
+ll_chains(const float *pts, const int32_t *grp, const float *cb,
+          LloydState *S, const int32_t *first, const int32_t *last,
+          const int32_t *nruns, float *sum, uint32_t *count, float *dist_out) {
+  if (S->done) return;
+  const int chain = blockIdx.x;
+  const int cs = S->cs;
+  const int lane = threadIdx.x;
+  const int psize = S->psize;
+  if (chain < cs) {
+    const int nr = nruns[chain];
+    float s = 0.f;
+    uint32_t c = 0;
+    if (nr == 1) {
+      const int a = first[chain], b = last[chain] + 1;
+      c = (uint32_t)(b - a);
+      for (int base = a; base < b; base += 64) {
+        const int i = base + lane;
+        const float v = i < b ? pts[i] : 0.f;
+        const int m = min(64, b - base);
+        for (int k = 0; k < m; k++) s += __shfl(v, k);
+      }
+    } else if (nr > 1) {
+      for (int base = 0; base < psize; base += 64) {
+        const int i = base + lane;
+        const bool mine = i < psize && grp[i] == chain;
+        const float v = mine ? pts[i] : 0.f;
+        uint64_t m = __ballot(mine);
+        c += (uint32_t)__popcll(m);
+        while (m) {
+          const int k = __ffsll((unsigned long long)m) - 1;
+          s += __shfl(v, k);
+          m &= m - 1;
+        }
+      }
+    }
+    if (lane == 0) { sum[chain] = s; count[chain] = c; }
+  } else if (DIST) {
+    // dist += codebook[idx] - p over all points in order (lloyds.cpp:64,132)
+    float d = 0.f;
+    for (int base = 0; base < psize; base += 64) {
+      const int i = base + lane;
+      const float v = i < psize ? cb[grp[i]] - pts[i] : 0.f;
+      const int m = min(64, psize - base);
+      for (int k = 0; k < m; k++) d += __shfl(v, k);
+    }
+    if (lane == 0) *dist_out = d;
+  }
+}
+
+// before the loop: partition from the initial codebook, dist, reldist
+__global__ void ll_init_finish(LloydState *S, const float *cb, float *part,
+                               const float *dsum, const float *pts) {
+  const int cs = S->cs;
+  for (int i = threadIdx.x; i < cs - 1; i += blockDim.x)
+    part[i] = (cb[i] + cb[i + 1]) / 2;
+  if (threadIdx.x == 0) {
+    // pmax / pmin by sequential comparisons equal the ends of the sorted
+    // points (lloyds.cpp:40-49)
+    S->pmin = pts[0];
+    S->pmax = pts[S->psize - 1];
+    float dist = *dsum / (float)(unsigned)S->psize;
+    S->dist = dist;
+    S->reldist = fabsf(dist);
+    S->iters = 0;
+    S->done = !(S->reldist > S->stop) || (S->max_iters >= 0 && S->max_iters == 0);
+  }
+}
+
+// loop head + codebook update (lloyds.cpp:69-87)
+__global__ void ll_update_codebook(LloydState *S, float *cb, const float *part,
+                                   const float *sum, const uint32_t *count) {
+  if (S->done) return;
+  const int cs = S->cs;
+  __syncthreads();
+  for (int i = threadIdx.x; i < cs; i += blockDim.x) {
+    if (count[i] != 0) cb[i] = sum[i] / (float)count[i];
+    else if (i == 0) cb[i] = (part[0] + S->pmin) / 2;
+    else if (i == cs - 1) cb[i] = (part[i - 1] + S->pmax) / 2;
+    else cb[i] = (part[i - 1] + part[i]) / 2;
+  }
+}
+
+// partition update (lloyds.cpp:102-111)
+__global__ void ll_update_partition(LloydState *S, const float *cb, float *part,
+                                    const float *sum, const uint32_t *count) {
+  if (S->done) return;
+  const int cs = S->cs;
+  for (int i = threadIdx.x; i < cs - 1; i += blockDim.x) {
+    if (count[i] != 0) part[i] = sum[i] / (float)count[i];
+    else part[i] = (cb[i] + cb[i + 1]) / 2;
+  }
+}
+
+// loop tail: reldist = |dist_old - dist_new|, then the loop test
+__global__ void ll_finish_iter(LloydState *S, const float *dsum) {
+  if (S->done) return;
+  const float prev = S->dist;
+  const float dist = *dsum / (float)(unsigned)S->psize;
+  S->dist = dist;
+  S->reldist = fabsf(prev - dist);
+  S->iters++;
+  S->done = !(S->reldist > S->stop) || (S->max_iters >= 0 && S->iters >= S->max_iters);
+}
+
+__global__ void ll_fill_dups(uint32_t *tail, uint32_t ndup, uint32_t tkey) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ndup;
+       i += gridDim.x * blockDim.x)
+    tail[i] = tkey;
+}
+
+__global__ void ll_state_init(LloydState *S, int psize, int cs, int max_iters) {
+  S->psize = psize;
+  S->cs = cs;
+  S->iters = 0;
+  S->done = 0;
+  S->max_iters = max_iters;
+  S->stop = (float)10e-7;    // DEFAULT_LLOYD_STOP_CRITERIA, lloyds.h:1
+  S->n_gt = 0;
+}
+
+}  // namespace wv
+
+using namespace wv;
+
+namespace {
+inline int64_t a256(int64_t v) { return (v + 255) & ~(int64_t)255; }
+struct LWs {
+  int64_t state, keys, keys2, hist, grp, cb, part, sum, count, first, last, nruns, dsum, total;
+};
+LWs lloyd_layout(int64_t n, int bins) {
+  LWs L{};
+  const int cs = (bins - 1) / 2;
+  const int64_t nblk = (n + LL_SORT_CHUNK - 1) / LL_SORT_CHUNK + 1;
+  int64_t o = 0;
+  L.state = o; o += a256(sizeof(LloydState));
+  L.keys = o; o += a256(n * 4);
+  L.keys2 = o; o += a256(n * 4);
+  L.hist = o; o += a256(nblk * 256 * 4);
+  L.grp = o; o += a256(n * 4);
+  L.cb = o; o += a256((int64_t)cs * 4 + 4);
+  L.part = o; o += a256((int64_t)cs * 4 + 4);
+  L.sum = o; o += a256((int64_t)cs * 4 + 4);
+  L.count = o; o += a256((int64_t)cs * 4 + 4);
+  L.first = o; o += a256((int64_t)cs * 4 + 4);
+  L.last = o; o += a256((int64_t)cs * 4 + 4);
+  L.nruns = o; o += a256((int64_t)cs * 4 + 4);
+  L.dsum = o; o += a256(16);
+  L.total = o;
+  return L;
+}
+}  // namespace
+
+extern "C" int64_t wv_lloyd_ws_bytes(int64_t n, int bin_count) {
+  return lloyd_layout(n, bin_count).total;
+}
+
+#define LCHK(expr, what)                                          \
+  do {                                                            \
+    hipError_t _e = (expr);                                       \
+    if (_e != hipSuccess) {                                       \
+      set_error("lloyd %s: %s", what, hipGetErrorString(_e));     \
+      return WV_ERR_HIP;                                          \
+    }                                                             \
+  } while (0)
+
+// initialLloydCodebook (test_compress_common.cc:749-779) with the host libm
+static void initial_codebook(float *cb, int cs, float min_abs, float max_abs) {
+  const float log_scale = logf(max_abs / min_abs) / cs;
+  for (int i = 0; i < cs; i++) cb[i] = min_abs * expf(i * log_scale);
+}
+
+// setBinsFromCodebook (test_compress_common.cc:784-846)
+static int bins_from_codebook(float *v, float *b, int bins, const float *c, int cs,
+                              float thr, float mn, float mx) {
+  int nv = 0, nb = 0;
+  if ((bins & 1) == 0 && fabsf(mn) > mx) { v[nv++] = mn; b[nb++] = (mn + -c[cs - 1]) / 2; }
+  v[nv++] = -c[cs - 1];
+  for (int i = cs - 2; i >= 0; i--) { b[nb++] = (-c[i] + -c[i + 1]) / 2; v[nv++] = -c[i]; }
+  b[nb++] = -thr;
+  v[nv++] = 0;
+  b[nb++] = thr;
+  for (int i = 0; i < cs - 1; i++) { v[nv++] = c[i]; b[nb++] = (c[i] + c[i + 1]) / 2; }
+  v[nv++] = c[cs - 1];
+  if ((bins & 1) == 0 && mx >= fabsf(mn)) { v[nv++] = mx; b[nb++] = (mx + c[cs - 1]) / 2; }
+  return (nv == bins && nb == bins - 1) ? WV_OK : WV_ERR_STATE;
+}
+
+extern "C" int wv_lloyd(const float *coef_d, int64_t n, const wv_thresh_result *res,
+                        int bin_count, int max_iters, void *ws_d,
+                        float *bin_boundaries, float *bin_values, int *iters_out,
+                        void *stream) {
+  hipStream_t s = (hipStream_t)stream;
+  if (!coef_d || !res || !ws_d || !bin_boundaries || !bin_values || n <= 0) {
+    set_error("bad arguments");
+    return WV_ERR_ARG;
+  }
+  const int cs = (bin_count - 1) / 2;
+  const int64_t nz = res->nonzero_count;
+  const int64_t psize = nz - 1;
+  if (cs < 2 || cs > LL_MAX_CS || psize < 2) {
+    set_error("lloyd: %lld points / %d codebook entries unsupported", (long long)psize, cs);
+    return WV_ERR_UNSUPPORTED;
+  }
+  const LWs L = lloyd_layout(n, bin_count);
+  char *w = (char *)ws_d;
+  LloydState *S = (LloydState *)(w + L.state);
+  uint32_t *keys = (uint32_t *)(w + L.keys), *keys2 = (uint32_t *)(w + L.keys2);
+  uint32_t *hist = (uint32_t *)(w + L.hist);
+  int32_t *grp = (int32_t *)(w + L.grp);
+  float *cb = (float *)(w + L.cb), *part = (float *)(w + L.part), *sum = (float *)(w + L.sum);
+  uint32_t *count = (uint32_t *)(w + L.count);
+  int32_t *first = (int32_t *)(w + L.first), *last = (int32_t *)(w + L.last),
+          *nruns = (int32_t *)(w + L.nruns);
+  float *dsum = (float *)(w + L.dsum);
+
+  hipLaunchKernelGGL(ll_state_init, dim3(1), dim3(1), 0, s, S, (int)psize, cs, max_iters);
+  // 1. tail = threshold duplicates ranked after idx, then sorted keys > T
+  int64_t gb = (n + WV_BLOCK * 8 - 1) / (WV_BLOCK * 8);
+  if (gb > 8192) gb = 8192;
+  hipLaunchKernelGGL(ll_gather, dim3((unsigned)gb), dim3(WV_BLOCK), 0, s, coef_d, n,
+                     res->threshold_key, keys, S);
+  LloydState hs;
+  LCHK(hipMemcpyAsync(&hs, S, sizeof hs, hipMemcpyDeviceToHost, s), "D2H");
+  LCHK(hipStreamSynchronize(s), "sync");
+  const int64_t ngt = hs.n_gt;
+  const int64_t ndup = nz - ngt;
+  if (ndup < 0 || ndup > n) { set_error("lloyd: inconsistent threshold result"); return WV_ERR_STATE; }
+  const int nblocks = (int)((ngt + LL_SORT_CHUNK - 1) / LL_SORT_CHUNK);
+  uint32_t *src = keys, *dst = keys2;
+  for (int pass = 0; pass < 4 && nblocks > 0; pass++) {
+    const int shift = pass * 8;
+    hipLaunchKernelGGL(ll_sort_hist, dim3(nblocks), dim3(WV_BLOCK), 0, s, src, S, shift, hist);
+    hipLaunchKernelGGL(ll_sort_scan, dim3(1), dim3(256), 0, s, hist, nblocks);
+    hipLaunchKernelGGL(ll_sort_scatter, dim3(nblocks), dim3(WV_BLOCK), 0, s, src, dst, S,
+                       shift, hist);
+    std::swap(src, dst);
+  }
+  // after 4 passes the sorted keys are in 'src' (== keys); lay out the tail
+  // [dups..., sorted...] in keys2
+  uint32_t *tail = keys2;
+  if (ndup > 0)
+    hipLaunchKernelGGL(ll_fill_dups, dim3(64), dim3(256), 0, s, tail, (uint32_t)ndup,
+                       res->threshold_key);
+  if (ngt > 0)
+    LCHK(hipMemcpyAsync(tail + ndup, src, ngt * 4, hipMemcpyDeviceToDevice, s), "tail copy");
+  const float *pts = (const float *)tail;   // points = tail[0 .. nz-2]
+
+  // 2. initial codebook on the host: minAbs = threshold, maxAbs = tail[nz-2]
+  float max2;
+  LCHK(hipMemcpyAsync(&max2, tail + (nz - 2), 4, hipMemcpyDeviceToHost, s), "D2H max");
+  LCHK(hipStreamSynchronize(s), "sync");
+  std::vector<float> hcb(cs);
+  initial_codebook(hcb.data(), cs, res->threshold, max2);
+  LCHK(hipMemcpyAsync(cb, hcb.data(), cs * 4, hipMemcpyHostToDevice, s), "H2D codebook");
+
+  const unsigned ag = (unsigned)std::min<int64_t>((psize + WV_BLOCK - 1) / WV_BLOCK, 4096);
+  const size_t cb_lds = (size_t)cs * 4;
+  auto assign_and_chains = [&](bool nearest_pass) {
+    if (nearest_pass)
+      hipLaunchKernelGGL(ll_assign<true>, dim3(ag), dim3(WV_BLOCK), cb_lds, s, pts, cb, S, grp,
+                         first, last);
+    else
+      hipLaunchKernelGGL(ll_assign<false>, dim3(ag), dim3(WV_BLOCK), cb_lds, s, pts, cb, S, grp,
+                         first, last);
+    hipLaunchKernelGGL(ll_clear_runs, dim3(1), dim3(256), 0, s, S, nruns, first, last);
+    hipLaunchKernelGGL(ll_runs, dim3(ag), dim3(WV_BLOCK), 0, s, S, grp, first, last, nruns);
+    if (nearest_pass)
+      hipLaunchKernelGGL(ll_chains<true>, dim3(cs + 1), dim3(64), 0, s, pts, grp, cb, S, first,
+                         last, nruns, sum, count, dsum);
+    else
+      hipLaunchKernelGGL(ll_chains<false>, dim3(cs), dim3(64), 0, s, pts, grp, cb, S, first,
+                         last, nruns, sum, count, dsum);
+  };
+
+  // 3. initial assignment (lloyds.cpp:35-67)
+  assign_and_chains(true);
+  hipLaunchKernelGGL(ll_init_finish, dim3(1), dim3(256), 0, s, S, cb, part, dsum, pts);
+
+  // 4. iterations, polled every few
+  const int poll = 4;
+  int enqueued = 0;
+  for (;;) {
+    for (int k = 0; k < poll; k++) {
+      hipLaunchKernelGGL(ll_update_codebook, dim3(1), dim3(256), 0, s, S, cb, part, sum, count);
+      assign_and_chains(false);
+      hipLaunchKernelGGL(ll_update_partition, dim3(1), dim3(256), 0, s, S, cb, part, sum, count);
+      assign_and_chains(true);
+      hipLaunchKernelGGL(ll_finish_iter, dim3(1), dim3(1), 0, s, S, dsum);
+      enqueued++;
+    }
+    LCHK(hipGetLastError(), "launch");
+    LCHK(hipMemcpyAsync(&hs, S, sizeof hs, hipMemcpyDeviceToHost, s), "D2H state");
+    LCHK(hipStreamSynchronize(s), "sync");
+    if (hs.done) break;
+    if (max_iters >= 0 && enqueued >= max_iters) break;
+    if (enqueued >= LL_HARD_CAP) {
+      set_error("lloyd: no convergence after %d iterations", enqueued);
+      return WV_ERR_STATE;
+    }
+  }
+  std::vector<float> fin(cs);
+  LCHK(hipMemcpyAsync(fin.data(), cb, cs * 4, hipMemcpyDeviceToHost, s), "D2H codebook");
+  LCHK(hipStreamSynchronize(s), "sync");
+  if (iters_out) *iters_out = hs.iters;
+  return bins_from_codebook(bin_values, bin_boundaries, bin_count, fin.data(), cs,
+                            res->threshold, res->min_val, res->max_val);
+}

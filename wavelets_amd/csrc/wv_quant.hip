@@ -1,0 +1,278 @@
+// wv_quant.hip -- quantization fused with the bin histogram, and
+// dequantization, for gfx950.
+//
+// One pass reads the coefficients (16 B vector loads), writes int32 bins
+// (16 B vector stores) and accumulates the bin histogram in LDS (the zero
+// bin, which holds most values after thresholding, is counted in a register
+// per thread), so the stage moves 8 B per coefficient.
+//
+//  uniform : the float expressions of QuantUniform::quant (quant.h:244-276),
+//            contraction off, so bins equal the host's;
+//  log     : bins are assigned by comparing |x| with the host-tabulated bin
+//            edges (the smallest float of every bin under the host logf), a
+//            binary search in LDS; this reproduces QuantLog::quant
+//            (quant.h:374-406) exactly without evaluating logf on the device;
+//  codebook: lower_bound over the boundaries plus the lastBoundary rule of
+//            the host QuantCodebook::quant (quant.h:515-526).
+#include "wv_device.h"
+#include "wv_internal.h"
+
+namespace wv {
+
+#define WV_LDS_TABLE 4096    // floats of edge / boundary table kept in LDS
+#define WV_LDS_HIST 4096     // bins counted in LDS (more: global atomics)
+
+// number of entries of e[0..m) that are <= x  (e ascending)
+__device__ inline int count_le(const float *e, int m, float x) {
+  int lo = 0, len = m;
+  while (len > 0) {
+    const int half = len >> 1;
+    if (e[lo + half] <= x) { lo += half + 1; len -= half + 1; }
+    else len = half;
+  }
+  return lo;
+}
+
+// first index with e[i] >= x (std::lower_bound)
+__device__ inline int lower_bound_f(const float *e, int m, float x) {
+  int lo = 0, len = m;
+  while (len > 0) {
+    const int half = len >> 1;
+    if (e[lo + half] < x) { lo += half + 1; len -= half + 1; }
+    else len = half;
+  }
+  return lo;
+}
+
+template <int ALG>
+__device__ inline int quant_one(const QuantDev &q, const float *tab, float x) {
+  const int zb = q.zero_bin;
+  if (ALG == WV_QUANT_UNIFORM) {
+    if (x < 0) {
+      if (x >= -q.threshold) return zb;
+      x = x * q.neg_scale - q.neg_offset;
+      const int bin = (int)x + zb;
+      return bin < 0 ? 0 : bin;
+    } else {
+      if (x <= q.threshold) return zb;
+      x = x * q.pos_scale + q.pos_offset + (float)zb;
+      if (x > (float)(q.bin_count - 1)) return q.bin_count - 1;
+      return (int)x;
+    }
+  } else if (ALG == WV_QUANT_LOG) {
+    if (x < 0) {
+      const float a = -x;
+      if (a <= q.threshold) return zb;
+      return zb - count_le(tab + q.n_pos, q.n_neg, a);
+    } else {
+      if (x <= q.threshold) return zb;
+      if (x >= q.pos_clamp) return q.bin_count - 1;
+      return zb + count_le(tab, q.n_pos, x);
+    }
+  } else {
+    const int m = q.n_pos;   // boundaries
+    if (x >= tab[m - 1]) return m;
+    return lower_bound_f(tab, m, x);
+  }
+}
+
+template <int ALG>
+__global__ void __launch_bounds__(WV_BLOCK)
+quant_hist_kernel(const float *coef, int64_t n, QuantDev q,
+                  int32_t *bins, uint32_t *__restrict__ counts) {
+  __shared__ float tab[WV_LDS_TABLE];
+  __shared__ uint32_t h[WV_LDS_HIST];
+  const int ntab = q.n_pos + (ALG == WV_QUANT_LOG ? q.n_neg : 0);
+  const bool tab_in_lds = ntab <= WV_LDS_TABLE;
+  if (ALG != WV_QUANT_UNIFORM && tab_in_lds)
+    for (int i = threadIdx.x; i < ntab; i += WV_BLOCK) tab[i] = q.table[i];
+  const bool hist_in_lds = q.bin_count <= WV_LDS_HIST;
+  if (hist_in_lds)
+    for (int i = threadIdx.x; i < q.bin_count; i += WV_BLOCK) h[i] = 0;
+  __syncthreads();
+  const float *t = tab_in_lds ? tab : q.table;
+  unsigned zcount = 0;
+  const int zb = q.zero_bin;
+
+  // counts[bin_count] collects out-of-range bins (the log quantizer's bin
+  // B above the max it was built with, quant.h:398-403)
+  auto count = [&](int b) {
+    if (b == zb) zcount++;
+    else if (b >= q.bin_count) atomicAdd(&counts[q.bin_count], 1u);
+    else if (hist_in_lds) atomicAdd(&h[b], 1u);
+    else atomicAdd(&counts[b], 1u);
+  };
+
+  const bool aligned = (((uintptr_t)coef | (uintptr_t)bins) & 15) == 0;
+  const int64_t nvec = aligned ? (n >> 2) : 0;
+  const float4 *c4 = (const float4 *)coef;
+  int4 *b4 = (int4 *)bins;
+  for (int64_t i = (int64_t)blockIdx.x * WV_BLOCK + threadIdx.x; i < nvec;
+       i += (int64_t)gridDim.x * WV_BLOCK) {
+    const float4 v = c4[i];
+    int4 r;
+    r.x = quant_one<ALG>(q, t, v.x);
+    r.y = quant_one<ALG>(q, t, v.y);
+    r.z = quant_one<ALG>(q, t, v.z);
+    r.w = quant_one<ALG>(q, t, v.w);
+    b4[i] = r;
+    count(r.x); count(r.y); count(r.z); count(r.w);
+  }
+  // tail
+  for (int64_t i = nvec * 4 + (int64_t)blockIdx.x * WV_BLOCK + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * WV_BLOCK) {
+    const int r = quant_one<ALG>(q, t, coef[i]);
+    bins[i] = r;
+    count(r);
+  }
+  for (int off = 32; off > 0; off >>= 1)
+    zcount += (unsigned)__shfl_xor((int)zcount, off);
+  if ((threadIdx.x & 63) == 0 && zcount) {
+    if (hist_in_lds) atomicAdd(&h[zb], zcount);
+    else atomicAdd(&counts[zb], zcount);
+  }
+  __syncthreads();
+  if (hist_in_lds)
+    for (int i = threadIdx.x; i < q.bin_count; i += WV_BLOCK)
+      if (h[i]) atomicAdd(&counts[i], h[i]);
+}
+
+__global__ void zero_u32(uint32_t *p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    p[i] = 0;
+}
+
+static unsigned grid_for(int64_t n) {
+  int64_t b = (n / 4 + WV_BLOCK * 4 - 1) / (WV_BLOCK * 4);
+  if (b > 4096) b = 4096;
+  if (b < 1) b = 1;
+  return (unsigned)b;
+}
+
+hipError_t quantize_hist(const float *coef, int64_t n, const QuantDev &q,
+                         int32_t *bins, uint32_t *counts, hipStream_t s) {
+  hipLaunchKernelGGL(zero_u32, dim3((q.bin_count + 256) / 256), dim3(256), 0, s,
+                     counts, q.bin_count + 1);
+  const unsigned g = grid_for(n);
+  if (q.alg == WV_QUANT_UNIFORM)
+    hipLaunchKernelGGL(quant_hist_kernel<WV_QUANT_UNIFORM>, dim3(g), dim3(WV_BLOCK), 0, s,
+                       coef, n, q, bins, counts);
+  else if (q.alg == WV_QUANT_LOG)
+    hipLaunchKernelGGL(quant_hist_kernel<WV_QUANT_LOG>, dim3(g), dim3(WV_BLOCK), 0, s,
+                       coef, n, q, bins, counts);
+  else
+    hipLaunchKernelGGL(quant_hist_kernel<WV_QUANT_LLOYD>, dim3(g), dim3(WV_BLOCK), 0, s,
+                       coef, n, q, bins, counts);
+  return hipGetLastError();
+}
+
+// Uniform parameters from the device-resident threshold result: the same
+// float expressions as QuantUniform::init (quant.h:227-239) evaluated by one
+// thread, then the fused pass.
+__global__ void uniform_params_kernel(const wv_thresh_result *res, int bins,
+                                      QuantDev *q) {
+  const float threshold = res->threshold, maxv = res->max_abs;
+  const int negBins = (bins - 1) / 2, posBins = bins / 2;
+  QuantDev o;
+  o.alg = WV_QUANT_UNIFORM;
+  o.bin_count = bins;
+  o.zero_bin = (bins - 1) >> 1;
+  o.threshold = threshold;
+  o.neg_scale = (float)(negBins - 1) / (maxv - threshold);
+  o.pos_scale = (float)(posBins - 1) / (maxv - threshold);
+  o.neg_offset = 1 - threshold * o.neg_scale;
+  o.pos_offset = 1 - threshold * o.pos_scale;
+  o.n_pos = o.n_neg = 0;
+  o.table = nullptr;
+  *q = o;
+}
+
+__global__ void __launch_bounds__(WV_BLOCK)
+quant_hist_uniform_dev_kernel(const float *coef, int64_t n,
+                              const QuantDev *qp, int32_t *bins,
+                              uint32_t *__restrict__ counts) {
+  __shared__ uint32_t h[WV_LDS_HIST];
+  const QuantDev q = *qp;
+  const bool hist_in_lds = q.bin_count <= WV_LDS_HIST;
+  if (hist_in_lds)
+    for (int i = threadIdx.x; i < q.bin_count; i += WV_BLOCK) h[i] = 0;
+  __syncthreads();
+  unsigned zcount = 0;
+  const int zb = q.zero_bin;
+  // counts[bin_count] collects out-of-range bins (the log quantizer's bin
+  // B above the max it was built with, quant.h:398-403)
+  auto count = [&](int b) {
+    if (b == zb) zcount++;
+    else if (b >= q.bin_count) atomicAdd(&counts[q.bin_count], 1u);
+    else if (hist_in_lds) atomicAdd(&h[b], 1u);
+    else atomicAdd(&counts[b], 1u);
+  };
+  const bool aligned = (((uintptr_t)coef | (uintptr_t)bins) & 15) == 0;
+  const int64_t nvec = aligned ? (n >> 2) : 0;
+  const float4 *c4 = (const float4 *)coef;
+  int4 *b4 = (int4 *)bins;
+  for (int64_t i = (int64_t)blockIdx.x * WV_BLOCK + threadIdx.x; i < nvec;
+       i += (int64_t)gridDim.x * WV_BLOCK) {
+    const float4 v = c4[i];
+    int4 r;
+    r.x = quant_one<WV_QUANT_UNIFORM>(q, nullptr, v.x);
+    r.y = quant_one<WV_QUANT_UNIFORM>(q, nullptr, v.y);
+    r.z = quant_one<WV_QUANT_UNIFORM>(q, nullptr, v.z);
+    r.w = quant_one<WV_QUANT_UNIFORM>(q, nullptr, v.w);
+    b4[i] = r;
+    count(r.x); count(r.y); count(r.z); count(r.w);
+  }
+  for (int64_t i = nvec * 4 + (int64_t)blockIdx.x * WV_BLOCK + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * WV_BLOCK) {
+    const int r = quant_one<WV_QUANT_UNIFORM>(q, nullptr, coef[i]);
+    bins[i] = r;
+    count(r);
+  }
+  for (int off = 32; off > 0; off >>= 1)
+    zcount += (unsigned)__shfl_xor((int)zcount, off);
+  if ((threadIdx.x & 63) == 0 && zcount) {
+    if (hist_in_lds) atomicAdd(&h[zb], zcount);
+    else atomicAdd(&counts[zb], zcount);
+  }
+  __syncthreads();
+  if (hist_in_lds)
+    for (int i = threadIdx.x; i < q.bin_count; i += WV_BLOCK)
+      if (h[i]) atomicAdd(&counts[i], h[i]);
+}
+
+hipError_t quantize_hist_uniform_dev(const float *coef, int64_t n,
+                                     int bin_count,
+                                     const wv_thresh_result *res,
+                                     int32_t *bins, uint32_t *counts,
+                                     hipStream_t s) {
+  // the parameter block lives in the tail of the counts buffer region the
+  // caller provides: counts has bin_count entries plus 64 spare words
+  QuantDev *qp = (QuantDev *)(counts + ((bin_count + 63) & ~63));
+  hipLaunchKernelGGL(zero_u32, dim3((bin_count + 256) / 256), dim3(256), 0, s,
+                     counts, bin_count + 1);
+  hipLaunchKernelGGL(uniform_params_kernel, dim3(1), dim3(1), 0, s, res,
+                     bin_count, qp);
+  hipLaunchKernelGGL(quant_hist_uniform_dev_kernel, dim3(grid_for(n)), dim3(WV_BLOCK),
+                     0, s, coef, n, qp, bins, counts);
+  return hipGetLastError();
+}
+
+// dequantization through a host-built table of the bin values
+__global__ void dequant_kernel(const int32_t *bins, int64_t n,
+                               const float *__restrict__ table, int nb,
+                               float *__restrict__ out) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int b = bins[i];
+    out[i] = (b >= 0 && b < nb) ? table[b] : 0.f;
+  }
+}
+
+hipError_t dequantize_table(const int32_t *bins, int64_t n, const float *table,
+                            int bin_count, float *out, hipStream_t s) {
+  hipLaunchKernelGGL(dequant_kernel, dim3(grid_for(n * 4)), dim3(WV_BLOCK), 0, s,
+                     bins, n, table, bin_count, out);
+  return hipGetLastError();
+}
+
+}  // namespace wv
