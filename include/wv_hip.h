@@ -43,6 +43,10 @@ extern "C" {
 #define WV_DTYPE_UINT8 1
 #define WV_DTYPE_INT32 2
 #define WV_DTYPE_FLOAT32 3
+/* 16-bit signed input, read as such by the first pass and stored as an INT32
+   cubelet (the format has no int16; shortImagesToCubelets.cc:9-27 widens
+   16-bit data to INT32 the same way).  No counterpart in WaveletDataType. */
+#define WV_DTYPE_INT16 4
 
 const char *wv_last_error(void);
 /* library version string, e.g. "wv_hip 1.0 gfx950" */
@@ -68,10 +72,10 @@ int wv_geom_init(wv_geom *g, const int orig[3], const int steps[3],
                  int wavelet, int dtype);
 
 /* ------------------------------------------------------------------ *
- * Device statistics of a coefficient array, produced as a side effect of
- * the last DWT pass (or by wv_stats_compute): a 2048-bin histogram of bits
+ * Device statistics of a coefficient array (wv_stats_compute, or
+ * wv_dwt3d_forward with a stats block): a 2048-bin histogram of bits
  * 30..20 of |x|, the signed extremes and max |x|.  One WV_STATS_BYTES
- * device block; clear it with wv_stats_clear before a forward transform.
+ * device block.  The threshold select does not need it.
  * ------------------------------------------------------------------ */
 #define WV_STATS_BYTES 8256
 
@@ -89,7 +93,7 @@ int wv_stats_compute(const float *coef_d, int64_t n, void *stats_d,
  * translateCubeDataToFloat + padData do, test_compress_common.cc:386-488),
  * runs every level of every axis, and writes out_d in the rotated layout
  * (size g->out_size).  ws_d must hold g->ws_floats floats.  stats_d (may
- * be NULL) receives the fused statistics and must be cleared first.
+ * be NULL) receives the coefficient statistics (an extra read pass).
  * Coefficients are bit-identical to the reference CPU transform.
  * ------------------------------------------------------------------ */
 int wv_dwt3d_forward(const wv_geom *g, const void *in_d, float *out_d,
@@ -102,6 +106,14 @@ int wv_dwt3d_forward_ev(const wv_geom *g, const void *in_d, float *out_d,
                         float *ws_d, void *stats_d, void **events,
                         void *stream);
 int wv_dwt3d_num_passes(const wv_geom *g);
+
+/* thin wrappers over hipEvent_t for callers that time kernels on the
+   library's streams (no reference counterpart; CudaTimer, cuda_timer.h:22-117,
+   plays this role in the reference) */
+int wv_event_create(void **ev);
+int wv_event_destroy(void *ev);
+int wv_event_record(void *ev, void *stream);
+int wv_event_elapsed_ms(void *a, void *b, float *ms);
 
 /* Inverse 3-D transform (waveletTransformGPU(..., inverse=true); the math
    of cdf97_inverse / haar_inv, dwt_cpu.cc:174-203,757-799,966-984).
@@ -127,8 +139,8 @@ typedef struct {
 
 /* workspace bytes for wv_threshold on n coefficients */
 int64_t wv_threshold_ws_bytes(int64_t n);
-/* stats_d from the forward transform (already complete); result_d is a
-   device wv_thresh_result.  Does not synchronise. */
+/* stats_d is unused (may be NULL); result_d is a device wv_thresh_result.
+   Does not synchronise. */
 int wv_threshold(const float *coef_d, int64_t n, float frac,
                  const void *stats_d, void *ws_d, wv_thresh_result *result_d,
                  void *stream);

@@ -205,3 +205,48 @@ def test_compress_lloyd_matches_oracle(wv, shape, dtype, bins, frac):
     assert info.lloyd_iters == want["res"].lloyd_iters
     assert np.array_equal(bins_h, want["bins"])
     assert got == want["file"]
+
+
+@pytest.mark.parametrize("wavelet", [1, 2])
+def test_int16_input_equals_widened_int32(wv, wavelet):
+    vol = _vol((12, 20, 36), np.int32, 13).astype(np.int16)
+    want, size, st = _oracle_coef(vol.astype(np.int32), wavelet, [3, 3, 2])
+    coef, g, _ = wv.dwt3d_forward(torch.from_numpy(vol).cuda(), [3, 3, 2], wavelet,
+                                  with_stats=False)
+    got = coef.cpu().numpy()
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    f16, b16, _ = _compress(wv, vol, wavelet, 2, 256, [3, 3, 2], 0.9)
+    f32, b32, _ = _compress(wv, vol.astype(np.int32), wavelet, 2, 256, [3, 3, 2], 0.9)
+    assert f16 == f32 and np.array_equal(b16, b32)
+
+
+def test_threshold_exact_fallback_when_sample_misleads(wv):
+    """Values at the sampled positions lie far below the rest, so the
+    sampled window misses the rank and the exact full-histogram path runs."""
+    n = 1 << 20
+    rng = np.random.default_rng(12)
+    x = (rng.random(n).astype(np.float32) * 1000 + 100).astype(np.float32)
+    nch, chunk = 1024, 16
+    starts = np.arange(nch, dtype=np.int64) * (n - chunk) // (nch - 1)
+    for s0 in starts:
+        x[s0:s0 + chunk] = rng.random(chunk).astype(np.float32) * 0.01
+    xd = torch.from_numpy(x).cuda()
+    for frac in (0.3, 0.9):
+        _, r = wv.threshold(xd, frac)
+        t = O.thresh(x, frac)
+        assert r.n_candidates < 0            # the fallback ran
+        assert r.threshold == t["threshold"] and r.thresh_idx == t["idx"]
+        assert r.max_abs == t["max_abs"] and r.min_val == t["min"]
+
+
+@pytest.mark.parametrize("n", [1, 7, 1000, 16384, 16385, 300_007])
+def test_threshold_sizes(wv, n):
+    rng = np.random.default_rng(n)
+    x = (rng.standard_normal(n) * 10).astype(np.float32)
+    xd = torch.from_numpy(x).cuda()
+    for frac in (0.0, 0.5, 0.99):
+        _, r = wv.threshold(xd, frac)
+        t = O.thresh(x, frac)
+        if t["idx"] < n:
+            assert r.threshold == t["threshold"] and r.thresh_idx == t["idx"]
+        assert r.max_abs == t["max_abs"]

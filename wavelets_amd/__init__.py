@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(HERE, "libwv_hip.so")
 
 HAAR, CDF97 = 1, 2
 UNIFORM, LOG, LLOYD = 1, 2, 3
-UINT8, INT32, FLOAT32 = 1, 2, 3
+UINT8, INT32, FLOAT32, INT16 = 1, 2, 3, 4
 STATS_BYTES = 8256
 
 _lib = None
@@ -92,6 +92,10 @@ def _declare(L):
     L.wv_dwt3d_inverse.argtypes = [C.POINTER(Geom), P, P, P, P]
     L.wv_dwt3d_forward_ev.argtypes = [C.POINTER(Geom), P, P, P, P, P, P]
     L.wv_dwt3d_num_passes.argtypes = [C.POINTER(Geom)]
+    L.wv_event_create.argtypes = [C.POINTER(C.c_void_p)]
+    L.wv_event_destroy.argtypes = [P]
+    L.wv_event_record.argtypes = [P, P]
+    L.wv_event_elapsed_ms.argtypes = [P, P, C.POINTER(C.c_float)]
     L.wv_threshold_ws_bytes.restype = I64
     L.wv_threshold_ws_bytes.argtypes = [I64]
     L.wv_threshold.argtypes = [P, I64, C.c_float, P, P, P, P]
@@ -144,6 +148,28 @@ def check(rc):
     return rc
 
 
+class Event:
+    """A hipEvent_t owned by the library (timing on any stream)."""
+
+    def __init__(self):
+        self.h = C.c_void_p()
+        check(lib().wv_event_create(C.byref(self.h)))
+
+    def record(self, stream=None):
+        check(lib().wv_event_record(self.h, _stream(stream)))
+
+    def elapsed_ms(self, other):
+        ms = C.c_float()
+        check(lib().wv_event_elapsed_ms(self.h, other.h, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            lib().wv_event_destroy(self.h)
+        except Exception:
+            pass
+
+
 def _i3(v):
     return (C.c_int * 3)(*[int(a) for a in v])
 
@@ -158,13 +184,13 @@ def _ptr(t):
     return C.c_void_p(t.data_ptr()) if t is not None else None
 
 
-DTYPE_ID = {"uint8": UINT8, "int32": INT32, "float32": FLOAT32}
+DTYPE_ID = {"uint8": UINT8, "int32": INT32, "float32": FLOAT32, "int16": INT16}
 
 
 def dtype_id(t):
     name = str(t.dtype).replace("torch.", "")
     if name not in DTYPE_ID:
-        raise ValueError("unsupported input dtype %s (uint8, int32, float32)" % name)
+        raise ValueError("unsupported input dtype %s (uint8, int16, int32, float32)" % name)
     return DTYPE_ID[name]
 
 
@@ -209,14 +235,15 @@ def stats_compute(coef, stream=None):
     return stats
 
 
-def threshold(coef, frac, stats, stream=None, sync=True):
+def threshold(coef, frac, stats=None, stream=None, sync=True):
     """Returns a device tensor holding a wv_thresh_result and, with sync,
     the host ThreshResult."""
     import torch
     n = coef.numel()
     ws = torch.empty(lib().wv_threshold_ws_bytes(n), dtype=torch.uint8, device=coef.device)
     res = torch.zeros(C.sizeof(ThreshResult), dtype=torch.uint8, device=coef.device)
-    check(lib().wv_threshold(_ptr(coef), n, C.c_float(frac), _ptr(stats), _ptr(ws), _ptr(res),
+    check(lib().wv_threshold(_ptr(coef), n, C.c_float(frac), _ptr(stats) if stats is not None
+                             else None, _ptr(ws), _ptr(res),
                              _stream(stream)))
     if not sync:
         return res, None

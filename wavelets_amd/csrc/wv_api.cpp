@@ -87,7 +87,7 @@ int wv_geom_init(wv_geom *g, const int orig[3], const int steps[3],
     set_error("unknown wavelet %d", wavelet);
     return WV_ERR_ARG;
   }
-  if (dtype < WV_DTYPE_UINT8 || dtype > WV_DTYPE_FLOAT32) {
+  if (dtype < WV_DTYPE_UINT8 || dtype > WV_DTYPE_INT16) {
     set_error("unknown dtype %d", dtype);
     return WV_ERR_ARG;
   }
@@ -169,7 +169,7 @@ int wv_dwt3d_forward_ev(const wv_geom *g, const void *in_d, float *out_d,
     P.dst = last ? out_d : inter;
     P.store_zxy = (last && plan.axis[i] == 2 && (g->steps[0] || g->steps[1] || g->steps[2])) ? 1 : 0;
     P.load_zxy = 0;
-    P.stats = (last && stats_d) ? (WvStats *)stats_d : nullptr;
+    P.stats = nullptr;
     const int ntiles = plan_pass(P, plan.axis[i]);
     if (ntiles < 0) { set_error("line too long"); return WV_ERR_UNSUPPORTED; }
     if (events && i == 0) HIPCHK(hipEventRecord((hipEvent_t)events[0], s), "event");
@@ -179,6 +179,7 @@ int wv_dwt3d_forward_ev(const wv_geom *g, const void *in_d, float *out_d,
     src = P.dst;
     clamp = false;
   }
+  if (stats_d) HIPCHK(stats_compute(out_d, g->count, stats_d, s), "stats");
   return WV_OK;
 }
 
@@ -226,12 +227,34 @@ int wv_dwt3d_inverse(const wv_geom *g, const float *coef_d, float *out_d,
 
 int wv_dwt3d_num_passes(const wv_geom *g) { return forward_plan(g).n; }
 
+int wv_event_create(void **ev) {
+  hipEvent_t e;
+  HIPCHK(hipEventCreate(&e), "event create");
+  *ev = (void *)e;
+  return WV_OK;
+}
+
+int wv_event_destroy(void *ev) {
+  HIPCHK(hipEventDestroy((hipEvent_t)ev), "event destroy");
+  return WV_OK;
+}
+
+int wv_event_record(void *ev, void *stream) {
+  HIPCHK(hipEventRecord((hipEvent_t)ev, (hipStream_t)stream), "event record");
+  return WV_OK;
+}
+
+int wv_event_elapsed_ms(void *a, void *b, float *ms) {
+  HIPCHK(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b), "event elapsed");
+  return WV_OK;
+}
+
 int64_t wv_threshold_ws_bytes(int64_t n) { return threshold_ws_bytes(n); }
 
 int wv_threshold(const float *coef_d, int64_t n, float frac,
                  const void *stats_d, void *ws_d, wv_thresh_result *result_d,
                  void *stream) {
-  if (!coef_d || !stats_d || !ws_d || !result_d || n <= 0) {
+  if (!coef_d || !ws_d || !result_d || n <= 0 || n > 0x7fffffffLL) {
     set_error("bad arguments");
     return WV_ERR_ARG;
   }

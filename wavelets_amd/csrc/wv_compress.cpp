@@ -76,7 +76,7 @@ extern "C" int wv_compress(const void *in_d, int dtype, const int orig[3],
   char *w = (char *)work_d;
   float *coef = (float *)(w + L.coef);
   float *ws = (float *)(w + L.ws);
-  void *stats = w + L.stats;
+  void *stats = nullptr;   // the threshold select needs no precomputed statistics
   wv_thresh_result *res_d = (wv_thresh_result *)(w + L.result);
   uint32_t *counts_d = (uint32_t *)(w + L.counts);
   void *table_d = w + L.table;
@@ -86,8 +86,7 @@ extern "C" int wv_compress(const void *in_d, int dtype, const int orig[3],
   HCHK(hipEventCreate(&e1), "event");
   HCHK(hipEventRecord(e0, s), "event record");
 
-  CHK(wv_stats_clear(stats, s));
-  CHK(wv_dwt3d_forward(&g, in_d, coef, ws, stats, s));
+  CHK(wv_dwt3d_forward(&g, in_d, coef, ws, nullptr, s));
   CHK(wv_threshold(coef, g.count, opt->thresh_frac, stats, ws, res_d, s));
   wv_thresh_result res;
   HCHK(hipMemcpyAsync(&res, res_d, sizeof res, hipMemcpyDeviceToHost, s), "D2H threshold");
@@ -141,7 +140,8 @@ extern "C" int wv_compress(const void *in_d, int dtype, const int orig[3],
     d.wavelet = opt->wavelet;
     d.quant_alg = opt->quant_alg;
     d.bin_count = opt->bin_count;
-    d.orig_dtype = dtype;
+    // 16-bit data is stored as an INT32 cubelet (shortImagesToCubelets.cc:9-27)
+    d.orig_dtype = dtype == WV_DTYPE_INT16 ? WV_DTYPE_INT32 : dtype;
     d.thresh_frac = opt->thresh_frac;
     d.threshold = res.threshold;
     d.max_val = res.max_abs;

@@ -3,7 +3,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define WV_BLOCK 256
+#define WV_BLOCK 256        // threads per block of the streaming kernels
+#define WV_DBLOCK 256       // threads per block of the DWT tile kernel
 #define WV_MAXP 32          // max (low,high) pairs a thread owns per level
 #define WV_H1_BITS 11       // threshold select digit 1: bits 30..20 of |x|
 #define WV_H1_BINS 2048
@@ -39,4 +40,21 @@ __device__ inline int wv_mirror(int o, int n) {
     if (o < 0) o = -o;
   }
   return o;
+}
+
+// Wave-aggregated histogram increment: lanes holding the same bin are found
+// with one ballot per bin bit and the lowest of them adds the group's
+// count, so an LDS bin takes one atomic per wave instead of one per lane.
+template <int BITS>
+__device__ inline void wv_wave_hist_add(uint32_t *h, uint32_t bin, bool valid) {
+  uint64_t peers = __ballot(valid);
+#pragma unroll
+  for (int b = 0; b < BITS; b++) {
+    const bool bit = (bin >> b) & 1u;
+    const uint64_t m = __ballot(bit);
+    peers &= bit ? m : ~m;
+  }
+  const int lane = threadIdx.x & 63;
+  if (valid && (peers & ((1ull << lane) - 1)) == 0)
+    atomicAdd(&h[bin], (uint32_t)__popcll(peers));
 }

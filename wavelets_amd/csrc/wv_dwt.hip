@@ -2,27 +2,31 @@
 // gfx950.
 //
 // One kernel, instantiated per (wavelet, direction, source dtype, clamping,
-// fused statistics), processes a *line tile*: T lines of one axis, each of
-// length L (the padded axis length).  The block
+// tile width), processes a *line tile*: T lines of one axis, each of length
+// L (the padded axis length).  The block
 //   1. loads the tile into LDS as tile[pos][line] (row stride T+1 so both
 //      line-contiguous and position-contiguous accesses are bank-conflict
-//      free), converting u8/i32 to float and REPEAT-padding by clamping
-//      coordinates into the unpadded source,
+//      free), converting u8/i16/i32 to float and REPEAT-padding by clamping
+//      coordinates into the unpadded source; every thread keeps WV_UNR
+//      independent loads in flight,
 //   2. runs every level of the 1-D transform in LDS: each thread owns up to
 //      WV_MAXP consecutive (low, high) output pairs of one line and walks a
-//      9-value register window over its inputs, then all threads write back
-//      after a barrier,
+//      9-value register window over its inputs (mirroring only in the
+//      chunks that touch a line end), then all threads write back after a
+//      barrier,
 //   3. stores the tile either back in x,y,z layout or, for the z pass, in the
-//      reference's rotated z,x,y output layout, optionally folding every
-//      stored coefficient into the threshold statistics (a 2048-bin LDS
-//      histogram of |x| bits 30..20, min, max, max |x|).
+//      reference's rotated z,x,y output layout.
 // Every global access of a tile is either a contiguous block of the tile or
-// T-float (128 B at T = 32) segments, so each pass moves 8 B per element
-// (plus the input's own size on the first pass).
+// T-float segments (128 B at T = 32), so each pass moves 8 B per element
+// (the input's own size plus 4 B on the first pass).
+//
+// With the tile width a compile-time constant, every LDS address inside a
+// line chunk is an immediate offset from one base register.
 //
 // Arithmetic follows the reference CPU code operation by operation with
 // contraction disabled (-ffp-contract=off), so coefficients are
 // bit-identical to cdf97 / haar (dwt_cpu.cc:139-203,683-799).
+#include <cstdlib>
 #include "wv_device.h"
 #include "wv_internal.h"
 
@@ -41,66 +45,37 @@ constexpr float H0 = -.85269867900940f, H1 = .418092273222210f,
                 H4 = -.037828455506995f;
 constexpr float INV_SQRT2 = 0.70710678118654752440f;   // dwt_cpu.cc:11
 
+#define WV_UNR 32  // independent loads / stores in flight per thread
+
 template <typename S> __device__ inline float to_f(S v) { return (float)v; }
 
 struct Tile {
   int nvalid;      // valid lines in this tile
-  int a, b;        // fixed coordinates: axis0 -> first row; axis1 -> x0,z;
-                   // axis2 -> x0,y
+  int a, b;        // axis0: first row; axis1: x0, z; axis2: x0, y
 };
 
-__device__ inline Tile decode_tile(const DwtPass &P) {
+__device__ inline Tile decode_tile(const DwtPass &P, int T) {
   Tile t;
   const int tile = blockIdx.x;
   if (P.axis == 0) {
     const int rows = P.Y * P.Z;
-    t.a = tile * P.T;
+    t.a = tile * T;
     t.b = 0;
-    t.nvalid = min(P.T, rows - t.a);
+    t.nvalid = min(T, rows - t.a);
   } else {
     const int xt = tile % P.ntiles_x;
-    t.a = xt * P.T;
+    t.a = xt * T;
     t.b = tile / P.ntiles_x;
-    t.nvalid = min(P.T, P.X - t.a);
+    t.nvalid = min(T, P.X - t.a);
   }
   return t;
 }
 
-// source index of tile element (line l, position p)
-template <bool CLAMP>
-__device__ inline size_t src_index(const DwtPass &P, const Tile &t, int l,
-                                   int p) {
-  int x, y, z;
-  if (P.axis == 0) {
-    const int r = t.a + l;
-    y = r % P.Y;
-    z = r / P.Y;
-    x = p;
-  } else if (P.axis == 1) {
-    x = t.a + l; y = p; z = t.b;
-  } else {
-    x = t.a + l; y = t.b; z = p;
-    if (P.load_zxy) return ((size_t)y * P.X + x) * P.Z + z;
-  }
-  if (CLAMP) {
-    x = min(x, P.ox - 1);
-    y = min(y, P.oy - 1);
-    z = min(z, P.oz - 1);
-    return ((size_t)z * P.oy + y) * P.ox + x;
-  }
-  return ((size_t)z * P.Y + y) * P.X + x;
-}
-
-__device__ inline size_t dst_index(const DwtPass &P, const Tile &t, int l,
-                                   int p) {
-  if (P.axis == 0) return (size_t)(t.a + l) * P.X + p;
-  if (P.axis == 1) return ((size_t)t.b * P.Y + p) * P.X + (t.a + l);
-  if (P.store_zxy) return ((size_t)t.b * P.X + (t.a + l)) * P.Z + p;
-  return ((size_t)p * P.Y + t.b) * P.X + (t.a + l);
-}
-
-// line-major traversal (consecutive i walk along a line) or position-major
-// (consecutive i walk across lines at one position)
+// ---- tile addressing -------------------------------------------------------
+// A load/store walks the tile either line-major (consecutive threads step
+// along a line: x lines, and z lines of the rotated z,x,y layout) or
+// position-major (consecutive threads step across lines: y and z lines of
+// the x,y,z layout, where T neighbouring lines are T neighbouring x).
 __device__ inline bool line_major_load(const DwtPass &P) {
   return P.axis == 0 || (P.axis == 2 && P.load_zxy);
 }
@@ -108,30 +83,108 @@ __device__ inline bool line_major_store(const DwtPass &P) {
   return P.axis == 0 || (P.axis == 2 && P.store_zxy);
 }
 
-__device__ inline void split(const DwtPass &P, bool line_major, int i, int &l,
-                             int &p) {
-  if (line_major) {
-    l = (int)__umulhi((unsigned)i, P.magic_L);
-    p = i - l * P.L;
-  } else {
-    l = i & (P.T - 1);
-    p = i >> P.log2T;
+// global offset of (line l, position 0) for a line-major access
+template <bool CLAMP>
+__device__ inline size_t line_base_load(const DwtPass &P, const Tile &t, int l) {
+  if (P.axis == 0) {
+    const int r = t.a + l;
+    int y = r % P.Y, z = r / P.Y;
+    if (CLAMP) {
+      y = min(y, P.oy - 1);
+      z = min(z, P.oz - 1);
+      return ((size_t)z * P.oy + y) * P.ox;
+    }
+    return ((size_t)z * P.Y + y) * P.X;
   }
+  return ((size_t)t.b * P.X + (t.a + l)) * P.Z;     // z,x,y layout
+}
+__device__ inline size_t line_base_store(const DwtPass &P, const Tile &t, int l) {
+  if (P.axis == 0) return (size_t)(t.a + l) * P.X;
+  return ((size_t)t.b * P.X + (t.a + l)) * P.Z;
+}
+
+// global offset of (line 0, position p) for a position-major access; the
+// line index is added as +x (clamped for the unpadded source)
+template <bool CLAMP>
+__device__ inline size_t pos_base_load(const DwtPass &P, const Tile &t, int p) {
+  if (CLAMP) {
+    if (P.axis == 1) {
+      const int z = min(t.b, P.oz - 1), y = min(p, P.oy - 1);
+      return ((size_t)z * P.oy + y) * P.ox;
+    }
+    const int z = min(p, P.oz - 1), y = min(t.b, P.oy - 1);
+    return ((size_t)z * P.oy + y) * P.ox;
+  }
+  if (P.axis == 1) return ((size_t)t.b * P.Y + p) * P.X;
+  return ((size_t)p * P.Y + t.b) * P.X;
+}
+__device__ inline size_t pos_base_store(const DwtPass &P, const Tile &t, int p) {
+  if (P.axis == 1) return ((size_t)t.b * P.Y + p) * P.X;
+  return ((size_t)p * P.Y + t.b) * P.X;
 }
 
 // ---- forward levels -------------------------------------------------------
+template <bool MIRROR>
 __device__ inline float ldm(const float *col, int p, int n, int LS) {
-  if ((unsigned)p >= (unsigned)n) p = wv_mirror(p, n);
+  if (MIRROR && (unsigned)p >= (unsigned)n) p = wv_mirror(p, n);
   return col[p * LS];
 }
 
-template <int WAVE>
+// one thread's chunk of pairs [j0, j0+cnt) of one line at level length n
+template <bool MIRROR>
+__device__ inline void cdf97_chunk(const float *col, int LS, int n, int j0, int cnt,
+                                   float *lo, float *hi) {
+  const int c = 2 * j0;
+  const float *cc = col + c * LS;       // positions relative to c
+  float w0, w1, w2, w3, w4, w5, w6, w7, w8;
+  if (MIRROR) {
+    w0 = ldm<true>(col, c - 4, n, LS); w1 = ldm<true>(col, c - 3, n, LS);
+    w2 = ldm<true>(col, c - 2, n, LS); w3 = ldm<true>(col, c - 1, n, LS);
+    w4 = ldm<true>(col, c, n, LS);     w5 = ldm<true>(col, c + 1, n, LS);
+    w6 = ldm<true>(col, c + 2, n, LS); w7 = ldm<true>(col, c + 3, n, LS);
+    w8 = ldm<true>(col, c + 4, n, LS);
+  } else {
+    w0 = cc[-4 * LS]; w1 = cc[-3 * LS]; w2 = cc[-2 * LS]; w3 = cc[-1 * LS];
+    w4 = cc[0];       w5 = cc[LS];      w6 = cc[2 * LS];  w7 = cc[3 * LS];
+    w8 = cc[4 * LS];
+  }
+#pragma unroll
+  for (int q = 0; q < WV_MAXP; q++) {
+    if (q < cnt) {
+      // cdf97_tmpl dwt_cpu.cc:709-723: centre tap first, then
+      // F[k]*(a[i+k]+a[i-k]) for k = 1..4, no contraction
+      float sl = F0 * w4;
+      sl += F1 * (w5 + w3);
+      sl += F2 * (w6 + w2);
+      sl += F3 * (w7 + w1);
+      sl += F4 * (w8 + w0);
+      float sh = G0 * w5;
+      sh += G1 * (w6 + w4);
+      sh += G2 * (w7 + w3);
+      sh += G3 * (w8 + w2);
+      lo[q] = sl;
+      hi[q] = sh;
+      w0 = w2; w1 = w3; w2 = w4; w3 = w5; w4 = w6; w5 = w7; w6 = w8;
+      if (MIRROR) {
+        const int nc = c + 2 * q + 2;
+        w7 = ldm<true>(col, nc + 3, n, LS);
+        w8 = ldm<true>(col, nc + 4, n, LS);
+      } else {
+        w7 = cc[(2 * q + 5) * LS];
+        w8 = cc[(2 * q + 6) * LS];
+      }
+    }
+  }
+}
+
+template <int WAVE, int TT>
 __device__ void forward_levels(float *tile, const DwtPass &P) {
+  const int T = TT > 0 ? TT : P.T;
+  const int LS = T + 1;
   const int tid = threadIdx.x;
-  const int LS = P.T + 1;
-  const int line = tid & (P.T - 1);
-  const int k = tid >> P.log2T;
-  const int tpl = WV_BLOCK >> P.log2T;
+  const int line = tid % T;
+  const int k = tid / T;
+  const int tpl = WV_DBLOCK / T;
   float *col = tile + line;
   int n = P.L;
   for (int s = 0; s < P.levels; s++) {
@@ -142,40 +195,17 @@ __device__ void forward_levels(float *tile, const DwtPass &P) {
     float lo[WV_MAXP], hi[WV_MAXP];
     if (cnt > 0) {
       if (WAVE == WV_WAVELET_CDF97) {
-        const int c = 2 * j0;
-        float w0 = ldm(col, c - 4, n, LS), w1 = ldm(col, c - 3, n, LS),
-              w2 = ldm(col, c - 2, n, LS), w3 = ldm(col, c - 1, n, LS),
-              w4 = ldm(col, c, n, LS), w5 = ldm(col, c + 1, n, LS),
-              w6 = ldm(col, c + 2, n, LS), w7 = ldm(col, c + 3, n, LS),
-              w8 = ldm(col, c + 4, n, LS);
-#pragma unroll
-        for (int q = 0; q < WV_MAXP; q++) {
-          if (q < cnt) {
-            // cdf97_tmpl dwt_cpu.cc:709-723: centre tap first, then
-            // F[k]*(a[i+k]+a[i-k]) for k = 1..4, no contraction
-            float sl = F0 * w4;
-            sl += F1 * (w5 + w3);
-            sl += F2 * (w6 + w2);
-            sl += F3 * (w7 + w1);
-            sl += F4 * (w8 + w0);
-            float sh = G0 * w5;
-            sh += G1 * (w6 + w4);
-            sh += G2 * (w7 + w3);
-            sh += G3 * (w8 + w2);
-            lo[q] = sl;
-            hi[q] = sh;
-            const int nc = c + 2 * q + 2;
-            w0 = w2; w1 = w3; w2 = w4; w3 = w5; w4 = w6; w5 = w7; w6 = w8;
-            w7 = ldm(col, nc + 3, n, LS);
-            w8 = ldm(col, nc + 4, n, LS);
-          }
-        }
+        // the window reaches 4 positions beyond the chunk on both sides
+        if (2 * j0 - 4 >= 0 && 2 * (j0 + cnt) + 4 < n)
+          cdf97_chunk<false>(col, LS, n, j0, cnt, lo, hi);
+        else
+          cdf97_chunk<true>(col, LS, n, j0, cnt, lo, hi);
       } else {
+        const float *cc = col + 2 * j0 * LS;
 #pragma unroll
         for (int q = 0; q < WV_MAXP; q++) {
           if (q < cnt) {
-            const int j = j0 + q;
-            const float a = col[(2 * j) * LS], b = col[(2 * j + 1) * LS];
+            const float a = cc[(2 * q) * LS], b = cc[(2 * q + 1) * LS];
             // haar_internal dwt_cpu.cc:159-162
             hi[q] = (a - b) * INV_SQRT2;
             lo[q] = (a + b) * INV_SQRT2;
@@ -185,11 +215,12 @@ __device__ void forward_levels(float *tile, const DwtPass &P) {
     }
     __syncthreads();
     if (cnt > 0) {
+      float *lw = col + j0 * LS, *hw = col + (pairs + j0) * LS;
 #pragma unroll
       for (int q = 0; q < WV_MAXP; q++) {
         if (q < cnt) {
-          col[(j0 + q) * LS] = lo[q];
-          col[(pairs + j0 + q) * LS] = hi[q];
+          lw[q * LS] = lo[q];
+          hw[q * LS] = hi[q];
         }
       }
     }
@@ -206,13 +237,14 @@ __device__ inline float ldi(const float *col, int i, int n, int half, int LS) {
   return (i & 1) ? col[(half + (i >> 1)) * LS] : col[(i >> 1) * LS];
 }
 
-template <int WAVE>
+template <int WAVE, int TT>
 __device__ void inverse_levels(float *tile, const DwtPass &P) {
+  const int T = TT > 0 ? TT : P.T;
+  const int LS = T + 1;
   const int tid = threadIdx.x;
-  const int LS = P.T + 1;
-  const int line = tid & (P.T - 1);
-  const int k = tid >> P.log2T;
-  const int tpl = WV_BLOCK >> P.log2T;
+  const int line = tid % T;
+  const int k = tid / T;
+  const int tpl = WV_DBLOCK / T;
   float *col = tile + line;
   int n = P.L >> (P.levels - 1);
   for (int s = 0; s < P.levels; s++) {
@@ -265,11 +297,12 @@ __device__ void inverse_levels(float *tile, const DwtPass &P) {
     }
     __syncthreads();
     if (cnt > 0) {
+      float *w = col + 2 * j0 * LS;
 #pragma unroll
       for (int q = 0; q < WV_MAXP; q++) {
         if (q < cnt) {
-          col[(2 * (j0 + q)) * LS] = ev[q];
-          col[(2 * (j0 + q) + 1) * LS] = od[q];
+          w[(2 * q) * LS] = ev[q];
+          w[(2 * q + 1) * LS] = od[q];
         }
       }
     }
@@ -279,127 +312,157 @@ __device__ void inverse_levels(float *tile, const DwtPass &P) {
 }
 
 // ---- the pass kernel ------------------------------------------------------
-template <int WAVE, bool INV, typename SRC, bool CLAMP, bool STATS>
-__global__ void __launch_bounds__(WV_BLOCK)
+template <int WAVE, bool INV, typename SRC, bool CLAMP, int TT>
+__global__ void __launch_bounds__(WV_DBLOCK, 2)
 dwt_pass_kernel(DwtPass P) {
   extern __shared__ float smem[];
+  __shared__ size_t lbase[32];
   float *tile = smem;
-  const int LS = P.T + 1;
+  const int T = TT > 0 ? TT : P.T;
+  const int LS = T + 1;
   const int tid = threadIdx.x;
-  const Tile t = decode_tile(P);
-  const int total = P.T * P.L;
+  const Tile t = decode_tile(P, T);
+  const int total = T * P.L;
   const SRC *src = (const SRC *)P.src;
 
-  uint32_t *sh_hist = nullptr;
-  if (STATS) {
-    sh_hist = (uint32_t *)(smem + P.L * LS);
-    for (int i = tid; i < WV_H1_BINS; i += WV_BLOCK) sh_hist[i] = 0;
-  }
-
   // 1. load
-  {
-    const bool lm = line_major_load(P);
-    for (int i = tid; i < total; i += WV_BLOCK) {
-      int l, p;
-      split(P, lm, i, l, p);
-      float v = 0.f;
-      if (l < t.nvalid) v = to_f(src[src_index<CLAMP>(P, t, l, p)]);
-      tile[p * LS + l] = v;
+  if (line_major_load(P)) {
+    if (tid < T) lbase[tid] = tid < t.nvalid ? line_base_load<CLAMP>(P, t, tid) : 0;
+    __syncthreads();
+    const int pmax = (CLAMP && P.axis == 0) ? P.ox - 1 : P.L - 1;
+    for (int i0 = tid; i0 < total; i0 += WV_DBLOCK * WV_UNR) {
+      float v[WV_UNR];
+#pragma unroll
+      for (int u = 0; u < WV_UNR; u++) {
+        const int i = i0 + u * WV_DBLOCK;
+        v[u] = 0.f;
+        if (i < total) {
+          const int l = (int)__umulhi((unsigned)i, P.magic_L);
+          const int p = i - l * P.L;
+          if (l < t.nvalid) v[u] = to_f(src[lbase[l] + min(p, pmax)]);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < WV_UNR; u++) {
+        const int i = i0 + u * WV_DBLOCK;
+        if (i < total) {
+          const int l = (int)__umulhi((unsigned)i, P.magic_L);
+          const int p = i - l * P.L;
+          tile[p * LS + l] = v[u];
+        }
+      }
+    }
+  } else {
+    const int l = tid % T;
+    const int pstep = WV_DBLOCK / T;
+    const bool lvalid = l < t.nvalid;
+    const int x = CLAMP ? min(t.a + l, P.ox - 1) : t.a + l;
+    for (int p0 = tid / T; p0 < P.L; p0 += pstep * WV_UNR) {
+      float v[WV_UNR];
+#pragma unroll
+      for (int u = 0; u < WV_UNR; u++) {
+        const int p = p0 + u * pstep;
+        v[u] = 0.f;
+        if (p < P.L && lvalid) v[u] = to_f(src[pos_base_load<CLAMP>(P, t, p) + x]);
+      }
+#pragma unroll
+      for (int u = 0; u < WV_UNR; u++) {
+        const int p = p0 + u * pstep;
+        if (p < P.L) tile[p * LS + l] = v[u];
+      }
     }
   }
   __syncthreads();
 
   // 2. levels
   if (P.levels > 0) {
-    if (INV) inverse_levels<WAVE>(tile, P);
-    else forward_levels<WAVE>(tile, P);
+    if (INV) inverse_levels<WAVE, TT>(tile, P);
+    else forward_levels<WAVE, TT>(tile, P);
   }
 
-  // 3. store (+ statistics)
-  uint32_t mn = 0xffffffffu, mx = 0u, mk = 0u;
-  unsigned zero_run = 0;   // |x| keys landing in bin 0 (counted in a register)
-  {
-    const bool lm = line_major_store(P);
-    for (int i = tid; i < total; i += WV_BLOCK) {
-      int l, p;
-      split(P, lm, i, l, p);
-      if (l < t.nvalid) {
-        const float v = tile[p * LS + l];
-        P.dst[dst_index(P, t, l, p)] = v;
-        if (STATS) {
-          const uint32_t o = wv_f2ord(v);
-          mn = min(mn, o);
-          mx = max(mx, o);
-          const uint32_t key = __builtin_bit_cast(uint32_t, v) & 0x7fffffffu;
-          mk = max(mk, key);
-          const uint32_t bin = key >> WV_H1_SHIFT;
-          if (bin == 0) zero_run++;
-          else atomicAdd(&sh_hist[bin], 1u);
+  // 3. store
+  if (line_major_store(P)) {
+    if (tid < T) lbase[tid] = tid < t.nvalid ? line_base_store(P, t, tid) : 0;
+    __syncthreads();
+    for (int i0 = tid; i0 < total; i0 += WV_DBLOCK * WV_UNR) {
+#pragma unroll
+      for (int u = 0; u < WV_UNR; u++) {
+        const int i = i0 + u * WV_DBLOCK;
+        if (i < total) {
+          const int l = (int)__umulhi((unsigned)i, P.magic_L);
+          const int p = i - l * P.L;
+          if (l < t.nvalid) P.dst[lbase[l] + p] = tile[p * LS + l];
         }
       }
     }
-  }
-  if (STATS) {
-    WvStats *st = P.stats;
-    // wave reductions, then one global atomic per wave
-    for (int off = 32; off > 0; off >>= 1) {
-      mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
-      mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
-      mk = max(mk, (uint32_t)__shfl_xor((int)mk, off));
-      zero_run += (unsigned)__shfl_xor((int)zero_run, off);
-    }
-    if ((tid & 63) == 0) {
-      if (zero_run) atomicAdd(&sh_hist[0], zero_run);
-      atomicMin(&st->min_ord, mn);
-      atomicMax(&st->max_ord, mx);
-      atomicMax(&st->max_key, mk);
-    }
-    __syncthreads();
-    for (int i = tid; i < WV_H1_BINS; i += WV_BLOCK) {
-      const uint32_t c = sh_hist[i];
-      if (c) atomicAdd(&st->hist[i], c);
+  } else {
+    const int l = tid % T;
+    const int pstep = WV_DBLOCK / T;
+    if (l < t.nvalid) {
+      for (int p0 = tid / T; p0 < P.L; p0 += pstep * WV_UNR) {
+#pragma unroll
+        for (int u = 0; u < WV_UNR; u++) {
+          const int p = p0 + u * pstep;
+          if (p < P.L) P.dst[pos_base_store(P, t, p) + t.a + l] = tile[p * LS + l];
+        }
+      }
     }
   }
 }
 
 // ---- host side ------------------------------------------------------------
-template <int WAVE, bool INV, typename SRC, bool CLAMP, bool STATS>
+template <int WAVE, bool INV, typename SRC, bool CLAMP, int TT>
 static hipError_t launch_one(const DwtPass &P, int ntiles, hipStream_t s) {
-  auto k = dwt_pass_kernel<WAVE, INV, SRC, CLAMP, STATS>;
-  const size_t lds = ((size_t)P.L * (P.T + 1) + (STATS ? WV_H1_BINS : 0)) * 4;
+  auto k = dwt_pass_kernel<WAVE, INV, SRC, CLAMP, TT>;
+  const size_t lds = (size_t)P.L * (P.T + 1) * 4;
   static bool attr_set = false;
   if (!attr_set) {
+    // 160 KB of LDS per workgroup on gfx950, minus the kernel's static LDS
     hipError_t e = hipFuncSetAttribute((const void *)k,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       160 * 1024);
+                                       159 * 1024);
     if (e != hipSuccess) return e;
     attr_set = true;
   }
-  hipLaunchKernelGGL(k, dim3(ntiles), dim3(WV_BLOCK), lds, s, P);
+  hipLaunchKernelGGL(k, dim3(ntiles), dim3(WV_DBLOCK), lds, s, P);
   return hipGetLastError();
 }
 
+template <int WAVE, bool INV, typename SRC, bool CLAMP>
+static hipError_t launch_tile(const DwtPass &P, int ntiles, hipStream_t s) {
+  switch (P.T) {
+    case 32: return launch_one<WAVE, INV, SRC, CLAMP, 32>(P, ntiles, s);
+    case 16: return launch_one<WAVE, INV, SRC, CLAMP, 16>(P, ntiles, s);
+    case 8: return launch_one<WAVE, INV, SRC, CLAMP, 8>(P, ntiles, s);
+    default: return launch_one<WAVE, INV, SRC, CLAMP, 0>(P, ntiles, s);
+  }
+}
+
 template <int WAVE, bool INV>
-static hipError_t launch_wave(const DwtPass &P, int src_dtype, bool clamp,
-                              bool stats, int ntiles, hipStream_t s) {
-  if (INV) {   // inverse passes read float coefficients, never clamp
-    return launch_one<WAVE, true, float, false, false>(P, ntiles, s);
-  }
+static hipError_t launch_wave(const DwtPass &P, int src_dtype, bool clamp, int ntiles,
+                              hipStream_t s) {
+  if (INV) return launch_tile<WAVE, true, float, false>(P, ntiles, s);
   if (clamp) {
-    if (src_dtype == WV_DTYPE_UINT8)
-      return stats ? launch_one<WAVE, false, uint8_t, true, true>(P, ntiles, s)
-                   : launch_one<WAVE, false, uint8_t, true, false>(P, ntiles, s);
-    if (src_dtype == WV_DTYPE_INT32)
-      return stats ? launch_one<WAVE, false, int32_t, true, true>(P, ntiles, s)
-                   : launch_one<WAVE, false, int32_t, true, false>(P, ntiles, s);
-    return stats ? launch_one<WAVE, false, float, true, true>(P, ntiles, s)
-                 : launch_one<WAVE, false, float, true, false>(P, ntiles, s);
+    if (src_dtype == WV_DTYPE_UINT8) return launch_tile<WAVE, false, uint8_t, true>(P, ntiles, s);
+    if (src_dtype == WV_DTYPE_INT32) return launch_tile<WAVE, false, int32_t, true>(P, ntiles, s);
+    if (src_dtype == WV_DTYPE_INT16) return launch_tile<WAVE, false, int16_t, true>(P, ntiles, s);
+    return launch_tile<WAVE, false, float, true>(P, ntiles, s);
   }
-  return stats ? launch_one<WAVE, false, float, false, true>(P, ntiles, s)
-               : launch_one<WAVE, false, float, false, false>(P, ntiles, s);
+  return launch_tile<WAVE, false, float, false>(P, ntiles, s);
 }
 
 static int ilog2(int v) { int r = 0; while ((1 << (r + 1)) <= v) r++; return r; }
+
+// tile width cap (autotuning hook): WV_DWT_TILE in the environment
+static int tile_cap() {
+  static int cap = -1;
+  if (cap < 0) {
+    const char *e = getenv("WV_DWT_TILE");
+    cap = e ? atoi(e) : 32;
+    if (cap != 8 && cap != 16 && cap != 32) cap = 32;
+  }
+  return cap;
+}
 
 // Fills the tile geometry of a pass; returns the number of tiles or -1 if
 // the line is longer than one LDS tile can hold.
@@ -408,10 +471,10 @@ int plan_pass(DwtPass &P, int axis) {
   P.axis = axis;
   P.L = dims[axis];
   if (P.L > WV_MAX_LINE) return -1;
-  int T = 32;
-  while (T > 1 && (long)P.L * T > 2L * WV_BLOCK * WV_MAXP) T >>= 1;
+  int T = tile_cap();
+  while (T > 1 && (long)P.L * T > 2L * WV_DBLOCK * WV_MAXP) T >>= 1;
   if (axis != 0) {
-    while (T > 1 && T / 2 >= P.X) T >>= 1;   // narrow cubelets
+    while (T > 8 && T / 2 >= P.X) T >>= 1;   // narrow cubelets
   }
   P.T = T;
   P.log2T = ilog2(T);
@@ -426,12 +489,13 @@ int plan_pass(DwtPass &P, int axis) {
 
 hipError_t launch_pass(const DwtPass &P, int ntiles, int wavelet, bool inverse,
                        int src_dtype, bool clamp, bool stats, hipStream_t s) {
+  (void)stats;
   if (ntiles <= 0) return hipSuccess;
   if (wavelet == WV_WAVELET_CDF97)
-    return inverse ? launch_wave<WV_WAVELET_CDF97, true>(P, src_dtype, clamp, stats, ntiles, s)
-                   : launch_wave<WV_WAVELET_CDF97, false>(P, src_dtype, clamp, stats, ntiles, s);
-  return inverse ? launch_wave<WV_WAVELET_HAAR, true>(P, src_dtype, clamp, stats, ntiles, s)
-                 : launch_wave<WV_WAVELET_HAAR, false>(P, src_dtype, clamp, stats, ntiles, s);
+    return inverse ? launch_wave<WV_WAVELET_CDF97, true>(P, src_dtype, clamp, ntiles, s)
+                   : launch_wave<WV_WAVELET_CDF97, false>(P, src_dtype, clamp, ntiles, s);
+  return inverse ? launch_wave<WV_WAVELET_HAAR, true>(P, src_dtype, clamp, ntiles, s)
+                 : launch_wave<WV_WAVELET_HAAR, false>(P, src_dtype, clamp, ntiles, s);
 }
 
 }  // namespace wv

@@ -39,14 +39,22 @@ struct LloydState {
 };
 
 // ---------------------------------------------------------------- gather
+// keys > threshold, block-buffered in LDS so the global counter takes one
+// atomic per 4096 coefficients
+#define LL_GITEMS 16
+#define LL_GCHUNK (WV_BLOCK * LL_GITEMS)
 __global__ void __launch_bounds__(WV_BLOCK)
 ll_gather(const float *coef, int64_t n, uint32_t tkey, uint32_t *out,
           LloydState *S) {
-  const int lane = threadIdx.x & 63;
-  const int64_t stride = (int64_t)gridDim.x * WV_BLOCK;
-  const int64_t nround = (n + stride - 1) / stride;
-  for (int64_t r = 0; r < nround; r++) {
-    const int64_t i = r * stride + (int64_t)blockIdx.x * WV_BLOCK + threadIdx.x;
+  __shared__ uint32_t buf[LL_GCHUNK];
+  __shared__ uint32_t cnt, base;
+  const int tid = threadIdx.x, lane = tid & 63;
+  if (tid == 0) cnt = 0;
+  __syncthreads();
+  const int64_t start = (int64_t)blockIdx.x * LL_GCHUNK;
+#pragma unroll
+  for (int u = 0; u < LL_GITEMS; u++) {
+    const int64_t i = start + (int64_t)u * WV_BLOCK + tid;
     uint32_t key = 0;
     bool hit = false;
     if (i < n) {
@@ -55,12 +63,17 @@ ll_gather(const float *coef, int64_t n, uint32_t tkey, uint32_t *out,
     }
     const uint64_t m = __ballot(hit);
     if (m) {
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(&S->n_gt, (uint32_t)__popcll(m));
-      base = __shfl(base, 0);
-      if (hit) out[base + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = key;
+      uint32_t off = 0;
+      if (lane == 0) off = atomicAdd(&cnt, (uint32_t)__popcll(m));
+      off = __shfl(off, 0);
+      if (hit) buf[off + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = key;
     }
   }
+  __syncthreads();
+  const uint32_t c = cnt;
+  if (tid == 0 && c) base = atomicAdd(&S->n_gt, c);
+  __syncthreads();
+  for (uint32_t i = tid; i < c; i += WV_BLOCK) out[base + i] = buf[i];
 }
 
 // ---------------------------------------------------------------- sort
@@ -437,8 +450,7 @@ extern "C" int wv_lloyd(const float *coef_d, int64_t n, const wv_thresh_result *
 
   hipLaunchKernelGGL(ll_state_init, dim3(1), dim3(1), 0, s, S, (int)psize, cs, max_iters);
   // 1. tail = threshold duplicates ranked after idx, then sorted keys > T
-  int64_t gb = (n + WV_BLOCK * 8 - 1) / (WV_BLOCK * 8);
-  if (gb > 8192) gb = 8192;
+  const int64_t gb = (n + LL_GCHUNK - 1) / LL_GCHUNK;
   hipLaunchKernelGGL(ll_gather, dim3((unsigned)gb), dim3(WV_BLOCK), 0, s, coef_d, n,
                      res->threshold_key, keys, S);
   LloydState hs;

@@ -91,6 +91,7 @@ quant_hist_kernel(const float *coef, int64_t n, QuantDev q,
     for (int i = threadIdx.x; i < q.bin_count; i += WV_BLOCK) h[i] = 0;
   __syncthreads();
   const float *t = tab_in_lds ? tab : q.table;
+#define QUANT_ONE(x) quant_one<ALG>(q, t, (x))
   unsigned zcount = 0;
   const int zb = q.zero_bin;
 
@@ -107,8 +108,26 @@ quant_hist_kernel(const float *coef, int64_t n, QuantDev q,
   const int64_t nvec = aligned ? (n >> 2) : 0;
   const float4 *c4 = (const float4 *)coef;
   int4 *b4 = (int4 *)bins;
-  for (int64_t i = (int64_t)blockIdx.x * WV_BLOCK + threadIdx.x; i < nvec;
-       i += (int64_t)gridDim.x * WV_BLOCK) {
+  // four independent 16-byte loads per thread in flight per iteration
+  const int64_t stride = (int64_t)gridDim.x * WV_BLOCK;
+  int64_t i0 = (int64_t)blockIdx.x * WV_BLOCK + threadIdx.x;
+  for (; i0 + 3 * stride < nvec; i0 += 4 * stride) {
+    float4 vv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) vv[u] = c4[i0 + u * stride];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const float4 v = vv[u];
+      int4 r;
+      r.x = QUANT_ONE(v.x);
+      r.y = QUANT_ONE(v.y);
+      r.z = QUANT_ONE(v.z);
+      r.w = QUANT_ONE(v.w);
+      b4[i0 + u * stride] = r;
+      count(r.x); count(r.y); count(r.z); count(r.w);
+    }
+  }
+  for (int64_t i = i0; i < nvec; i += stride) {
     const float4 v = c4[i];
     int4 r;
     r.x = quant_one<ALG>(q, t, v.x);
@@ -143,8 +162,8 @@ __global__ void zero_u32(uint32_t *p, int n) {
 }
 
 static unsigned grid_for(int64_t n) {
-  int64_t b = (n / 4 + WV_BLOCK * 4 - 1) / (WV_BLOCK * 4);
-  if (b > 4096) b = 4096;
+  int64_t b = (n / 4 + WV_BLOCK * 16 - 1) / (WV_BLOCK * 16);
+  if (b > 2048) b = 2048;
   if (b < 1) b = 1;
   return (unsigned)b;
 }
@@ -193,6 +212,8 @@ quant_hist_uniform_dev_kernel(const float *coef, int64_t n,
                               uint32_t *__restrict__ counts) {
   __shared__ uint32_t h[WV_LDS_HIST];
   const QuantDev q = *qp;
+#undef QUANT_ONE
+#define QUANT_ONE(x) quant_one<WV_QUANT_UNIFORM>(q, nullptr, (x))
   const bool hist_in_lds = q.bin_count <= WV_LDS_HIST;
   if (hist_in_lds)
     for (int i = threadIdx.x; i < q.bin_count; i += WV_BLOCK) h[i] = 0;
@@ -211,8 +232,26 @@ quant_hist_uniform_dev_kernel(const float *coef, int64_t n,
   const int64_t nvec = aligned ? (n >> 2) : 0;
   const float4 *c4 = (const float4 *)coef;
   int4 *b4 = (int4 *)bins;
-  for (int64_t i = (int64_t)blockIdx.x * WV_BLOCK + threadIdx.x; i < nvec;
-       i += (int64_t)gridDim.x * WV_BLOCK) {
+  // four independent 16-byte loads per thread in flight per iteration
+  const int64_t stride = (int64_t)gridDim.x * WV_BLOCK;
+  int64_t i0 = (int64_t)blockIdx.x * WV_BLOCK + threadIdx.x;
+  for (; i0 + 3 * stride < nvec; i0 += 4 * stride) {
+    float4 vv[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) vv[u] = c4[i0 + u * stride];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const float4 v = vv[u];
+      int4 r;
+      r.x = QUANT_ONE(v.x);
+      r.y = QUANT_ONE(v.y);
+      r.z = QUANT_ONE(v.z);
+      r.w = QUANT_ONE(v.w);
+      b4[i0 + u * stride] = r;
+      count(r.x); count(r.y); count(r.z); count(r.w);
+    }
+  }
+  for (int64_t i = i0; i < nvec; i += stride) {
     const float4 v = c4[i];
     int4 r;
     r.x = quant_one<WV_QUANT_UNIFORM>(q, nullptr, v.x);

@@ -1,14 +1,27 @@
-// wv_select.hip -- threshold selection by radix select on the bit patterns
-// of |x| (their order is the order of the values), replacing the full
-// thrust::sort of the reference GPU path (test_compress_gpu.cu:632-662).
+// wv_select.hip -- threshold selection on the device, replacing the full
+// thrust::sort of the reference GPU path (test_compress_gpu.cu:632-662) with
+// the same result as thresh_cpu (thresh_cpu.cc:80-109): the rank
+// idx = (int)(frac * (float)n) element of the ascending |x|.
 //
-//   digit 1: bits 30..20 (2048 bins) -- histogram fused into the last DWT
-//            pass (or wv_stats_compute);
-//   scan 1 : one block finds the bucket holding rank idx = (int)(frac*n);
-//   collect: one read of the coefficients appends the keys of that bucket
-//            to a candidate list (wave-aggregated atomics) and histograms
-//            digit 2 (bits 19..9) of them;
-//   scan 2 / hist 3 (bits 8..0) / scan 3 over the candidates only.
+// The bit patterns of |x| order like the values, so the select works on
+// 31-bit keys:
+//   sample   one block reads 16K keys spread over the array (1024 chunks of
+//            16 consecutive elements), and from their two-level histogram
+//            picks a key window [lo, hi] around the target rank with a
+//            margin of 1% of the ranks on each side;
+//   collect  one streaming read of the coefficients by SEL_GRID blocks:
+//            counts the keys below the window, appends the keys inside it
+//            to a candidate list (buffered in LDS, one global atomic per
+//            block flush) and writes per-block partials (count below, min,
+//            max, max |x|) that a single block then reduces -- no
+//            same-address global atomics per element;
+//   verify   the rank falls inside the window when below <= idx <
+//            below + candidates; otherwise an exact fallback (full 2048-bin
+//            histogram of bits 30..20, then the keys of the rank's bucket)
+//            rebuilds the candidate list -- its kernels return at once when
+//            the window held the rank;
+//   radix    three histogram/scan rounds (bits 30..20, 19..9, 8..0) over
+//            the candidates give the exact key.
 // Everything stays on the device; the result struct is written by the last
 // single-block kernel.
 #include "wv_device.h"
@@ -16,21 +29,37 @@
 
 namespace wv {
 
+#define SEL_SAMPLE 16384
+#define SEL_SCHUNK 16
+#define SEL_MARGIN 0.01f
+#define SEL_ITEMS 16
+#define SEL_CHUNK (WV_BLOCK * SEL_ITEMS)
+#define SEL_FB_GRID 2048
+#define SEL_GRID 1024       // collect blocks (each streams n / SEL_GRID keys)
+#define SEL_CAND_CAP 8192   // LDS candidate buffer per collect block
+
 struct SelState {
   int32_t idx;          // target rank
   int32_t special;      // 1: frac >= 1 (threshold = max |x|)
-  uint32_t b1, b2;      // chosen buckets
-  uint32_t rank;        // rank inside the current bucket
-  uint32_t ncand;       // candidates appended
+  uint32_t key_lo, key_hi;
+  uint32_t below;       // keys < key_lo
+  uint32_t ncand;
+  int32_t fallback;
+  uint32_t crank;       // rank inside the candidate list
+  uint32_t b1, b2, b3;
+  uint32_t min_ord, max_ord, max_key;
   uint32_t pad[2];
+  uint32_t hist1[WV_H1_BINS];
   uint32_t hist2[WV_H2_BINS];
   uint32_t hist3[WV_H3_BINS];
+  uint32_t part[SEL_GRID][4];   // per collect block: below, min, max, max key
 };
 
 int64_t threshold_ws_bytes(int64_t n) {
-  return (int64_t)sizeof(SelState) + 256 + n * 4;
+  return (int64_t)((sizeof(SelState) + 255) & ~(size_t)255) + n * 4 + 256;
 }
 
+// ---- statistics block (API: wv_stats_clear / wv_stats_compute) ----------
 __global__ void stats_clear_kernel(WvStats *st) {
   for (int i = threadIdx.x; i < WV_H1_BINS; i += blockDim.x) st->hist[i] = 0;
   if (threadIdx.x == 0) {
@@ -47,51 +76,68 @@ hipError_t stats_clear(void *stats, hipStream_t s) {
   return hipGetLastError();
 }
 
+// full 2048-bin histogram of bits 30..20 of |x| plus extremes; grid-stride
 __global__ void __launch_bounds__(WV_BLOCK)
-stats_kernel(const float *__restrict__ coef, int64_t n, WvStats *st) {
+hist1_kernel(const float *__restrict__ coef, int64_t n, uint32_t *hist,
+             uint32_t *min_ord, uint32_t *max_ord, uint32_t *max_key,
+             const int32_t *gate) {
+  if (gate && !*gate) return;
   __shared__ uint32_t h[WV_H1_BINS];
   for (int i = threadIdx.x; i < WV_H1_BINS; i += WV_BLOCK) h[i] = 0;
   __syncthreads();
   uint32_t mn = 0xffffffffu, mx = 0, mk = 0;
-  for (int64_t i = (int64_t)blockIdx.x * WV_BLOCK + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * WV_BLOCK) {
-    const float v = coef[i];
-    const uint32_t o = wv_f2ord(v);
-    mn = min(mn, o);
-    mx = max(mx, o);
-    const uint32_t key = __builtin_bit_cast(uint32_t, v) & 0x7fffffffu;
-    mk = max(mk, key);
-    atomicAdd(&h[key >> WV_H1_SHIFT], 1u);
+  for (int64_t base = (int64_t)blockIdx.x * WV_BLOCK * 8; base < n;
+       base += (int64_t)gridDim.x * WV_BLOCK * 8) {
+    float v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int64_t i = base + (int64_t)u * WV_BLOCK + threadIdx.x;
+      v[u] = i < n ? coef[i] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; u++) {
+      const int64_t i = base + (int64_t)u * WV_BLOCK + threadIdx.x;
+      const bool ok = i < n;
+      const uint32_t key = __builtin_bit_cast(uint32_t, v[u]) & 0x7fffffffu;
+      if (ok) {
+        const uint32_t o = wv_f2ord(v[u]);
+        mn = min(mn, o);
+        mx = max(mx, o);
+        mk = max(mk, key);
+      }
+      wv_wave_hist_add<WV_H1_BITS>(h, key >> WV_H1_SHIFT, ok);
+    }
   }
   for (int off = 32; off > 0; off >>= 1) {
     mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
     mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
     mk = max(mk, (uint32_t)__shfl_xor((int)mk, off));
   }
-  if ((threadIdx.x & 63) == 0) {
-    atomicMin(&st->min_ord, mn);
-    atomicMax(&st->max_ord, mx);
-    atomicMax(&st->max_key, mk);
+  if ((threadIdx.x & 63) == 0 && min_ord) {
+    atomicMin(min_ord, mn);
+    atomicMax(max_ord, mx);
+    atomicMax(max_key, mk);
   }
   __syncthreads();
   for (int i = threadIdx.x; i < WV_H1_BINS; i += WV_BLOCK)
-    if (h[i]) atomicAdd(&st->hist[i], h[i]);
+    if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
 hipError_t stats_compute(const float *coef, int64_t n, void *stats,
                          hipStream_t s) {
   hipError_t e = stats_clear(stats, s);
   if (e != hipSuccess) return e;
-  int64_t blocks = (n + WV_BLOCK * 16 - 1) / (WV_BLOCK * 16);
+  WvStats *st = (WvStats *)stats;
+  int64_t blocks = (n + WV_BLOCK * 8 - 1) / (WV_BLOCK * 8);
   if (blocks > 4096) blocks = 4096;
-  if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(stats_kernel, dim3((unsigned)blocks), dim3(WV_BLOCK), 0, s,
-                     coef, n, (WvStats *)stats);
+  hipLaunchKernelGGL(hist1_kernel, dim3((unsigned)blocks), dim3(WV_BLOCK), 0, s,
+                     coef, n, st->hist, &st->min_ord, &st->max_ord, &st->max_key,
+                     (const int32_t *)nullptr);
   return hipGetLastError();
 }
 
-// Block-wide search: given per-bin counts (nb bins) find the bin b with
-// cum(b) <= rank < cum(b) + cnt(b).  1024 threads.
+// Block-wide search (1024 threads): the bin b of cnt[0..NB) with
+// cum(b) <= rank < cum(b) + cnt(b).  Writes *out_b / *out_rank.
 template <int NB>
 __device__ void find_bucket(const uint32_t *cnt, uint32_t rank, uint32_t *out_b,
                             uint32_t *out_rank) {
@@ -108,8 +154,7 @@ __device__ void find_bucket(const uint32_t *cnt, uint32_t rank, uint32_t *out_b,
   }
   part[t] = sum;
   __syncthreads();
-  // inclusive scan (Hillis-Steele)
-  for (int off = 1; off < 1024; off <<= 1) {
+  for (int off = 1; off < 1024; off <<= 1) {   // inclusive scan
     const uint32_t v = t >= off ? part[t - off] : 0;
     __syncthreads();
     part[t] += v;
@@ -124,43 +169,243 @@ __device__ void find_bucket(const uint32_t *cnt, uint32_t rank, uint32_t *out_b,
     }
     before += local[k];
   }
+  __syncthreads();
 }
 
+// ---- 1. sample -------------------------------------------------------------
 __global__ void __launch_bounds__(1024)
-sel_scan1(const WvStats *st, int64_t n, float frac, SelState *S) {
-  __shared__ uint32_t b, r;
+sel_sample(const float *__restrict__ coef, int64_t n, float frac, SelState *S) {
+  __shared__ uint32_t keys[SEL_SAMPLE];
+  __shared__ uint32_t h[WV_H1_BINS];
+  __shared__ uint32_t res_b[2], res_r[2];
+  const int t = threadIdx.x;
   // (int)(compRatio * count) evaluated in float (thresh_cpu.cc:96)
   const float prod = frac * (float)n;
   int64_t idx = (int64_t)prod;
-  if (threadIdx.x == 0) {
-    S->special = (frac >= 1.0f || idx >= n) ? 1 : 0;
-    S->idx = (int32_t)(idx >= n ? n : idx);
-    S->ncand = 0;
-    b = 0;
-    r = 0;
+  if (idx < 0) idx = 0;
+  const bool special = frac >= 1.0f || idx >= n;
+  // sample: all keys when n is small, else 1024 chunks of 16
+  const int m = n <= SEL_SAMPLE ? (int)n : SEL_SAMPLE;
+  if (n <= SEL_SAMPLE) {
+    for (int i = t; i < m; i += 1024)
+      keys[i] = __builtin_bit_cast(uint32_t, coef[i]) & 0x7fffffffu;
+  } else {
+    const int nch = SEL_SAMPLE / SEL_SCHUNK;
+    for (int i = t; i < m; i += 1024) {
+      const int c = i / SEL_SCHUNK, k = i % SEL_SCHUNK;
+      const int64_t start = (int64_t)c * (n - SEL_SCHUNK) / (nch - 1);
+      keys[i] = __builtin_bit_cast(uint32_t, coef[start + k]) & 0x7fffffffu;
+    }
   }
-  for (int i = threadIdx.x; i < WV_H2_BINS; i += 1024) S->hist2[i] = 0;
-  for (int i = threadIdx.x; i < WV_H3_BINS; i += 1024) S->hist3[i] = 0;
+  for (int i = t; i < WV_H1_BINS; i += 1024) h[i] = 0;
+  if (t < 2) { res_b[t] = 0; res_r[t] = 0; }
   __syncthreads();
-  if (idx >= n) idx = n - 1;
-  find_bucket<WV_H1_BINS>(st->hist, (uint32_t)idx, &b, &r);
+  for (int i = t; i < m; i += 1024) atomicAdd(&h[keys[i] >> WV_H1_SHIFT], 1u);
   __syncthreads();
-  if (threadIdx.x == 0) {
-    S->b1 = b;
-    S->rank = r;
+  // sample ranks of the window ends
+  const float p = (float)idx / (float)n;
+  const float plo = p - SEL_MARGIN, phi = p + SEL_MARGIN;
+  const int r_lo = plo <= 0.f ? -1 : (int)(plo * m);
+  const int r_hi = phi >= 1.f ? m : min(m - 1, (int)(phi * m) + 1);
+  uint32_t lo_b = 0, lo_r = 0, hi_b = 0, hi_r = 0;
+  if (r_lo >= 0) {
+    find_bucket<WV_H1_BINS>(h, (uint32_t)r_lo, &res_b[0], &res_r[0]);
+    lo_b = res_b[0]; lo_r = res_r[0];
+  }
+  if (r_hi < m) {
+    find_bucket<WV_H1_BINS>(h, (uint32_t)r_hi, &res_b[1], &res_r[1]);
+    hi_b = res_b[1]; hi_r = res_r[1];
+  }
+  // refine each end to a digit-2 sub-bucket with the samples of its bucket
+  uint32_t key_lo = 0, key_hi = 0x7fffffffu;
+  for (int end = 0; end < 2; end++) {
+    const bool use = end == 0 ? r_lo >= 0 : r_hi < m;
+    if (!use) continue;
+    const uint32_t b = end == 0 ? lo_b : hi_b, r = end == 0 ? lo_r : hi_r;
+    __syncthreads();
+    for (int i = t; i < WV_H2_BINS; i += 1024) h[i] = 0;
+    __syncthreads();
+    for (int i = t; i < m; i += 1024)
+      if ((keys[i] >> WV_H1_SHIFT) == b)
+        atomicAdd(&h[(keys[i] >> WV_H2_SHIFT) & (WV_H2_BINS - 1)], 1u);
+    __syncthreads();
+    find_bucket<WV_H2_BINS>(h, r, &res_b[end], &res_r[end]);
+    const uint32_t sub = res_b[end];
+    if (end == 0) key_lo = (b << WV_H1_SHIFT) | (sub << WV_H2_SHIFT);
+    else key_hi = (b << WV_H1_SHIFT) | (sub << WV_H2_SHIFT) | ((1u << WV_H2_SHIFT) - 1);
+  }
+  for (int i = t; i < WV_H1_BINS; i += 1024) S->hist1[i] = 0;
+  for (int i = t; i < WV_H2_BINS; i += 1024) S->hist2[i] = 0;
+  for (int i = t; i < WV_H3_BINS; i += 1024) S->hist3[i] = 0;
+  if (t == 0) {
+    S->idx = (int32_t)(special ? n : idx);
+    S->special = special;
+    S->key_lo = key_lo;
+    S->key_hi = key_hi;
+    S->below = 0;
+    S->ncand = 0;
+    S->fallback = 0;
+    S->crank = 0;
+    S->b1 = S->b2 = S->b3 = 0;
+    S->min_ord = 0xffffffffu;
+    S->max_ord = 0u;
+    S->max_key = 0u;
   }
 }
 
+// ---- 2. collect ------------------------------------------------------------
+// SEL_GRID blocks, each streaming a contiguous range in sub-chunks of
+// SEL_CHUNK keys (16-byte loads, 16 keys per thread).  Per-block results go
+// to S->part (no same-address global atomics); the candidate list is
+// reserved with one atomic per LDS-buffer flush.
 __global__ void __launch_bounds__(WV_BLOCK)
 sel_collect(const float *__restrict__ coef, int64_t n, SelState *S,
             uint32_t *cand) {
-  __shared__ uint32_t h[WV_H2_BINS];
-  for (int i = threadIdx.x; i < WV_H2_BINS; i += WV_BLOCK) h[i] = 0;
+  __shared__ uint32_t buf[SEL_CAND_CAP];
+  __shared__ uint32_t cnt, base;
+  __shared__ uint32_t red[4][WV_BLOCK / 64];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (tid == 0) cnt = 0;
   __syncthreads();
-  const uint32_t b1 = S->b1;
+  const uint32_t klo = S->key_lo, khi = S->key_hi;
+  const int64_t nchunk = (n + SEL_CHUNK - 1) / SEL_CHUNK;
+  const int64_t c0 = nchunk * blockIdx.x / gridDim.x;
+  const int64_t c1 = nchunk * (blockIdx.x + 1) / gridDim.x;
+  uint32_t nb = 0, mn = 0xffffffffu, mx = 0, mk = 0;
+  for (int64_t c = c0; c < c1; c++) {
+    const int64_t start = c * SEL_CHUNK;
+    float v[SEL_ITEMS];
+    const bool full = start + SEL_CHUNK <= n;
+    if (full) {
+      const float4 *c4 = (const float4 *)(coef + start);
+#pragma unroll
+      for (int u = 0; u < SEL_ITEMS / 4; u++) {
+        const float4 q = c4[tid + u * WV_BLOCK];
+        v[4 * u] = q.x; v[4 * u + 1] = q.y; v[4 * u + 2] = q.z; v[4 * u + 3] = q.w;
+      }
+    } else {
+      for (int u = 0; u < SEL_ITEMS; u++) {
+        const int64_t i = start + (int64_t)(u >> 2) * 4 * WV_BLOCK + 4 * tid + (u & 3);
+        v[u] = i < n ? coef[i] : __builtin_bit_cast(float, 0xffffffffu);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < SEL_ITEMS; u++) {
+      const uint32_t bits = __builtin_bit_cast(uint32_t, v[u]);
+      const bool ok = bits != 0xffffffffu;     // this NaN pattern marks the tail
+      const uint32_t key = bits & 0x7fffffffu;
+      if (ok) {
+        const uint32_t o = wv_f2ord(v[u]);
+        mn = min(mn, o);
+        mx = max(mx, o);
+        mk = max(mk, key);
+        nb += key < klo;
+      }
+      const bool hit = ok && key >= klo && key <= khi;
+      const uint64_t m = __ballot(hit);
+      if (m) {
+        uint32_t off = 0;
+        if (lane == 0) off = atomicAdd(&cnt, (uint32_t)__popcll(m));
+        off = __shfl(off, 0);
+        if (hit) buf[off + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = key;
+      }
+    }
+    __syncthreads();
+    // flush when the next sub-chunk might not fit
+    if (cnt > SEL_CAND_CAP - SEL_CHUNK || c == c1 - 1) {
+      const uint32_t k = cnt;
+      if (tid == 0 && k) base = atomicAdd(&S->ncand, k);
+      __syncthreads();
+      for (uint32_t i = tid; i < k; i += WV_BLOCK) cand[base + i] = buf[i];
+      __syncthreads();
+      if (tid == 0) cnt = 0;
+      __syncthreads();
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    nb += (uint32_t)__shfl_xor((int)nb, off);
+    mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
+    mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
+    mk = max(mk, (uint32_t)__shfl_xor((int)mk, off));
+  }
+  if (lane == 0) { red[0][wave] = nb; red[1][wave] = mn; red[2][wave] = mx; red[3][wave] = mk; }
+  __syncthreads();
+  if (tid == 0) {
+    for (int w = 1; w < WV_BLOCK / 64; w++) {
+      nb += red[0][w];
+      mn = min(mn, red[1][w]);
+      mx = max(mx, red[2][w]);
+      mk = max(mk, red[3][w]);
+    }
+    S->part[blockIdx.x][0] = nb;
+    S->part[blockIdx.x][1] = mn;
+    S->part[blockIdx.x][2] = mx;
+    S->part[blockIdx.x][3] = mk;
+  }
+}
+
+// ---- 3. verify -------------------------------------------------------------
+__global__ void __launch_bounds__(1024) sel_verify(SelState *S, int nparts) {
+  __shared__ uint32_t r[4][16];
+  const int t = threadIdx.x;
+  uint32_t nb = 0, mn = 0xffffffffu, mx = 0, mk = 0;
+  for (int i = t; i < nparts; i += 1024) {
+    nb += S->part[i][0];
+    mn = min(mn, S->part[i][1]);
+    mx = max(mx, S->part[i][2]);
+    mk = max(mk, S->part[i][3]);
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    nb += (uint32_t)__shfl_xor((int)nb, off);
+    mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
+    mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
+    mk = max(mk, (uint32_t)__shfl_xor((int)mk, off));
+  }
+  if ((t & 63) == 0) { r[0][t >> 6] = nb; r[1][t >> 6] = mn; r[2][t >> 6] = mx; r[3][t >> 6] = mk; }
+  __syncthreads();
+  if (t != 0) return;
+  for (int w = 1; w < 16; w++) {
+    nb += r[0][w];
+    mn = min(mn, r[1][w]);
+    mx = max(mx, r[2][w]);
+    mk = max(mk, r[3][w]);
+  }
+  S->below = nb;
+  S->min_ord = mn;
+  S->max_ord = mx;
+  S->max_key = mk;
+  if (S->special) { S->crank = 0; return; }
+  const uint32_t idx = (uint32_t)S->idx;
+  if (S->below <= idx && idx < S->below + S->ncand) {
+    S->crank = idx - S->below;
+  } else {
+    S->fallback = 1;
+    S->ncand = 0;
+  }
+}
+
+// ---- fallback: exact bucket of the rank from a full histogram --------------
+__global__ void __launch_bounds__(1024) sel_fb_scan(SelState *S) {
+  if (!S->fallback) return;
+  __shared__ uint32_t b, r;
+  if (threadIdx.x == 0) { b = 0; r = 0; }
+  __syncthreads();
+  find_bucket<WV_H1_BINS>(S->hist1, (uint32_t)S->idx, &b, &r);
+  if (threadIdx.x == 0) {
+    S->key_lo = b << WV_H1_SHIFT;
+    S->key_hi = (b << WV_H1_SHIFT) | ((1u << WV_H1_SHIFT) - 1);
+    S->crank = r;
+  }
+  for (int i = threadIdx.x; i < WV_H1_BINS; i += 1024) S->hist1[i] = 0;
+}
+
+__global__ void __launch_bounds__(WV_BLOCK)
+sel_fb_collect(const float *__restrict__ coef, int64_t n, SelState *S,
+               uint32_t *cand) {
+  if (!S->fallback) return;
+  const uint32_t klo = S->key_lo, khi = S->key_hi;
   const int lane = threadIdx.x & 63;
   const int64_t stride = (int64_t)gridDim.x * WV_BLOCK;
-  // whole waves iterate together so the ballot covers all lanes
   const int64_t nround = (n + stride - 1) / stride;
   for (int64_t r = 0; r < nround; r++) {
     const int64_t i = r * stride + (int64_t)blockIdx.x * WV_BLOCK + threadIdx.x;
@@ -168,108 +413,143 @@ sel_collect(const float *__restrict__ coef, int64_t n, SelState *S,
     bool hit = false;
     if (i < n) {
       key = __builtin_bit_cast(uint32_t, coef[i]) & 0x7fffffffu;
-      hit = (key >> WV_H1_SHIFT) == b1;
+      hit = key >= klo && key <= khi;
     }
     const uint64_t m = __ballot(hit);
     if (m) {
       uint32_t base = 0;
       if (lane == 0) base = atomicAdd(&S->ncand, (uint32_t)__popcll(m));
       base = __shfl(base, 0);
-      if (hit) {
-        const uint32_t off = (uint32_t)__popcll(m & ((1ull << lane) - 1));
-        cand[base + off] = key;
-        atomicAdd(&h[(key >> WV_H2_SHIFT) & (WV_H2_BINS - 1)], 1u);
-      }
+      if (hit) cand[base + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = key;
     }
   }
-  __syncthreads();
-  for (int i = threadIdx.x; i < WV_H2_BINS; i += WV_BLOCK)
-    if (h[i]) atomicAdd(&S->hist2[i], h[i]);
 }
 
-__global__ void __launch_bounds__(1024) sel_scan2(SelState *S) {
-  __shared__ uint32_t b, r;
-  if (threadIdx.x == 0) { b = 0; r = 0; }
-  __syncthreads();
-  find_bucket<WV_H2_BINS>(S->hist2, S->rank, &b, &r);
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    S->b2 = b;
-    S->rank = r;
-  }
-}
-
+// ---- 4. radix select over the candidates -----------------------------------
+// digit d of the candidates whose higher digits match; few blocks, so the
+// global histogram takes few atomics
+template <int DIGIT>
 __global__ void __launch_bounds__(WV_BLOCK)
-sel_hist3(const uint32_t *cand, SelState *S) {
-  __shared__ uint32_t h[WV_H3_BINS];
-  for (int i = threadIdx.x; i < WV_H3_BINS; i += WV_BLOCK) h[i] = 0;
+sel_chist(const uint32_t *cand, SelState *S) {
+  constexpr int NB = DIGIT == 3 ? WV_H3_BINS : 2048;
+  constexpr int BITS = DIGIT == 3 ? 9 : 11;
+  __shared__ uint32_t h[NB];
+  if (S->special) return;
+  const uint32_t nc = S->ncand;
+  const uint32_t per = (nc + gridDim.x - 1) / gridDim.x;
+  const uint32_t a = blockIdx.x * per, e = min(nc, a + per);
+  if (a >= e) return;
+  for (int i = threadIdx.x; i < NB; i += WV_BLOCK) h[i] = 0;
   __syncthreads();
-  const uint32_t nc = S->ncand, b2 = S->b2;
-  for (uint32_t i = blockIdx.x * WV_BLOCK + threadIdx.x; i < nc;
-       i += gridDim.x * WV_BLOCK) {
-    const uint32_t key = cand[i];
-    if (((key >> WV_H2_SHIFT) & (WV_H2_BINS - 1)) == b2)
-      atomicAdd(&h[key & (WV_H3_BINS - 1)], 1u);
+  const uint32_t b1 = S->b1, b2 = S->b2;
+  for (uint32_t i0 = a; i0 < e; i0 += WV_BLOCK * 4) {
+    uint32_t kk[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const uint32_t i = i0 + u * WV_BLOCK + threadIdx.x;
+      kk[u] = i < e ? cand[i] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const uint32_t i = i0 + u * WV_BLOCK + threadIdx.x;
+      bool ok = i < e;
+      const uint32_t key = kk[u];
+      uint32_t d;
+      if (DIGIT == 1) {
+        d = key >> WV_H1_SHIFT;
+      } else if (DIGIT == 2) {
+        ok = ok && (key >> WV_H1_SHIFT) == b1;
+        d = (key >> WV_H2_SHIFT) & (WV_H2_BINS - 1);
+      } else {
+        ok = ok && (key >> WV_H1_SHIFT) == b1 &&
+             ((key >> WV_H2_SHIFT) & (WV_H2_BINS - 1)) == b2;
+        d = key & (WV_H3_BINS - 1);
+      }
+      wv_wave_hist_add<BITS>(h, d, ok);
+    }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < WV_H3_BINS; i += WV_BLOCK)
-    if (h[i]) atomicAdd(&S->hist3[i], h[i]);
+  uint32_t *gh = DIGIT == 1 ? S->hist1 : DIGIT == 2 ? S->hist2 : S->hist3;
+  for (int i = threadIdx.x; i < NB; i += WV_BLOCK)
+    if (h[i]) atomicAdd(&gh[i], h[i]);
 }
 
-__global__ void __launch_bounds__(1024)
-sel_scan3(const WvStats *st, int64_t n, SelState *S, wv_thresh_result *res) {
+template <int DIGIT>
+__global__ void __launch_bounds__(1024) sel_cscan(SelState *S) {
+  if (S->special) return;
+  constexpr int NB = DIGIT == 3 ? WV_H3_BINS : 2048;
   __shared__ uint32_t b, r;
   if (threadIdx.x == 0) { b = 0; r = 0; }
   __syncthreads();
-  find_bucket<WV_H3_BINS>(S->hist3, S->rank, &b, &r);
-  __syncthreads();
+  const uint32_t *gh = DIGIT == 1 ? S->hist1 : DIGIT == 2 ? S->hist2 : S->hist3;
+  find_bucket<NB>(gh, S->crank, &b, &r);
   if (threadIdx.x == 0) {
-    const uint32_t key = (S->b1 << WV_H1_SHIFT) | (S->b2 << WV_H2_SHIFT) | b;
-    float t = __builtin_bit_cast(float, key);
-    wv_thresh_result o;
-    // Traverse starts min at FLT_MAX and max at FLT_MIN (thresh_cpu.cc:67-68)
-    float mn = wv_ord2f(st->min_ord), mx = wv_ord2f(st->max_ord);
-    const float fmax = 3.40282347e+38f, fmin = 1.17549435e-38f;
-    o.min_val = mn < fmax ? mn : fmax;
-    o.max_val = mx > fmin ? mx : fmin;
-    o.max_abs = __builtin_bit_cast(float, st->max_key);
-    if (S->special) {
-      // frac >= 1: the reference GPU driver takes max |x|
-      // (test_compress_gpu.cu:299-301)
-      o.threshold = o.max_abs;
-      o.threshold_key = st->max_key;
-      o.thresh_idx = (int32_t)n;
-      o.nonzero_count = 0;
-    } else {
-      o.threshold_key = key;
-      if (t == 0.f) t = 1e-16f;    // thresh_cpu.cc:98
-      o.threshold = t;
-      o.thresh_idx = S->idx;
-      o.nonzero_count = (int32_t)(n - S->idx - 1);
-    }
-    o.n_candidates = (int32_t)S->ncand;
-    *res = o;
+    if (DIGIT == 1) S->b1 = b;
+    else if (DIGIT == 2) S->b2 = b;
+    else S->b3 = b;
+    S->crank = r;
   }
+}
+
+__global__ void sel_result(int64_t n, const SelState *S, wv_thresh_result *res) {
+  if (threadIdx.x != 0) return;
+  wv_thresh_result o;
+  // Traverse starts min at FLT_MAX and max at FLT_MIN (thresh_cpu.cc:67-68)
+  const float mn = wv_ord2f(S->min_ord), mx = wv_ord2f(S->max_ord);
+  const float fmax = 3.40282347e+38f, fmin = 1.17549435e-38f;
+  o.min_val = mn < fmax ? mn : fmax;
+  o.max_val = mx > fmin ? mx : fmin;
+  o.max_abs = __builtin_bit_cast(float, S->max_key);
+  if (S->special) {
+    // frac >= 1: the reference GPU driver takes max |x|
+    // (test_compress_gpu.cu:299-301)
+    o.threshold = o.max_abs;
+    o.threshold_key = S->max_key;
+    o.thresh_idx = (int32_t)n;
+    o.nonzero_count = 0;
+  } else {
+    const uint32_t key = (S->b1 << WV_H1_SHIFT) | (S->b2 << WV_H2_SHIFT) | S->b3;
+    float t = __builtin_bit_cast(float, key);
+    o.threshold_key = key;
+    if (t == 0.f) t = 1e-16f;    // thresh_cpu.cc:98
+    o.threshold = t;
+    o.thresh_idx = S->idx;
+    o.nonzero_count = (int32_t)(n - S->idx - 1);
+  }
+  // candidates examined; the top bit records an exact-fallback run
+  o.n_candidates = (int32_t)(S->ncand & 0x7fffffffu) |
+                   (S->fallback ? (int32_t)0x80000000 : 0);
+  *res = o;
 }
 
 hipError_t threshold_select(const float *coef, int64_t n, float frac,
                             const void *stats, void *ws,
                             wv_thresh_result *res, hipStream_t s) {
+  (void)stats;
   SelState *S = (SelState *)ws;
   uint32_t *cand = (uint32_t *)((char *)ws + ((sizeof(SelState) + 255) & ~(size_t)255));
-  const WvStats *st = (const WvStats *)stats;
-  hipLaunchKernelGGL(sel_scan1, dim3(1), dim3(1024), 0, s, st, n, frac, S);
-  int64_t blocks = (n + WV_BLOCK * 8 - 1) / (WV_BLOCK * 8);
-  if (blocks > 8192) blocks = 8192;
-  if (blocks < 1) blocks = 1;
-  hipLaunchKernelGGL(sel_collect, dim3((unsigned)blocks), dim3(WV_BLOCK), 0, s,
-                     coef, n, S, cand);
-  hipLaunchKernelGGL(sel_scan2, dim3(1), dim3(1024), 0, s, S);
-  int64_t b3 = (n / 64 + WV_BLOCK - 1) / WV_BLOCK;
-  if (b3 > 1024) b3 = 1024;
-  if (b3 < 1) b3 = 1;
-  hipLaunchKernelGGL(sel_hist3, dim3((unsigned)b3), dim3(WV_BLOCK), 0, s, cand, S);
-  hipLaunchKernelGGL(sel_scan3, dim3(1), dim3(1024), 0, s, st, n, S, res);
+  hipLaunchKernelGGL(sel_sample, dim3(1), dim3(1024), 0, s, coef, n, frac, S);
+  const int64_t nchunk = (n + SEL_CHUNK - 1) / SEL_CHUNK;
+  const int grid = (int)(nchunk < SEL_GRID ? nchunk : SEL_GRID);
+  hipLaunchKernelGGL(sel_collect, dim3(grid), dim3(WV_BLOCK), 0, s, coef, n, S, cand);
+  hipLaunchKernelGGL(sel_verify, dim3(1), dim3(1024), 0, s, S, grid);
+  // exact fallback (returns at once unless the window missed the rank)
+  int64_t fb = (n + WV_BLOCK * 8 - 1) / (WV_BLOCK * 8);
+  if (fb > SEL_FB_GRID) fb = SEL_FB_GRID;
+  hipLaunchKernelGGL(hist1_kernel, dim3((unsigned)fb), dim3(WV_BLOCK), 0, s, coef, n,
+                     S->hist1, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                     (uint32_t *)nullptr, (const int32_t *)&S->fallback);
+  hipLaunchKernelGGL(sel_fb_scan, dim3(1), dim3(1024), 0, s, S);
+  hipLaunchKernelGGL(sel_fb_collect, dim3((unsigned)fb), dim3(WV_BLOCK), 0, s, coef, n, S,
+                     cand);
+  // radix select over the candidates
+  hipLaunchKernelGGL(sel_chist<1>, dim3(256), dim3(WV_BLOCK), 0, s, cand, S);
+  hipLaunchKernelGGL(sel_cscan<1>, dim3(1), dim3(1024), 0, s, S);
+  hipLaunchKernelGGL(sel_chist<2>, dim3(256), dim3(WV_BLOCK), 0, s, cand, S);
+  hipLaunchKernelGGL(sel_cscan<2>, dim3(1), dim3(1024), 0, s, S);
+  hipLaunchKernelGGL(sel_chist<3>, dim3(256), dim3(WV_BLOCK), 0, s, cand, S);
+  hipLaunchKernelGGL(sel_cscan<3>, dim3(1), dim3(1024), 0, s, S);
+  hipLaunchKernelGGL(sel_result, dim3(1), dim3(64), 0, s, n, S, res);
   return hipGetLastError();
 }
 
