@@ -4,8 +4,13 @@
 #include <stdint.h>
 
 #define WV_BLOCK 256        // threads per block of the streaming kernels
-#define WV_DBLOCK 256       // threads per block of the DWT tile kernel
-#define WV_MAXP 32          // max (low,high) pairs a thread owns per level
+#define WV_DBLOCK 512       // threads per block of the DWT tile kernel
+#define WV_MAXP 16          // max (low,high) pairs a thread owns per level
+// waves per SIMD the DWT kernel is compiled for (the launch-bounds second
+// argument is the waves-per-EU hint on AMDGPU): 4 caps it at 128 VGPRs
+#ifndef WV_DWAVES
+#define WV_DWAVES 4
+#endif
 #define WV_H1_BITS 11       // threshold select digit 1: bits 30..20 of |x|
 #define WV_H1_BINS 2048
 #define WV_H1_SHIFT 20

@@ -2,26 +2,37 @@
 // gfx950.
 //
 // One kernel, instantiated per (wavelet, direction, source dtype, clamping,
-// tile width), processes a *line tile*: T lines of one axis, each of length
-// L (the padded axis length).  The block
-//   1. loads the tile into LDS as tile[pos][line] (row stride T+1 so both
-//      line-contiguous and position-contiguous accesses are bank-conflict
-//      free), converting u8/i16/i32 to float and REPEAT-padding by clamping
-//      coordinates into the unpadded source; every thread keeps WV_UNR
-//      independent loads in flight,
+// tile width), processes *line tiles*: T lines of one axis, each of length
+// L (the padded axis length), held in LDS as tile[row][line] with row
+// stride T+1 (line-contiguous and position-contiguous accesses are both
+// bank-conflict free) and 4 spare rows before and after the line.
+//
+// The grid is persistent: block b handles tiles b, b+G, b+2G, ...  For
+// each tile the block
+//   1. moves the elements it prefetched into registers to LDS (u8/i16/i32
+//      converted to float; REPEAT padding by clamping coordinates into the
+//      unpadded source), then issues the loads of the NEXT tile into the
+//      same registers -- they stay in flight through steps 2 and 3, since
+//      barriers only wait for LDS traffic,
 //   2. runs every level of the 1-D transform in LDS: each thread owns up to
 //      WV_MAXP consecutive (low, high) output pairs of one line and walks a
-//      9-value register window over its inputs (mirroring only in the
-//      chunks that touch a line end), then all threads write back after a
+//      9-value register window over them; all threads write back after a
 //      barrier,
-//   3. stores the tile either back in x,y,z layout or, for the z pass, in the
+//   3. stores the tile back in x,y,z layout or, for the z pass, in the
 //      reference's rotated z,x,y output layout.
-// Every global access of a tile is either a contiguous block of the tile or
-// T-float segments (128 B at T = 32), so each pass moves 8 B per element
-// (the input's own size plus 4 B on the first pass).
+// Global accesses are either whole lines (consecutive lanes on consecutive
+// positions) or T-float segments across lines, so a pass moves 8 B per
+// element (the input's own size plus 4 B on the first pass).  Elements past
+// the line end and lines past the volume edge are clamped onto the last
+// valid ones, whose values they then duplicate: every load and store is
+// unconditional and the memory phases carry no branches.
 //
-// With the tile width a compile-time constant, every LDS address inside a
-// line chunk is an immediate offset from one base register.
+// Forward CDF 9/7 levels use a gap layout: the high band of a level is
+// written 4 rows after the end of its low band, so rows [n, n+3) past a
+// level's input are free, and the threads at the two line ends write the
+// mirrored values there (and at rows -4..-1) before reading them.  Every
+// chunk then reads plain rows -- no per-element mirroring -- and the final
+// coefficients sit at row p (p < final low length) or p + 4.
 //
 // Arithmetic follows the reference CPU code operation by operation with
 // contraction disabled (-ffp-contract=off), so coefficients are
@@ -45,37 +56,76 @@ constexpr float H0 = -.85269867900940f, H1 = .418092273222210f,
                 H4 = -.037828455506995f;
 constexpr float INV_SQRT2 = 0.70710678118654752440f;   // dwt_cpu.cc:11
 
-#define WV_UNR 32  // independent loads / stores in flight per thread
+#define WV_EPT (2 * WV_MAXP)   // tile elements per thread: L*T <= WV_DBLOCK*WV_EPT
+#define WV_HALO 4              // spare rows before and after a line
 
 template <typename S> __device__ inline float to_f(S v) { return (float)v; }
 
+// An int behind an empty asm statement.  Values derived from it are
+// recomputed where they are used instead of being hoisted out of the tile
+// loop and held in registers (or spilled) for its whole length.
+__device__ inline int opaque(int v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
+__device__ inline int opaque_tid() { return opaque((int)threadIdx.x); }
+
+// ---- tiles ----------------------------------------------------------------
 struct Tile {
-  int nvalid;      // valid lines in this tile
+  int nvalid;      // valid lines in this tile (>= 1)
   int a, b;        // axis0: first row; axis1: x0, z; axis2: x0, y
 };
 
-__device__ inline Tile decode_tile(const DwtPass &P, int T) {
+// Position of a block in its tile sequence: tile index and, for the y / z
+// passes, (x tile, other coordinate), advanced without divisions.
+struct Cursor {
+  int tile, xt, ob;
+};
+
+__device__ inline Cursor cursor_at(const DwtPass &P, int tile) {
+  Cursor c;
+  c.tile = tile;
+  c.xt = 0;
+  c.ob = 0;
+  if (P.axis != 0) {
+    c.xt = tile % P.ntiles_x;
+    c.ob = tile / P.ntiles_x;
+  }
+  return c;
+}
+
+struct Step {
+  int g, gx, gb;   // grid size and its split into (x tiles, rows)
+};
+
+__device__ inline void cursor_advance(const DwtPass &P, const Step &st, Cursor &c) {
+  c.tile += st.g;
+  if (P.axis != 0) {
+    c.xt += st.gx;
+    c.ob += st.gb;
+    if (c.xt >= P.ntiles_x) {
+      c.xt -= P.ntiles_x;
+      c.ob++;
+    }
+  }
+}
+
+__device__ inline Tile cursor_tile(const DwtPass &P, int T, const Cursor &c) {
   Tile t;
-  const int tile = blockIdx.x;
   if (P.axis == 0) {
-    const int rows = P.Y * P.Z;
-    t.a = tile * T;
+    t.a = c.tile * T;
     t.b = 0;
-    t.nvalid = min(T, rows - t.a);
+    t.nvalid = min(T, P.Y * P.Z - t.a);
   } else {
-    const int xt = tile % P.ntiles_x;
-    t.a = xt * T;
-    t.b = tile / P.ntiles_x;
+    t.a = c.xt * T;
+    t.b = c.ob;
     t.nvalid = min(T, P.X - t.a);
   }
   return t;
 }
 
-// ---- tile addressing -------------------------------------------------------
-// A load/store walks the tile either line-major (consecutive threads step
-// along a line: x lines, and z lines of the rotated z,x,y layout) or
-// position-major (consecutive threads step across lines: y and z lines of
-// the x,y,z layout, where T neighbouring lines are T neighbouring x).
+// ---- line-major addressing -------------------------------------------------
+// x lines, and z lines of the rotated z,x,y layout: a line is contiguous.
 __device__ inline bool line_major_load(const DwtPass &P) {
   return P.axis == 0 || (P.axis == 2 && P.load_zxy);
 }
@@ -83,12 +133,15 @@ __device__ inline bool line_major_store(const DwtPass &P) {
   return P.axis == 0 || (P.axis == 2 && P.store_zxy);
 }
 
-// global offset of (line l, position 0) for a line-major access
+// global offset of (line l, position 0); lines past the edge map onto the
+// last valid line
 template <bool CLAMP>
 __device__ inline size_t line_base_load(const DwtPass &P, const Tile &t, int l) {
+  l = min(l, t.nvalid - 1);
   if (P.axis == 0) {
     const int r = t.a + l;
-    int y = r % P.Y, z = r / P.Y;
+    const int Yo = opaque(P.Y);
+    int y = r % Yo, z = r / Yo;
     if (CLAMP) {
       y = min(y, P.oy - 1);
       z = min(z, P.oz - 1);
@@ -99,89 +152,185 @@ __device__ inline size_t line_base_load(const DwtPass &P, const Tile &t, int l) 
   return ((size_t)t.b * P.X + (t.a + l)) * P.Z;     // z,x,y layout
 }
 __device__ inline size_t line_base_store(const DwtPass &P, const Tile &t, int l) {
+  l = min(l, t.nvalid - 1);
   if (P.axis == 0) return (size_t)(t.a + l) * P.X;
   return ((size_t)t.b * P.X + (t.a + l)) * P.Z;
 }
 
-// global offset of (line 0, position p) for a position-major access; the
-// line index is added as +x (clamped for the unpadded source)
+// ---- position-major addressing ---------------------------------------------
+// y lines and z lines of the x,y,z layout: T neighbouring lines are T
+// neighbouring x.  Element (line l, position p) sits at
+// rowbase + min(p, pcap) * pstride + min(a + l, xcap).
+struct PosMap {
+  size_t rowbase, pstride;
+  int pcap, xcap;
+};
+
 template <bool CLAMP>
-__device__ inline size_t pos_base_load(const DwtPass &P, const Tile &t, int p) {
+__device__ inline PosMap pos_map_load(const DwtPass &P, const Tile &t) {
+  PosMap m;
+  const int xlast = t.a + t.nvalid - 1;
   if (CLAMP) {
+    m.xcap = min(xlast, P.ox - 1);
     if (P.axis == 1) {
-      const int z = min(t.b, P.oz - 1), y = min(p, P.oy - 1);
-      return ((size_t)z * P.oy + y) * P.ox;
+      m.rowbase = (size_t)min(t.b, P.oz - 1) * P.oy * P.ox;
+      m.pstride = P.ox;
+      m.pcap = P.oy - 1;
+    } else {
+      m.rowbase = (size_t)min(t.b, P.oy - 1) * P.ox;
+      m.pstride = (size_t)P.oy * P.ox;
+      m.pcap = P.oz - 1;
     }
-    const int z = min(p, P.oz - 1), y = min(t.b, P.oy - 1);
-    return ((size_t)z * P.oy + y) * P.ox;
-  }
-  if (P.axis == 1) return ((size_t)t.b * P.Y + p) * P.X;
-  return ((size_t)p * P.Y + t.b) * P.X;
-}
-__device__ inline size_t pos_base_store(const DwtPass &P, const Tile &t, int p) {
-  if (P.axis == 1) return ((size_t)t.b * P.Y + p) * P.X;
-  return ((size_t)p * P.Y + t.b) * P.X;
-}
-
-// ---- forward levels -------------------------------------------------------
-template <bool MIRROR>
-__device__ inline float ldm(const float *col, int p, int n, int LS) {
-  if (MIRROR && (unsigned)p >= (unsigned)n) p = wv_mirror(p, n);
-  return col[p * LS];
-}
-
-// one thread's chunk of pairs [j0, j0+cnt) of one line at level length n
-template <bool MIRROR>
-__device__ inline void cdf97_chunk(const float *col, int LS, int n, int j0, int cnt,
-                                   float *lo, float *hi) {
-  const int c = 2 * j0;
-  const float *cc = col + c * LS;       // positions relative to c
-  float w0, w1, w2, w3, w4, w5, w6, w7, w8;
-  if (MIRROR) {
-    w0 = ldm<true>(col, c - 4, n, LS); w1 = ldm<true>(col, c - 3, n, LS);
-    w2 = ldm<true>(col, c - 2, n, LS); w3 = ldm<true>(col, c - 1, n, LS);
-    w4 = ldm<true>(col, c, n, LS);     w5 = ldm<true>(col, c + 1, n, LS);
-    w6 = ldm<true>(col, c + 2, n, LS); w7 = ldm<true>(col, c + 3, n, LS);
-    w8 = ldm<true>(col, c + 4, n, LS);
   } else {
-    w0 = cc[-4 * LS]; w1 = cc[-3 * LS]; w2 = cc[-2 * LS]; w3 = cc[-1 * LS];
-    w4 = cc[0];       w5 = cc[LS];      w6 = cc[2 * LS];  w7 = cc[3 * LS];
-    w8 = cc[4 * LS];
+    m.xcap = xlast;
+    m.pcap = P.L - 1;
+    if (P.axis == 1) {
+      m.rowbase = (size_t)t.b * P.Y * P.X;
+      m.pstride = P.X;
+    } else {
+      m.rowbase = (size_t)t.b * P.X;
+      m.pstride = (size_t)P.Y * P.X;
+    }
   }
+  return m;
+}
+__device__ inline PosMap pos_map_store(const DwtPass &P, const Tile &t) {
+  PosMap m;
+  m.xcap = t.a + t.nvalid - 1;
+  m.pcap = P.L - 1;
+  if (P.axis == 1) {
+    m.rowbase = (size_t)t.b * P.Y * P.X;
+    m.pstride = P.X;
+  } else {
+    m.rowbase = (size_t)t.b * P.X;
+    m.pstride = (size_t)P.Y * P.X;
+  }
+  return m;
+}
+
+// ---- element <-> thread mappings ------------------------------------------
+// Line-major, compile-time tile width TT >= 8 and L a multiple of 64: wave w
+// owns lines w*TT/8 .. w*TT/8 + TT/8 - 1; element u of a lane is line
+// w*TT/8 + u % (TT/8) at position (u / (TT/8)) * 64 + lane.
+// Line-major, generic: element u is flat index tid + u*WV_DBLOCK of the
+// line-major tile (division by L through magic_L).
+// Position-major: element u is line tid % T at position
+// tid / T + u * (WV_DBLOCK / T).
+template <int TT>
+__device__ inline void lm_index(const DwtPass &P, int tid, int u, int &l, int &p) {
+  if (TT >= 8) {
+    constexpr int LPW = TT >= 8 ? TT / 8 : 1;
+    l = (tid >> 6) * LPW + u % LPW;
+    p = (u / LPW) * 64 + (tid & 63);
+  } else {
+    const int i = min(tid + u * WV_DBLOCK, P.T * P.L - 1);
+    l = (int)__umulhi((unsigned)i, P.magic_L);
+    p = i - l * P.L;
+  }
+}
+
+template <int TT>
+__device__ inline void pm_index(const DwtPass &P, int tid, int u, int &l, int &p) {
+  if (TT > 0) {
+    l = tid % TT;
+    p = tid / TT + u * (WV_DBLOCK / TT);
+  } else {
+    l = tid & (P.T - 1);
+    p = (tid >> P.log2T) + (u << (P.log2dblock_T));
+  }
+}
+
+// Issues the loads of this thread's elements of tile t into v[] without
+// waiting for them.
+template <typename SRC, bool CLAMP, int TT>
+__device__ inline void load_regs(const DwtPass &P, const Tile &t, const size_t *lb,
+                                 float (&v)[WV_EPT]) {
+  const int tid = opaque_tid();
+  const SRC *src = (const SRC *)P.src;
+  if (line_major_load(P)) {
+    const int pmax = (CLAMP && P.axis == 0) ? P.ox - 1 : P.L - 1;
 #pragma unroll
-  for (int q = 0; q < WV_MAXP; q++) {
-    if (q < cnt) {
-      // cdf97_tmpl dwt_cpu.cc:709-723: centre tap first, then
-      // F[k]*(a[i+k]+a[i-k]) for k = 1..4, no contraction
-      float sl = F0 * w4;
-      sl += F1 * (w5 + w3);
-      sl += F2 * (w6 + w2);
-      sl += F3 * (w7 + w1);
-      sl += F4 * (w8 + w0);
-      float sh = G0 * w5;
-      sh += G1 * (w6 + w4);
-      sh += G2 * (w7 + w3);
-      sh += G3 * (w8 + w2);
-      lo[q] = sl;
-      hi[q] = sh;
-      w0 = w2; w1 = w3; w2 = w4; w3 = w5; w4 = w6; w5 = w7; w6 = w8;
-      if (MIRROR) {
-        const int nc = c + 2 * q + 2;
-        w7 = ldm<true>(col, nc + 3, n, LS);
-        w8 = ldm<true>(col, nc + 4, n, LS);
-      } else {
-        w7 = cc[(2 * q + 5) * LS];
-        w8 = cc[(2 * q + 6) * LS];
-      }
+    for (int u = 0; u < WV_EPT; u++) {
+      int l, p;
+      lm_index<TT>(P, tid, u, l, p);
+      v[u] = to_f(src[lb[l] + min(p, pmax)]);
+    }
+  } else {
+    const PosMap m = pos_map_load<CLAMP>(P, t);
+#pragma unroll
+    for (int u = 0; u < WV_EPT; u++) {
+      int l, p;
+      pm_index<TT>(P, tid, u, l, p);
+      v[u] = to_f(src[m.rowbase + (size_t)min(p, m.pcap) * m.pstride + min(t.a + l, m.xcap)]);
     }
   }
 }
 
-template <int WAVE, int TT>
-__device__ void forward_levels(float *tile, const DwtPass &P) {
+template <int TT>
+__device__ inline void regs_to_lds(const DwtPass &P, float *tile, const float (&v)[WV_EPT]) {
+  const int LS = (TT > 0 ? TT : P.T) + 1;
+  const int tid = opaque_tid();
+  const int plast = P.L - 1;
+  if (line_major_load(P)) {
+#pragma unroll
+    for (int u = 0; u < WV_EPT; u++) {
+      int l, p;
+      lm_index<TT>(P, tid, u, l, p);
+      tile[min(p, plast) * LS + l] = v[u];
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < WV_EPT; u++) {
+      int l, p;
+      pm_index<TT>(P, tid, u, l, p);
+      tile[min(p, plast) * LS + l] = v[u];
+    }
+  }
+}
+
+// row of output position p: forward CDF 9/7 keeps its high bands GAP rows
+// past the final low band
+template <int GAP>
+__device__ inline int out_row(int p, int nfin) {
+  return GAP ? p + (p >= nfin ? GAP : 0) : p;
+}
+
+template <int TT, int GAP>
+__device__ inline void store_tile(const DwtPass &P, const Tile &t, const float *tile,
+                                  const size_t *lb) {
+  const int LS = (TT > 0 ? TT : P.T) + 1;
+  const int tid = opaque_tid();
+  const int plast = P.L - 1;
+  const int nfin = P.levels > 0 ? P.L >> P.levels : P.L;
+  if (line_major_store(P)) {
+#pragma unroll
+    for (int u = 0; u < WV_EPT; u++) {
+      int l, p;
+      lm_index<TT>(P, tid, u, l, p);
+      p = min(p, plast);
+      P.dst[lb[l] + p] = tile[out_row<GAP>(p, nfin) * LS + l];
+    }
+  } else {
+    const PosMap m = pos_map_store(P, t);
+#pragma unroll
+    for (int u = 0; u < WV_EPT; u++) {
+      int l, p;
+      pm_index<TT>(P, tid, u, l, p);
+      p = min(p, plast);
+      P.dst[m.rowbase + (size_t)p * m.pstride + min(t.a + l, m.xcap)] =
+          tile[out_row<GAP>(p, nfin) * LS + l];
+    }
+  }
+}
+
+// ---- forward levels ---------------------------------------------------------
+// CDF 9/7 with the gap layout.  Level input: rows [0, n); the window of the
+// pair j spans rows 2j-4 .. 2j+4, i.e. rows -4 .. n+2 over the line.
+template <int TT>
+__device__ void forward97_levels(float *tile, const DwtPass &P) {
   const int T = TT > 0 ? TT : P.T;
   const int LS = T + 1;
-  const int tid = threadIdx.x;
+  const int tid = opaque_tid();
   const int line = tid % T;
   const int k = tid / T;
   const int tpl = WV_DBLOCK / T;
@@ -194,22 +343,85 @@ __device__ void forward_levels(float *tile, const DwtPass &P) {
     const int cnt = min(cpt, pairs - j0);
     float lo[WV_MAXP], hi[WV_MAXP];
     if (cnt > 0) {
-      if (WAVE == WV_WAVELET_CDF97) {
-        // the window reaches 4 positions beyond the chunk on both sides
-        if (2 * j0 - 4 >= 0 && 2 * (j0 + cnt) + 4 < n)
-          cdf97_chunk<false>(col, LS, n, j0, cnt, lo, hi);
-        else
-          cdf97_chunk<true>(col, LS, n, j0, cnt, lo, hi);
-      } else {
-        const float *cc = col + 2 * j0 * LS;
+      const int c = 2 * j0;
+      // mirrored halo rows this chunk reads (MirroredArray, dwt_cpu.h:162-176);
+      // threads of neighbouring chunks may write the same values
+      if (c < 4) {
 #pragma unroll
-        for (int q = 0; q < WV_MAXP; q++) {
-          if (q < cnt) {
-            const float a = cc[(2 * q) * LS], b = cc[(2 * q + 1) * LS];
-            // haar_internal dwt_cpu.cc:159-162
-            hi[q] = (a - b) * INV_SQRT2;
-            lo[q] = (a + b) * INV_SQRT2;
+        for (int h = 1; h <= 4; h++) col[-h * LS] = col[wv_mirror(-h, n) * LS];
+      }
+      if (c + 2 * cnt + 2 >= n) {
+#pragma unroll
+        for (int h = 0; h < 3; h++) col[(n + h) * LS] = col[wv_mirror(n + h, n) * LS];
+      }
+      const float *cc = col + c * LS;
+      float w0 = cc[-4 * LS], w1 = cc[-3 * LS], w2 = cc[-2 * LS], w3 = cc[-LS],
+            w4 = cc[0], w5 = cc[LS], w6 = cc[2 * LS], w7 = cc[3 * LS], w8 = cc[4 * LS];
+#pragma unroll
+      for (int q = 0; q < WV_MAXP; q++) {
+        if (q < cnt) {
+          if (q > 0) {
+            w0 = w2; w1 = w3; w2 = w4; w3 = w5; w4 = w6; w5 = w7; w6 = w8;
+            w7 = cc[(2 * q + 3) * LS];
+            w8 = cc[(2 * q + 4) * LS];
           }
+          // cdf97_tmpl dwt_cpu.cc:709-723: centre tap first, then
+          // F[k]*(a[i+k]+a[i-k]) for k = 1..4, no contraction
+          float sl = F0 * w4;
+          sl += F1 * (w5 + w3);
+          sl += F2 * (w6 + w2);
+          sl += F3 * (w7 + w1);
+          sl += F4 * (w8 + w0);
+          float sh = G0 * w5;
+          sh += G1 * (w6 + w4);
+          sh += G2 * (w7 + w3);
+          sh += G3 * (w8 + w2);
+          lo[q] = sl;
+          hi[q] = sh;
+        }
+      }
+    }
+    __syncthreads();
+    if (cnt > 0) {
+      float *lw = col + j0 * LS, *hw = col + (pairs + 4 + j0) * LS;
+#pragma unroll
+      for (int q = 0; q < WV_MAXP; q++) {
+        if (q < cnt) {
+          lw[q * LS] = lo[q];
+          hw[q * LS] = hi[q];
+        }
+      }
+    }
+    __syncthreads();
+    n = pairs;
+  }
+}
+
+template <int TT>
+__device__ void forward_haar_levels(float *tile, const DwtPass &P) {
+  const int T = TT > 0 ? TT : P.T;
+  const int LS = T + 1;
+  const int tid = opaque_tid();
+  const int line = tid % T;
+  const int k = tid / T;
+  const int tpl = WV_DBLOCK / T;
+  float *col = tile + line;
+  int n = P.L;
+  for (int s = 0; s < P.levels; s++) {
+    const int pairs = n >> 1;
+    const int cpt = (pairs + tpl - 1) / tpl;
+    const int j0 = k * cpt;
+    const int cnt = min(cpt, pairs - j0);
+    float lo[WV_MAXP], hi[WV_MAXP];
+    if (cnt > 0) {
+      const float *cc = col + 2 * j0 * LS;
+#pragma unroll
+      for (int q = 0; q < WV_MAXP; q++) {
+        if (q < cnt) {
+          const float a = cc[(2 * q) * LS], b = cc[(2 * q + 1) * LS];
+          // haar_internal dwt_cpu.cc:159-162
+          hi[q] = (a - b) * INV_SQRT2;
+          lo[q] = (a + b) * INV_SQRT2;
         }
       }
     }
@@ -241,7 +453,7 @@ template <int WAVE, int TT>
 __device__ void inverse_levels(float *tile, const DwtPass &P) {
   const int T = TT > 0 ? TT : P.T;
   const int LS = T + 1;
-  const int tid = threadIdx.x;
+  const int tid = opaque_tid();
   const int line = tid % T;
   const int k = tid / T;
   const int tpl = WV_DBLOCK / T;
@@ -264,6 +476,12 @@ __device__ void inverse_levels(float *tile, const DwtPass &P) {
 #pragma unroll
         for (int q = 0; q < WV_MAXP; q++) {
           if (q < cnt) {
+            if (q > 0) {
+              const int nc = c + 2 * q;
+              u0 = u2; u1 = u3; u2 = u4; u3 = u5; u4 = u6; u5 = u7; u6 = u8;
+              u7 = ldi(col, nc + 4, n, half, LS);
+              u8 = ldi(col, nc + 5, n, half, LS);
+            }
             // cdf97_inverse_tmpl dwt_cpu.cc:784-795
             float e = u3 * S0;
             e += (u4 + u2) * S1;
@@ -276,10 +494,6 @@ __device__ void inverse_levels(float *tile, const DwtPass &P) {
             o += (u8 + u0) * H4;
             ev[q] = e;
             od[q] = o;
-            const int nc = c + 2 * q + 2;
-            u0 = u2; u1 = u3; u2 = u4; u3 = u5; u4 = u6; u5 = u7; u6 = u8;
-            u7 = ldi(col, nc + 4, n, half, LS);
-            u8 = ldi(col, nc + 5, n, half, LS);
           }
         }
       } else {
@@ -312,130 +526,138 @@ __device__ void inverse_levels(float *tile, const DwtPass &P) {
 }
 
 // ---- the pass kernel ------------------------------------------------------
+template <int WAVE, bool INV, int TT>
+__device__ inline void run_levels(float *tile, const DwtPass &P) {
+  if (P.levels <= 0) return;
+  if (INV) inverse_levels<WAVE, TT>(tile, P);
+  else if (WAVE == WV_WAVELET_CDF97) forward97_levels<TT>(tile, P);
+  else forward_haar_levels<TT>(tile, P);
+}
+
+template <bool CLAMP>
+__device__ inline void set_bases(const DwtPass &P, const Tile &t, size_t *lb_ld,
+                                 const Tile *ts, size_t *lb_st) {
+  const int T = P.T;
+  for (int l = threadIdx.x; l < T; l += WV_DBLOCK) {
+    if (lb_ld) lb_ld[l] = line_base_load<CLAMP>(P, t, l);
+    if (lb_st) lb_st[l] = line_base_store(P, *ts, l);
+  }
+}
+
 template <int WAVE, bool INV, typename SRC, bool CLAMP, int TT>
-__global__ void __launch_bounds__(WV_DBLOCK, 2)
-dwt_pass_kernel(DwtPass P) {
+__global__ void __launch_bounds__(WV_DBLOCK, WV_DWAVES)
+dwt_pass_kernel(DwtPass P, int ntiles) {
+  constexpr int GAP = (WAVE == WV_WAVELET_CDF97 && !INV) ? WV_HALO : 0;
   extern __shared__ float smem[];
-  __shared__ size_t lbase[32];
-  float *tile = smem;
+  __shared__ size_t lb_ld[2][WV_MAXT];   // line bases of the tile being loaded
+  __shared__ size_t lb_st[WV_MAXT];      // line bases of the tile being stored
   const int T = TT > 0 ? TT : P.T;
-  const int LS = T + 1;
-  const int tid = threadIdx.x;
-  const Tile t = decode_tile(P, T);
-  const int total = T * P.L;
-  const SRC *src = (const SRC *)P.src;
-
-  // 1. load
-  if (line_major_load(P)) {
-    if (tid < T) lbase[tid] = tid < t.nvalid ? line_base_load<CLAMP>(P, t, tid) : 0;
-    __syncthreads();
-    const int pmax = (CLAMP && P.axis == 0) ? P.ox - 1 : P.L - 1;
-    for (int i0 = tid; i0 < total; i0 += WV_DBLOCK * WV_UNR) {
-      float v[WV_UNR];
-#pragma unroll
-      for (int u = 0; u < WV_UNR; u++) {
-        const int i = i0 + u * WV_DBLOCK;
-        v[u] = 0.f;
-        if (i < total) {
-          const int l = (int)__umulhi((unsigned)i, P.magic_L);
-          const int p = i - l * P.L;
-          if (l < t.nvalid) v[u] = to_f(src[lbase[l] + min(p, pmax)]);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < WV_UNR; u++) {
-        const int i = i0 + u * WV_DBLOCK;
-        if (i < total) {
-          const int l = (int)__umulhi((unsigned)i, P.magic_L);
-          const int p = i - l * P.L;
-          tile[p * LS + l] = v[u];
-        }
-      }
-    }
-  } else {
-    const int l = tid % T;
-    const int pstep = WV_DBLOCK / T;
-    const bool lvalid = l < t.nvalid;
-    const int x = CLAMP ? min(t.a + l, P.ox - 1) : t.a + l;
-    for (int p0 = tid / T; p0 < P.L; p0 += pstep * WV_UNR) {
-      float v[WV_UNR];
-#pragma unroll
-      for (int u = 0; u < WV_UNR; u++) {
-        const int p = p0 + u * pstep;
-        v[u] = 0.f;
-        if (p < P.L && lvalid) v[u] = to_f(src[pos_base_load<CLAMP>(P, t, p) + x]);
-      }
-#pragma unroll
-      for (int u = 0; u < WV_UNR; u++) {
-        const int p = p0 + u * pstep;
-        if (p < P.L) tile[p * LS + l] = v[u];
-      }
-    }
-  }
+  float *tile = smem + WV_HALO * (T + 1);
+  Cursor c = cursor_at(P, blockIdx.x);
+  if (c.tile >= ntiles) return;
+  Step st;
+  st.g = gridDim.x;
+  st.gx = P.axis != 0 ? st.g % P.ntiles_x : 0;
+  st.gb = P.axis != 0 ? st.g / P.ntiles_x : 0;
+  const bool lml = line_major_load(P), lms = line_major_store(P);
+  Tile t = cursor_tile(P, T, c);
+  set_bases<CLAMP>(P, t, lml ? lb_ld[0] : nullptr, &t, nullptr);
   __syncthreads();
-
-  // 2. levels
-  if (P.levels > 0) {
-    if (INV) inverse_levels<WAVE, TT>(tile, P);
-    else forward_levels<WAVE, TT>(tile, P);
-  }
-
-  // 3. store
-  if (line_major_store(P)) {
-    if (tid < T) lbase[tid] = tid < t.nvalid ? line_base_store(P, t, tid) : 0;
+  float v[WV_EPT];
+  load_regs<SRC, CLAMP, TT>(P, t, lb_ld[0], v);
+  int buf = 0;
+  for (;;) {
+    Cursor cn = c;
+    cursor_advance(P, st, cn);
+    const bool more = cn.tile < ntiles;
+    const Tile tn = cursor_tile(P, T, more ? cn : c);
+    regs_to_lds<TT>(P, tile, v);
+    set_bases<CLAMP>(P, tn, (more && lml) ? lb_ld[buf ^ 1] : nullptr, &t,
+                     lms ? lb_st : nullptr);
     __syncthreads();
-    for (int i0 = tid; i0 < total; i0 += WV_DBLOCK * WV_UNR) {
-#pragma unroll
-      for (int u = 0; u < WV_UNR; u++) {
-        const int i = i0 + u * WV_DBLOCK;
-        if (i < total) {
-          const int l = (int)__umulhi((unsigned)i, P.magic_L);
-          const int p = i - l * P.L;
-          if (l < t.nvalid) P.dst[lbase[l] + p] = tile[p * LS + l];
-        }
-      }
-    }
-  } else {
-    const int l = tid % T;
-    const int pstep = WV_DBLOCK / T;
-    if (l < t.nvalid) {
-      for (int p0 = tid / T; p0 < P.L; p0 += pstep * WV_UNR) {
-#pragma unroll
-        for (int u = 0; u < WV_UNR; u++) {
-          const int p = p0 + u * pstep;
-          if (p < P.L) P.dst[pos_base_store(P, t, p) + t.a + l] = tile[p * LS + l];
-        }
-      }
-    }
+    if (more) load_regs<SRC, CLAMP, TT>(P, tn, lb_ld[buf ^ 1], v);
+    run_levels<WAVE, INV, TT>(tile, P);
+    store_tile<TT, GAP>(P, t, tile, lb_st);
+    if (!more) break;
+    c = cn;
+    t = tn;
+    buf ^= 1;
+    __syncthreads();
   }
 }
 
 // ---- host side ------------------------------------------------------------
+// resident blocks per CU override (autotuning hook): WV_DWT_BPC
+static int bpc_override() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = getenv("WV_DWT_BPC");
+    v = e ? atoi(e) : 0;
+    if (v < 0 || v > 16) v = 0;
+  }
+  return v;
+}
+
+static int cu_count() {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (!cus[dev]) {
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        n <= 0)
+      n = 256;
+    cus[dev] = n;
+  }
+  return cus[dev];
+}
+
+static size_t tile_lds_bytes(const DwtPass &P) {
+  return (size_t)(P.L + 2 * WV_HALO) * (P.T + 1) * 4;
+}
+
 template <int WAVE, bool INV, typename SRC, bool CLAMP, int TT>
 static hipError_t launch_one(const DwtPass &P, int ntiles, hipStream_t s) {
   auto k = dwt_pass_kernel<WAVE, INV, SRC, CLAMP, TT>;
-  const size_t lds = (size_t)P.L * (P.T + 1) * 4;
+  const size_t lds = tile_lds_bytes(P);
   static bool attr_set = false;
   if (!attr_set) {
     // 160 KB of LDS per workgroup on gfx950, minus the kernel's static LDS
     hipError_t e = hipFuncSetAttribute((const void *)k,
                                        hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       159 * 1024);
+                                       152 * 1024);
     if (e != hipSuccess) return e;
     attr_set = true;
   }
-  hipLaunchKernelGGL(k, dim3(ntiles), dim3(WV_DBLOCK), lds, s, P);
+  // persistent grid: as many blocks as stay resident at once
+  int per_cu = bpc_override();
+  if (!per_cu) {
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, WV_DBLOCK, lds);
+    if (e != hipSuccess) return e;
+    if (per_cu < 1) per_cu = 1;
+  }
+  const long cap = (long)per_cu * cu_count();
+  const int grid = (int)(ntiles < cap ? ntiles : cap);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(WV_DBLOCK), lds, s, P, ntiles);
   return hipGetLastError();
 }
 
+// Compile-time tile widths need L % 64 == 0 (whole 64-position line
+// segments per wave instruction); anything else runs the generic mapping.
 template <int WAVE, bool INV, typename SRC, bool CLAMP>
 static hipError_t launch_tile(const DwtPass &P, int ntiles, hipStream_t s) {
-  switch (P.T) {
-    case 32: return launch_one<WAVE, INV, SRC, CLAMP, 32>(P, ntiles, s);
-    case 16: return launch_one<WAVE, INV, SRC, CLAMP, 16>(P, ntiles, s);
-    case 8: return launch_one<WAVE, INV, SRC, CLAMP, 8>(P, ntiles, s);
-    default: return launch_one<WAVE, INV, SRC, CLAMP, 0>(P, ntiles, s);
+  if (P.L % 64 == 0) {
+    switch (P.T) {
+      case 256: return launch_one<WAVE, INV, SRC, CLAMP, 256>(P, ntiles, s);
+      case 128: return launch_one<WAVE, INV, SRC, CLAMP, 128>(P, ntiles, s);
+      case 64: return launch_one<WAVE, INV, SRC, CLAMP, 64>(P, ntiles, s);
+      case 32: return launch_one<WAVE, INV, SRC, CLAMP, 32>(P, ntiles, s);
+      case 16: return launch_one<WAVE, INV, SRC, CLAMP, 16>(P, ntiles, s);
+      case 8: return launch_one<WAVE, INV, SRC, CLAMP, 8>(P, ntiles, s);
+      default: break;
+    }
   }
+  return launch_one<WAVE, INV, SRC, CLAMP, 0>(P, ntiles, s);
 }
 
 template <int WAVE, bool INV>
@@ -458,26 +680,30 @@ static int tile_cap() {
   static int cap = -1;
   if (cap < 0) {
     const char *e = getenv("WV_DWT_TILE");
-    cap = e ? atoi(e) : 32;
-    if (cap != 8 && cap != 16 && cap != 32) cap = 32;
+    cap = e ? atoi(e) : WV_MAXT;
+    if (cap < 1 || cap > WV_MAXT || (cap & (cap - 1))) cap = WV_MAXT;
   }
   return cap;
 }
 
 // Fills the tile geometry of a pass; returns the number of tiles or -1 if
-// the line is longer than one LDS tile can hold.
+// the line is longer than one LDS tile can hold.  T is the largest power of
+// two with L*T <= WV_DBLOCK*WV_EPT (every thread then carries at most
+// WV_EPT elements and owns at most WV_MAXP pairs per level), reduced for
+// y / z passes over narrow volumes so a tile does not run far past x.
 int plan_pass(DwtPass &P, int axis) {
   const int dims[3] = {P.X, P.Y, P.Z};
   P.axis = axis;
   P.L = dims[axis];
   if (P.L > WV_MAX_LINE) return -1;
   int T = tile_cap();
-  while (T > 1 && (long)P.L * T > 2L * WV_DBLOCK * WV_MAXP) T >>= 1;
+  while (T > 1 && (long)P.L * T > (long)WV_DBLOCK * WV_EPT) T >>= 1;
   if (axis != 0) {
-    while (T > 8 && T / 2 >= P.X) T >>= 1;   // narrow cubelets
+    while (T > 1 && T / 2 >= P.X) T >>= 1;
   }
   P.T = T;
   P.log2T = ilog2(T);
+  P.log2dblock_T = ilog2(WV_DBLOCK / T);
   P.magic_L = (unsigned)(((1ull << 32) + (unsigned)P.L - 1) / (unsigned)P.L);
   if (axis == 0) {
     const int rows = P.Y * P.Z;

@@ -6,6 +6,7 @@
 #include "../../include/wv_hip.h"
 
 #define WV_MAX_LINE 16384   // longest axis one LDS tile holds
+#define WV_MAXT 256         // most lines in one LDS tile
 
 namespace wv {
 
@@ -20,7 +21,8 @@ struct DwtPass {
   int load_zxy;       // source is in the rotated z,x,y layout (axis 2)
   int store_zxy;      // destination is in the z,x,y layout (axis 2)
   int levels;
-  int T, log2T;       // lines per tile
+  int T, log2T;       // lines per tile (a power of two <= WV_MAXT)
+  int log2dblock_T;   // log2(threads per block / T)
   int L;              // line length
   unsigned magic_L;   // ceil(2^32 / L) for exact division of i < 2^16
   int ntiles_x;
