@@ -5,11 +5,18 @@
 
 #define WV_BLOCK 256        // threads per block of the streaming kernels
 #define WV_DBLOCK 512       // threads per block of the DWT tile kernel
+#ifndef WV_MAXP
 #define WV_MAXP 16          // max (low,high) pairs a thread owns per level
+#endif
 // waves per SIMD the DWT kernel is compiled for (the launch-bounds second
-// argument is the waves-per-EU hint on AMDGPU): 4 caps it at 128 VGPRs
+// argument is the waves-per-EU hint on AMDGPU): 4 caps it at 128 VGPRs;
+// the half-size tile variant (16 elements per thread) asks for 6 (80 VGPRs,
+// three 512-thread blocks per CU), which measured faster than 8
 #ifndef WV_DWAVES
 #define WV_DWAVES 4
+#endif
+#ifndef WV_DWAVES16
+#define WV_DWAVES16 6
 #endif
 #define WV_H1_BITS 11       // threshold select digit 1: bits 30..20 of |x|
 #define WV_H1_BINS 2048

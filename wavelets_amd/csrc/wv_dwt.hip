@@ -15,7 +15,7 @@
 //      same registers -- they stay in flight through steps 2 and 3, since
 //      barriers only wait for LDS traffic,
 //   2. runs every level of the 1-D transform in LDS: each thread owns up to
-//      WV_MAXP consecutive (low, high) output pairs of one line and walks a
+//      EPT/2 consecutive (low, high) output pairs of one line and walks a
 //      9-value register window over them; all threads write back after a
 //      barrier,
 //   3. stores the tile back in x,y,z layout or, for the z pass, in the
@@ -56,7 +56,7 @@ constexpr float H0 = -.85269867900940f, H1 = .418092273222210f,
                 H4 = -.037828455506995f;
 constexpr float INV_SQRT2 = 0.70710678118654752440f;   // dwt_cpu.cc:11
 
-#define WV_EPT (2 * WV_MAXP)   // tile elements per thread: L*T <= WV_DBLOCK*WV_EPT
+#define WV_EPT (2 * WV_MAXP)   // most tile elements per thread (EPT): L*T <= WV_DBLOCK*EPT
 #define WV_HALO 4              // spare rows before and after a line
 
 template <typename S> __device__ inline float to_f(S v) { return (float)v; }
@@ -209,19 +209,28 @@ __device__ inline PosMap pos_map_store(const DwtPass &P, const Tile &t) {
 }
 
 // ---- element <-> thread mappings ------------------------------------------
-// Line-major, compile-time tile width TT >= 8 and L a multiple of 64: wave w
-// owns lines w*TT/8 .. w*TT/8 + TT/8 - 1; element u of a lane is line
-// w*TT/8 + u % (TT/8) at position (u / (TT/8)) * 64 + lane.
-// Line-major, generic: element u is flat index tid + u*WV_DBLOCK of the
-// line-major tile (division by L through magic_L).
-// Position-major: element u is line tid % T at position
-// tid / T + u * (WV_DBLOCK / T).
-template <int TT>
+// General mappings (any tile):
+//   line-major, compile-time TT >= 8 with L % 64 == 0: wave w owns lines
+//     w*TT/8 .. w*TT/8 + TT/8 - 1; element u = li*JN + j of a lane is line
+//     w*TT/8 + li at position j*64 + lane (JN = EPT / (TT/8));
+//   line-major, generic: element u is flat index tid + u*WV_DBLOCK of the
+//     line-major tile (division by L through magic_L);
+//   position-major: element u is line tid % T at position
+//     tid / T + u * (WV_DBLOCK / T).
+// Exact tiles (compile-time TT, L*T == WV_DBLOCK*EPT, no padding, all T
+// lines valid) use the same element order with no clamping, and address
+// global memory as a wave-uniform base plus a 32-bit lane offset.
+// compile-time line-major mapping: whole 64-position segments per wave
+// instruction, at most EPT lines per wave
+template <int TT, int EPT> constexpr bool lm_fast() { return TT >= 8 && TT / 8 <= EPT; }
+
+template <int TT, int EPT>
 __device__ inline void lm_index(const DwtPass &P, int tid, int u, int &l, int &p) {
-  if (TT >= 8) {
-    constexpr int LPW = TT >= 8 ? TT / 8 : 1;
-    l = (tid >> 6) * LPW + u % LPW;
-    p = (u / LPW) * 64 + (tid & 63);
+  if (lm_fast<TT, EPT>()) {
+    constexpr int LPW = lm_fast<TT, EPT>() ? TT / 8 : 1;
+    constexpr int JN = EPT / LPW;
+    l = (tid >> 6) * LPW + u / JN;
+    p = (u % JN) * 64 + (tid & 63);
   } else {
     const int i = min(tid + u * WV_DBLOCK, P.T * P.L - 1);
     l = (int)__umulhi((unsigned)i, P.magic_L);
@@ -229,7 +238,7 @@ __device__ inline void lm_index(const DwtPass &P, int tid, int u, int &l, int &p
   }
 }
 
-template <int TT>
+template <int TT, int EPT>
 __device__ inline void pm_index(const DwtPass &P, int tid, int u, int &l, int &p) {
   if (TT > 0) {
     l = tid % TT;
@@ -240,85 +249,161 @@ __device__ inline void pm_index(const DwtPass &P, int tid, int u, int &l, int &p
   }
 }
 
+template <int TT, int EPT>
+__device__ inline bool exact_tile(const DwtPass &P, const Tile &t) {
+  return lm_fast<TT, EPT>() && P.exact && t.nvalid == TT;
+}
+
+// 64-bit value that is the same in every lane, moved to scalar registers
+__device__ inline size_t uniform64(size_t v) {
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return ((size_t)hi << 32) | lo;
+}
+
 // Issues the loads of this thread's elements of tile t into v[] without
-// waiting for them.
-template <typename SRC, bool CLAMP, int TT>
+// waiting for them.  Every path issues exactly EPT loads.
+template <typename SRC, bool CLAMP, int TT, int EPT>
 __device__ inline void load_regs(const DwtPass &P, const Tile &t, const size_t *lb,
-                                 float (&v)[WV_EPT]) {
+                                 float (&v)[EPT]) {
   const int tid = opaque_tid();
   const SRC *src = (const SRC *)P.src;
+  if (exact_tile<TT, EPT>(P, t)) {
+    if (line_major_load(P)) {
+      constexpr int LPW = lm_fast<TT, EPT>() ? TT / 8 : 1;
+      constexpr int JN = EPT / LPW;
+      const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+      const int lane = tid & 63;
+#pragma unroll
+      for (int li = 0; li < LPW; li++) {
+        const SRC *base = src + uniform64(lb[w * LPW + li]);
+#pragma unroll
+        for (int j = 0; j < JN; j++) v[li * JN + j] = to_f(base[j * 64 + lane]);
+      }
+    } else {
+      constexpr int PSTEP = WV_DBLOCK / (TT > 0 ? TT : 1);
+      const PosMap m = pos_map_load<CLAMP>(P, t);
+      const char *base = (const char *)(src + m.rowbase + t.a);
+      constexpr int TW = TT > 0 ? TT : 1;
+      const unsigned voff = (unsigned)(((tid / TW) * (unsigned)m.pstride + tid % TW) * sizeof(SRC));
+      const size_t ustep = (size_t)PSTEP * m.pstride * sizeof(SRC);
+#pragma unroll
+      for (int u = 0; u < EPT; u++)
+        v[u] = to_f(*(const SRC *)(base + u * ustep + voff));
+    }
+    return;
+  }
   if (line_major_load(P)) {
     const int pmax = (CLAMP && P.axis == 0) ? P.ox - 1 : P.L - 1;
 #pragma unroll
-    for (int u = 0; u < WV_EPT; u++) {
+    for (int u = 0; u < EPT; u++) {
       int l, p;
-      lm_index<TT>(P, tid, u, l, p);
+      lm_index<TT, EPT>(P, tid, u, l, p);
       v[u] = to_f(src[lb[l] + min(p, pmax)]);
     }
   } else {
     const PosMap m = pos_map_load<CLAMP>(P, t);
 #pragma unroll
-    for (int u = 0; u < WV_EPT; u++) {
+    for (int u = 0; u < EPT; u++) {
       int l, p;
-      pm_index<TT>(P, tid, u, l, p);
+      pm_index<TT, EPT>(P, tid, u, l, p);
       v[u] = to_f(src[m.rowbase + (size_t)min(p, m.pcap) * m.pstride + min(t.a + l, m.xcap)]);
     }
   }
 }
 
-template <int TT>
-__device__ inline void regs_to_lds(const DwtPass &P, float *tile, const float (&v)[WV_EPT]) {
+template <int TT, int EPT>
+__device__ inline void regs_to_lds(const DwtPass &P, const Tile &t, float *tile,
+                                   const float (&v)[EPT]) {
   const int LS = (TT > 0 ? TT : P.T) + 1;
   const int tid = opaque_tid();
+  if (exact_tile<TT, EPT>(P, t)) {
+    if (line_major_load(P)) {
+      constexpr int LPW = lm_fast<TT, EPT>() ? TT / 8 : 1;
+      constexpr int JN = EPT / LPW;
+      float *b = tile + (tid & 63) * LS + (tid >> 6) * LPW;
+#pragma unroll
+      for (int li = 0; li < LPW; li++)
+#pragma unroll
+        for (int j = 0; j < JN; j++) b[j * 64 * LS + li] = v[li * JN + j];
+    } else {
+      constexpr int PSTEP = WV_DBLOCK / (TT > 0 ? TT : 1);
+      constexpr int TW = TT > 0 ? TT : 1;
+      float *b = tile + (tid / TW) * LS + tid % TW;
+#pragma unroll
+      for (int u = 0; u < EPT; u++) b[u * PSTEP * LS] = v[u];
+    }
+    return;
+  }
   const int plast = P.L - 1;
   if (line_major_load(P)) {
 #pragma unroll
-    for (int u = 0; u < WV_EPT; u++) {
+    for (int u = 0; u < EPT; u++) {
       int l, p;
-      lm_index<TT>(P, tid, u, l, p);
+      lm_index<TT, EPT>(P, tid, u, l, p);
       tile[min(p, plast) * LS + l] = v[u];
     }
   } else {
 #pragma unroll
-    for (int u = 0; u < WV_EPT; u++) {
+    for (int u = 0; u < EPT; u++) {
       int l, p;
-      pm_index<TT>(P, tid, u, l, p);
+      pm_index<TT, EPT>(P, tid, u, l, p);
       tile[min(p, plast) * LS + l] = v[u];
     }
   }
 }
 
-// row of output position p: forward CDF 9/7 keeps its high bands GAP rows
-// past the final low band
-template <int GAP>
-__device__ inline int out_row(int p, int nfin) {
-  return GAP ? p + (p >= nfin ? GAP : 0) : p;
-}
-
-template <int TT, int GAP>
-__device__ inline void store_tile(const DwtPass &P, const Tile &t, const float *tile,
+// Stores tile t from LDS; tout is the row of output position 0 (forward
+// CDF 9/7 leaves its coefficients WV_HALO rows down).
+template <int TT, int EPT>
+__device__ inline void store_tile(const DwtPass &P, const Tile &t, const float *tout,
                                   const size_t *lb) {
   const int LS = (TT > 0 ? TT : P.T) + 1;
   const int tid = opaque_tid();
+  if (exact_tile<TT, EPT>(P, t)) {
+    if (line_major_store(P)) {
+      constexpr int LPW = lm_fast<TT, EPT>() ? TT / 8 : 1;
+      constexpr int JN = EPT / LPW;
+      const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+      const int lane = tid & 63;
+      const float *b = tout + lane * LS + w * LPW;
+#pragma unroll
+      for (int li = 0; li < LPW; li++) {
+        float *base = P.dst + uniform64(lb[w * LPW + li]);
+#pragma unroll
+        for (int j = 0; j < JN; j++) base[j * 64 + lane] = b[j * 64 * LS + li];
+      }
+    } else {
+      constexpr int PSTEP = WV_DBLOCK / (TT > 0 ? TT : 1);
+      const PosMap m = pos_map_store(P, t);
+      char *base = (char *)(P.dst + m.rowbase + t.a);
+      constexpr int TW = TT > 0 ? TT : 1;
+      const unsigned voff = (unsigned)(((tid / TW) * (unsigned)m.pstride + tid % TW) * 4u);
+      const size_t ustep = (size_t)PSTEP * m.pstride * 4;
+      const float *b = tout + (tid / TW) * LS + tid % TW;
+#pragma unroll
+      for (int u = 0; u < EPT; u++)
+        *(float *)(base + u * ustep + voff) = b[u * PSTEP * LS];
+    }
+    return;
+  }
   const int plast = P.L - 1;
-  const int nfin = P.levels > 0 ? P.L >> P.levels : P.L;
   if (line_major_store(P)) {
 #pragma unroll
-    for (int u = 0; u < WV_EPT; u++) {
+    for (int u = 0; u < EPT; u++) {
       int l, p;
-      lm_index<TT>(P, tid, u, l, p);
+      lm_index<TT, EPT>(P, tid, u, l, p);
       p = min(p, plast);
-      P.dst[lb[l] + p] = tile[out_row<GAP>(p, nfin) * LS + l];
+      P.dst[lb[l] + p] = tout[p * LS + l];
     }
   } else {
     const PosMap m = pos_map_store(P, t);
 #pragma unroll
-    for (int u = 0; u < WV_EPT; u++) {
+    for (int u = 0; u < EPT; u++) {
       int l, p;
-      pm_index<TT>(P, tid, u, l, p);
+      pm_index<TT, EPT>(P, tid, u, l, p);
       p = min(p, plast);
-      P.dst[m.rowbase + (size_t)p * m.pstride + min(t.a + l, m.xcap)] =
-          tile[out_row<GAP>(p, nfin) * LS + l];
+      P.dst[m.rowbase + (size_t)p * m.pstride + min(t.a + l, m.xcap)] = tout[p * LS + l];
     }
   }
 }
@@ -326,7 +411,7 @@ __device__ inline void store_tile(const DwtPass &P, const Tile &t, const float *
 // ---- forward levels ---------------------------------------------------------
 // CDF 9/7 with the gap layout.  Level input: rows [0, n); the window of the
 // pair j spans rows 2j-4 .. 2j+4, i.e. rows -4 .. n+2 over the line.
-template <int TT>
+template <int TT, int EPT>
 __device__ void forward97_levels(float *tile, const DwtPass &P) {
   const int T = TT > 0 ? TT : P.T;
   const int LS = T + 1;
@@ -341,7 +426,7 @@ __device__ void forward97_levels(float *tile, const DwtPass &P) {
     const int cpt = (pairs + tpl - 1) / tpl;
     const int j0 = k * cpt;
     const int cnt = min(cpt, pairs - j0);
-    float lo[WV_MAXP], hi[WV_MAXP];
+    float lo[(EPT / 2)], hi[(EPT / 2)];
     if (cnt > 0) {
       const int c = 2 * j0;
       // mirrored halo rows this chunk reads (MirroredArray, dwt_cpu.h:162-176);
@@ -358,7 +443,7 @@ __device__ void forward97_levels(float *tile, const DwtPass &P) {
       float w0 = cc[-4 * LS], w1 = cc[-3 * LS], w2 = cc[-2 * LS], w3 = cc[-LS],
             w4 = cc[0], w5 = cc[LS], w6 = cc[2 * LS], w7 = cc[3 * LS], w8 = cc[4 * LS];
 #pragma unroll
-      for (int q = 0; q < WV_MAXP; q++) {
+      for (int q = 0; q < (EPT / 2); q++) {
         if (q < cnt) {
           if (q > 0) {
             w0 = w2; w1 = w3; w2 = w4; w3 = w5; w4 = w6; w5 = w7; w6 = w8;
@@ -383,9 +468,12 @@ __device__ void forward97_levels(float *tile, const DwtPass &P) {
     }
     __syncthreads();
     if (cnt > 0) {
-      float *lw = col + j0 * LS, *hw = col + (pairs + 4 + j0) * LS;
+      // the last level also moves its low band down WV_HALO rows, so every
+      // final coefficient p sits at row p + WV_HALO
+      const int lshift = s == P.levels - 1 ? WV_HALO : 0;
+      float *lw = col + (j0 + lshift) * LS, *hw = col + (pairs + WV_HALO + j0) * LS;
 #pragma unroll
-      for (int q = 0; q < WV_MAXP; q++) {
+      for (int q = 0; q < (EPT / 2); q++) {
         if (q < cnt) {
           lw[q * LS] = lo[q];
           hw[q * LS] = hi[q];
@@ -397,7 +485,7 @@ __device__ void forward97_levels(float *tile, const DwtPass &P) {
   }
 }
 
-template <int TT>
+template <int TT, int EPT>
 __device__ void forward_haar_levels(float *tile, const DwtPass &P) {
   const int T = TT > 0 ? TT : P.T;
   const int LS = T + 1;
@@ -412,11 +500,11 @@ __device__ void forward_haar_levels(float *tile, const DwtPass &P) {
     const int cpt = (pairs + tpl - 1) / tpl;
     const int j0 = k * cpt;
     const int cnt = min(cpt, pairs - j0);
-    float lo[WV_MAXP], hi[WV_MAXP];
+    float lo[(EPT / 2)], hi[(EPT / 2)];
     if (cnt > 0) {
       const float *cc = col + 2 * j0 * LS;
 #pragma unroll
-      for (int q = 0; q < WV_MAXP; q++) {
+      for (int q = 0; q < (EPT / 2); q++) {
         if (q < cnt) {
           const float a = cc[(2 * q) * LS], b = cc[(2 * q + 1) * LS];
           // haar_internal dwt_cpu.cc:159-162
@@ -429,7 +517,7 @@ __device__ void forward_haar_levels(float *tile, const DwtPass &P) {
     if (cnt > 0) {
       float *lw = col + j0 * LS, *hw = col + (pairs + j0) * LS;
 #pragma unroll
-      for (int q = 0; q < WV_MAXP; q++) {
+      for (int q = 0; q < (EPT / 2); q++) {
         if (q < cnt) {
           lw[q * LS] = lo[q];
           hw[q * LS] = hi[q];
@@ -449,7 +537,7 @@ __device__ inline float ldi(const float *col, int i, int n, int half, int LS) {
   return (i & 1) ? col[(half + (i >> 1)) * LS] : col[(i >> 1) * LS];
 }
 
-template <int WAVE, int TT>
+template <int WAVE, int TT, int EPT>
 __device__ void inverse_levels(float *tile, const DwtPass &P) {
   const int T = TT > 0 ? TT : P.T;
   const int LS = T + 1;
@@ -464,7 +552,7 @@ __device__ void inverse_levels(float *tile, const DwtPass &P) {
     const int cpt = (half + tpl - 1) / tpl;
     const int j0 = k * cpt;
     const int cnt = min(cpt, half - j0);
-    float ev[WV_MAXP], od[WV_MAXP];
+    float ev[(EPT / 2)], od[(EPT / 2)];
     if (cnt > 0) {
       if (WAVE == WV_WAVELET_CDF97) {
         const int c = 2 * j0;
@@ -474,7 +562,7 @@ __device__ void inverse_levels(float *tile, const DwtPass &P) {
               u6 = ldi(col, c + 3, n, half, LS), u7 = ldi(col, c + 4, n, half, LS),
               u8 = ldi(col, c + 5, n, half, LS);
 #pragma unroll
-        for (int q = 0; q < WV_MAXP; q++) {
+        for (int q = 0; q < (EPT / 2); q++) {
           if (q < cnt) {
             if (q > 0) {
               const int nc = c + 2 * q;
@@ -498,7 +586,7 @@ __device__ void inverse_levels(float *tile, const DwtPass &P) {
         }
       } else {
 #pragma unroll
-        for (int q = 0; q < WV_MAXP; q++) {
+        for (int q = 0; q < (EPT / 2); q++) {
           if (q < cnt) {
             const int j = j0 + q;
             const float sv = col[j * LS], dv = col[(half + j) * LS];
@@ -513,7 +601,7 @@ __device__ void inverse_levels(float *tile, const DwtPass &P) {
     if (cnt > 0) {
       float *w = col + 2 * j0 * LS;
 #pragma unroll
-      for (int q = 0; q < WV_MAXP; q++) {
+      for (int q = 0; q < (EPT / 2); q++) {
         if (q < cnt) {
           w[(2 * q) * LS] = ev[q];
           w[(2 * q + 1) * LS] = od[q];
@@ -526,12 +614,12 @@ __device__ void inverse_levels(float *tile, const DwtPass &P) {
 }
 
 // ---- the pass kernel ------------------------------------------------------
-template <int WAVE, bool INV, int TT>
+template <int WAVE, bool INV, int TT, int EPT>
 __device__ inline void run_levels(float *tile, const DwtPass &P) {
   if (P.levels <= 0) return;
-  if (INV) inverse_levels<WAVE, TT>(tile, P);
-  else if (WAVE == WV_WAVELET_CDF97) forward97_levels<TT>(tile, P);
-  else forward_haar_levels<TT>(tile, P);
+  if (INV) inverse_levels<WAVE, TT, EPT>(tile, P);
+  else if (WAVE == WV_WAVELET_CDF97) forward97_levels<TT, EPT>(tile, P);
+  else forward_haar_levels<TT, EPT>(tile, P);
 }
 
 template <bool CLAMP>
@@ -544,8 +632,28 @@ __device__ inline void set_bases(const DwtPass &P, const Tile &t, size_t *lb_ld,
   }
 }
 
-template <int WAVE, bool INV, typename SRC, bool CLAMP, int TT>
-__global__ void __launch_bounds__(WV_DBLOCK, WV_DWAVES)
+// One steady-state step: tile t goes from registers to LDS, the loads of
+// tile tn are issued, t is transformed and stored.
+template <int WAVE, bool INV, typename SRC, bool CLAMP, int TT, int EPT>
+__device__ __attribute__((always_inline)) inline void
+pipe_step(const DwtPass &P, float *tile, const float *tout, const Tile &t, const Tile &tn,
+          size_t *lb_next, size_t *lb_st, float (&v)[EPT]) {
+  regs_to_lds<TT, EPT>(P, t, tile, v);
+  set_bases<CLAMP>(P, tn, line_major_load(P) ? lb_next : nullptr, &t,
+                   line_major_store(P) ? lb_st : nullptr);
+  __syncthreads();
+  load_regs<SRC, CLAMP, TT, EPT>(P, tn, lb_next, v);
+  run_levels<WAVE, INV, TT, EPT>(tile, P);
+  store_tile<TT, EPT>(P, t, tout, lb_st);
+  __syncthreads();
+}
+
+// The first steady-state step is peeled off the loop so the loop is entered
+// with the same memory operations in flight as at its back edge (the
+// previous tile's stores behind the next tile's loads); waits on the loads
+// then never cover those stores.
+template <int WAVE, bool INV, typename SRC, bool CLAMP, int TT, int EPT>
+__global__ void __launch_bounds__(WV_DBLOCK, EPT == 16 ? WV_DWAVES16 : WV_DWAVES)
 dwt_pass_kernel(DwtPass P, int ntiles) {
   constexpr int GAP = (WAVE == WV_WAVELET_CDF97 && !INV) ? WV_HALO : 0;
   extern __shared__ float smem[];
@@ -553,37 +661,40 @@ dwt_pass_kernel(DwtPass P, int ntiles) {
   __shared__ size_t lb_st[WV_MAXT];      // line bases of the tile being stored
   const int T = TT > 0 ? TT : P.T;
   float *tile = smem + WV_HALO * (T + 1);
+  const float *tout = tile + (P.levels > 0 ? GAP : 0) * (T + 1);
   Cursor c = cursor_at(P, blockIdx.x);
   if (c.tile >= ntiles) return;
   Step st;
   st.g = gridDim.x;
   st.gx = P.axis != 0 ? st.g % P.ntiles_x : 0;
   st.gb = P.axis != 0 ? st.g / P.ntiles_x : 0;
-  const bool lml = line_major_load(P), lms = line_major_store(P);
   Tile t = cursor_tile(P, T, c);
-  set_bases<CLAMP>(P, t, lml ? lb_ld[0] : nullptr, &t, nullptr);
+  set_bases<CLAMP>(P, t, line_major_load(P) ? lb_ld[0] : nullptr, &t, nullptr);
   __syncthreads();
-  float v[WV_EPT];
-  load_regs<SRC, CLAMP, TT>(P, t, lb_ld[0], v);
+  float v[EPT];
+  load_regs<SRC, CLAMP, TT, EPT>(P, t, lb_ld[0], v);
   int buf = 0;
-  for (;;) {
-    Cursor cn = c;
-    cursor_advance(P, st, cn);
-    const bool more = cn.tile < ntiles;
-    const Tile tn = cursor_tile(P, T, more ? cn : c);
-    regs_to_lds<TT>(P, tile, v);
-    set_bases<CLAMP>(P, tn, (more && lml) ? lb_ld[buf ^ 1] : nullptr, &t,
-                     lms ? lb_st : nullptr);
-    __syncthreads();
-    if (more) load_regs<SRC, CLAMP, TT>(P, tn, lb_ld[buf ^ 1], v);
-    run_levels<WAVE, INV, TT>(tile, P);
-    store_tile<TT, GAP>(P, t, tile, lb_st);
-    if (!more) break;
-    c = cn;
+  cursor_advance(P, st, c);
+  if (c.tile < ntiles) {
+    Tile tn = cursor_tile(P, T, c);
+    pipe_step<WAVE, INV, SRC, CLAMP, TT, EPT>(P, tile, tout, t, tn, lb_ld[1], lb_st, v);
     t = tn;
-    buf ^= 1;
-    __syncthreads();
+    buf = 1;
+    cursor_advance(P, st, c);
+    while (c.tile < ntiles) {
+      tn = cursor_tile(P, T, c);
+      pipe_step<WAVE, INV, SRC, CLAMP, TT, EPT>(P, tile, tout, t, tn, lb_ld[buf ^ 1], lb_st, v);
+      t = tn;
+      buf ^= 1;
+      cursor_advance(P, st, c);
+    }
   }
+  // last tile: nothing left to prefetch
+  regs_to_lds<TT, EPT>(P, t, tile, v);
+  set_bases<CLAMP>(P, t, nullptr, &t, line_major_store(P) ? lb_st : nullptr);
+  __syncthreads();
+  run_levels<WAVE, INV, TT, EPT>(tile, P);
+  store_tile<TT, EPT>(P, t, tout, lb_st);
 }
 
 // ---- host side ------------------------------------------------------------
@@ -616,9 +727,9 @@ static size_t tile_lds_bytes(const DwtPass &P) {
   return (size_t)(P.L + 2 * WV_HALO) * (P.T + 1) * 4;
 }
 
-template <int WAVE, bool INV, typename SRC, bool CLAMP, int TT>
+template <int WAVE, bool INV, typename SRC, bool CLAMP, int TT, int EPT>
 static hipError_t launch_one(const DwtPass &P, int ntiles, hipStream_t s) {
-  auto k = dwt_pass_kernel<WAVE, INV, SRC, CLAMP, TT>;
+  auto k = dwt_pass_kernel<WAVE, INV, SRC, CLAMP, TT, EPT>;
   const size_t lds = tile_lds_bytes(P);
   static bool attr_set = false;
   if (!attr_set) {
@@ -646,18 +757,27 @@ static hipError_t launch_one(const DwtPass &P, int ntiles, hipStream_t s) {
 // segments per wave instruction); anything else runs the generic mapping.
 template <int WAVE, bool INV, typename SRC, bool CLAMP>
 static hipError_t launch_tile(const DwtPass &P, int ntiles, hipStream_t s) {
-  if (P.L % 64 == 0) {
+  if (P.L % 64 == 0 && P.ept == 16) {
     switch (P.T) {
-      case 256: return launch_one<WAVE, INV, SRC, CLAMP, 256>(P, ntiles, s);
-      case 128: return launch_one<WAVE, INV, SRC, CLAMP, 128>(P, ntiles, s);
-      case 64: return launch_one<WAVE, INV, SRC, CLAMP, 64>(P, ntiles, s);
-      case 32: return launch_one<WAVE, INV, SRC, CLAMP, 32>(P, ntiles, s);
-      case 16: return launch_one<WAVE, INV, SRC, CLAMP, 16>(P, ntiles, s);
-      case 8: return launch_one<WAVE, INV, SRC, CLAMP, 8>(P, ntiles, s);
+      case 128: return launch_one<WAVE, INV, SRC, CLAMP, 128, 16>(P, ntiles, s);
+      case 64: return launch_one<WAVE, INV, SRC, CLAMP, 64, 16>(P, ntiles, s);
+      case 32: return launch_one<WAVE, INV, SRC, CLAMP, 32, 16>(P, ntiles, s);
+      case 16: return launch_one<WAVE, INV, SRC, CLAMP, 16, 16>(P, ntiles, s);
       default: break;
     }
   }
-  return launch_one<WAVE, INV, SRC, CLAMP, 0>(P, ntiles, s);
+  if (P.L % 64 == 0) {
+    switch (P.T) {
+      case 256: return launch_one<WAVE, INV, SRC, CLAMP, 256, 32>(P, ntiles, s);
+      case 128: return launch_one<WAVE, INV, SRC, CLAMP, 128, 32>(P, ntiles, s);
+      case 64: return launch_one<WAVE, INV, SRC, CLAMP, 64, 32>(P, ntiles, s);
+      case 32: return launch_one<WAVE, INV, SRC, CLAMP, 32, 32>(P, ntiles, s);
+      case 16: return launch_one<WAVE, INV, SRC, CLAMP, 16, 32>(P, ntiles, s);
+      case 8: return launch_one<WAVE, INV, SRC, CLAMP, 8, 32>(P, ntiles, s);
+      default: break;
+    }
+  }
+  return launch_one<WAVE, INV, SRC, CLAMP, 0, 32>(P, ntiles, s);
 }
 
 template <int WAVE, bool INV>
@@ -686,18 +806,34 @@ static int tile_cap() {
   return cap;
 }
 
+// elements per thread for x-line passes (autotuning hook): WV_DWT_EPT0
+static int ept_axis0() {
+  static int v = -1;
+  if (v < 0) {
+    const char *e = getenv("WV_DWT_EPT0");
+    v = e ? atoi(e) : 16;
+    if (v != 16 && v != 32) v = 16;
+  }
+  return v;
+}
+
 // Fills the tile geometry of a pass; returns the number of tiles or -1 if
-// the line is longer than one LDS tile can hold.  T is the largest power of
-// two with L*T <= WV_DBLOCK*WV_EPT (every thread then carries at most
-// WV_EPT elements and owns at most WV_MAXP pairs per level), reduced for
-// y / z passes over narrow volumes so a tile does not run far past x.
+// the line is longer than one LDS tile can hold.  Each thread carries EPT
+// tile elements (and owns at most EPT/2 pairs per level); T is the largest
+// power of two with L*T <= WV_DBLOCK*EPT, reduced for y / z passes over
+// narrow volumes so a tile does not run far past x.  x-line passes of
+// 64..512 samples use EPT = 16 (half-size tiles, twice the resident blocks
+// per CU); y / z passes keep EPT = 32, whose T-float segments across lines
+// are twice as long.
 int plan_pass(DwtPass &P, int axis) {
   const int dims[3] = {P.X, P.Y, P.Z};
   P.axis = axis;
   P.L = dims[axis];
   if (P.L > WV_MAX_LINE) return -1;
+  P.ept = WV_EPT;
+  if (axis == 0 && P.L % 64 == 0 && P.L >= 64 && P.L <= 512) P.ept = ept_axis0();
   int T = tile_cap();
-  while (T > 1 && (long)P.L * T > (long)WV_DBLOCK * WV_EPT) T >>= 1;
+  while (T > 1 && (long)P.L * T > (long)WV_DBLOCK * P.ept) T >>= 1;
   if (axis != 0) {
     while (T > 1 && T / 2 >= P.X) T >>= 1;
   }
@@ -717,11 +853,20 @@ hipError_t launch_pass(const DwtPass &P, int ntiles, int wavelet, bool inverse,
                        int src_dtype, bool clamp, bool stats, hipStream_t s) {
   (void)stats;
   if (ntiles <= 0) return hipSuccess;
+  DwtPass Q = P;
+  // exact tiles: a tile holds exactly WV_DBLOCK*ept elements, the source
+  // needs no clamping, and lane offsets (< WV_DBLOCK/T rows of the widest
+  // stride, plus T) fit 32 bits
+  const bool unpadded = !clamp || (P.ox == P.X && P.oy == P.Y && P.oz == P.Z);
+  const long long stride = P.axis == 1 ? P.X : (long long)P.X * P.Y;
+  Q.exact = (long long)P.L * P.T == (long long)WV_DBLOCK * P.ept && P.T >= 8 &&
+            P.L % 64 == 0 && unpadded &&
+            ((long long)(WV_DBLOCK / P.T) * stride + P.T) * 4 < (1ll << 31);
   if (wavelet == WV_WAVELET_CDF97)
-    return inverse ? launch_wave<WV_WAVELET_CDF97, true>(P, src_dtype, clamp, ntiles, s)
-                   : launch_wave<WV_WAVELET_CDF97, false>(P, src_dtype, clamp, ntiles, s);
-  return inverse ? launch_wave<WV_WAVELET_HAAR, true>(P, src_dtype, clamp, ntiles, s)
-                 : launch_wave<WV_WAVELET_HAAR, false>(P, src_dtype, clamp, ntiles, s);
+    return inverse ? launch_wave<WV_WAVELET_CDF97, true>(Q, src_dtype, clamp, ntiles, s)
+                   : launch_wave<WV_WAVELET_CDF97, false>(Q, src_dtype, clamp, ntiles, s);
+  return inverse ? launch_wave<WV_WAVELET_HAAR, true>(Q, src_dtype, clamp, ntiles, s)
+                 : launch_wave<WV_WAVELET_HAAR, false>(Q, src_dtype, clamp, ntiles, s);
 }
 
 }  // namespace wv

@@ -26,6 +26,8 @@ struct DwtPass {
   int L;              // line length
   unsigned magic_L;   // ceil(2^32 / L) for exact division of i < 2^16
   int ntiles_x;
+  int ept;            // tile elements per thread (16 or 32)
+  int exact;          // L*T fills a tile exactly, no padding, 32-bit lane offsets
 };
 
 int plan_pass(DwtPass &P, int axis);
