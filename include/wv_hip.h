@@ -277,6 +277,51 @@ int wv_compress(const void *in_d, int dtype, const int orig[3],
 /* device bytes wv_compress needs in work_d */
 int64_t wv_compress_work_bytes(const int orig[3], const wv_options *opt);
 
+/* ------------------------------------------------------------------ *
+ * Batched compression of many cubelets of uneven size.  No counterpart in
+ * the reference, which compresses one cubelet per run of
+ * test_compress_gpu (compressFile, test_compress_gpu.cu:150-436); every
+ * item gets the results that path gives it alone.  Items are grouped by
+ * shape / dtype / wavelet; each transform pass of a group is one launch of
+ * the persistent pass kernel over all (cubelet, tile) items; select and
+ * quantize run a few launches for the whole batch; both sequences are
+ * replayed as hipGraphs.  Log quantizer tables are built on the host
+ * between the two (one synchronisation per submit).
+ * ------------------------------------------------------------------ */
+typedef struct {
+  const void *in_d;        /* device input: orig[0]*orig[1]*orig[2] samples */
+  int dtype;               /* WV_DTYPE_* */
+  int orig[3];
+  int wavelet;             /* WV_WAVELET_* */
+  int steps[3];            /* as wv_geom_init */
+  int quant_alg;           /* WV_QUANT_UNIFORM or WV_QUANT_LOG */
+  int bin_count;
+  float thresh_frac;       /* -thresh: fraction of |coef| set to zero */
+  float *coef_d;           /* out: padded count coefficients (z,x,y layout) */
+  int32_t *bins_d;         /* out: padded count bins; also the intermediate */
+  uint32_t *counts_d;      /* out: bin_count + 1 counts */
+} wv_batch_item;
+
+#define WV_BATCH_AUTOTUNE 1   /* time tile / occupancy candidates per group */
+#define WV_BATCH_NO_GRAPH 2   /* launch eagerly instead of replaying graphs */
+
+typedef struct wv_batch wv_batch;
+/* validates and groups the items, allocates the batch's device tables,
+   autotunes when asked (runs kernels; synchronises) */
+int wv_batch_plan(const wv_batch_item *items, int n, int flags, wv_batch **out);
+/* enqueues the batch on stream; returns after the quantize stage is
+   enqueued (log quantizers synchronise once for the host tables) */
+int wv_batch_submit(wv_batch *b, void *stream);
+int wv_batch_poll(wv_batch *b);          /* 1 done, 0 running, <0 error */
+int wv_batch_wait(wv_batch *b);
+/* threshold result and zero bin of item i after completion */
+int wv_batch_result(const wv_batch *b, int i, wv_thresh_result *res, int *zero_bin);
+int wv_batch_size(const wv_batch *b);
+int wv_batch_groups(const wv_batch *b);
+int wv_batch_group_tuning(const wv_batch *b, int g, int *tile_cap, int *ept0, int *bpc,
+                          int *ncub);
+int wv_batch_destroy(wv_batch *b);
+
 #ifdef __cplusplus
 }
 #endif

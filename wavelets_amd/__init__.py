@@ -78,6 +78,15 @@ class CompressInfo(C.Structure):
                 ("ms_total", C.c_double)]
 
 
+class BatchItem(C.Structure):
+    _fields_ = [("in_d", C.c_void_p), ("dtype", C.c_int), ("orig", C.c_int * 3),
+                ("wavelet", C.c_int), ("steps", C.c_int * 3), ("quant_alg", C.c_int),
+                ("bin_count", C.c_int), ("thresh_frac", C.c_float),
+                ("coef_d", C.c_void_p), ("bins_d", C.c_void_p), ("counts_d", C.c_void_p)]
+
+
+BATCH_AUTOTUNE, BATCH_NO_GRAPH = 1, 2
+
 P = C.c_void_p
 I64 = C.c_int64
 
@@ -118,6 +127,20 @@ def _declare(L):
                                   P, I64, C.POINTER(CompressInfo), P]
         L.wv_compress_work_bytes.restype = I64
         L.wv_compress_work_bytes.argtypes = [P, C.POINTER(Options)]
+    if hasattr(L, "wv_batch_plan"):
+        L.wv_batch_plan.argtypes = [C.POINTER(BatchItem), C.c_int, C.c_int,
+                                    C.POINTER(C.c_void_p)]
+        L.wv_batch_submit.argtypes = [P, P]
+        L.wv_batch_poll.argtypes = [P]
+        L.wv_batch_wait.argtypes = [P]
+        L.wv_batch_result.argtypes = [P, C.c_int, C.POINTER(ThreshResult),
+                                      C.POINTER(C.c_int)]
+        L.wv_batch_size.argtypes = [P]
+        L.wv_batch_groups.argtypes = [P]
+        L.wv_batch_group_tuning.argtypes = [P, C.c_int, C.POINTER(C.c_int),
+                                            C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                            C.POINTER(C.c_int)]
+        L.wv_batch_destroy.argtypes = [P]
     if hasattr(L, "wv_lloyd"):
         L.wv_lloyd_ws_bytes.restype = I64
         L.wv_lloyd_ws_bytes.argtypes = [I64, C.c_int]
@@ -338,3 +361,79 @@ def write_cube(desc_kwargs, bins_host, counts_host):
     n = check(lib().wv_write_cube(C.byref(d), b.ctypes.data_as(P), b.size, c.ctypes.data_as(P),
                                   buf.ctypes.data_as(P), cap, C.byref(pay)))
     return buf[:n].tobytes()
+
+
+class Batch:
+    """Many cubelets compressed as one batch (wv_batch_*).
+
+    cubes: list of dicts with keys vol (cuda tensor [z, y, x] of uint8 /
+    int16 / int32 / float32), wavelet, steps, alg (UNIFORM or LOG), bins,
+    frac.  Outputs per item after wait(): coef[i] (flat float32, z,x,y
+    layout), bins[i] (int32), counts[i] (bin_count + 1), result(i)."""
+
+    def __init__(self, cubes, flags=0):
+        import torch
+        self.cubes = cubes
+        self.geoms, self.coef, self.bins, self.counts = [], [], [], []
+        items = (BatchItem * len(cubes))()
+        for i, c in enumerate(cubes):
+            vol = c["vol"]
+            if not vol.is_contiguous():
+                raise ValueError("batch input %d is not contiguous" % i)
+            orig = [vol.shape[2], vol.shape[1], vol.shape[0]]
+            g = geom(orig, c["steps"], c["wavelet"], dtype_id(vol))
+            self.geoms.append(g)
+            dev = vol.device
+            self.coef.append(torch.empty(g.count, dtype=torch.float32, device=dev))
+            self.bins.append(torch.empty(g.count, dtype=torch.int32, device=dev))
+            self.counts.append(torch.empty(c["bins"] + 1, dtype=torch.int32, device=dev))
+            it = items[i]
+            it.in_d = vol.data_ptr()
+            it.dtype = dtype_id(vol)
+            it.orig = _i3(orig)
+            it.wavelet = c["wavelet"]
+            it.steps = _i3(c["steps"])
+            it.quant_alg = c["alg"]
+            it.bin_count = c["bins"]
+            it.thresh_frac = c["frac"]
+            it.coef_d = self.coef[-1].data_ptr()
+            it.bins_d = self.bins[-1].data_ptr()
+            it.counts_d = self.counts[-1].data_ptr()
+        self.h = C.c_void_p()
+        check(lib().wv_batch_plan(items, len(cubes), flags, C.byref(self.h)))
+
+    def submit(self, stream=None):
+        check(lib().wv_batch_submit(self.h, _stream(stream)))
+
+    def wait(self):
+        check(lib().wv_batch_wait(self.h))
+
+    def poll(self):
+        return check(lib().wv_batch_poll(self.h)) == 1
+
+    def result(self, i):
+        r = ThreshResult()
+        zb = C.c_int()
+        check(lib().wv_batch_result(self.h, i, C.byref(r), C.byref(zb)))
+        return r, zb.value
+
+    def groups(self):
+        out = []
+        for g in range(lib().wv_batch_groups(self.h)):
+            v = [C.c_int() for _ in range(4)]
+            check(lib().wv_batch_group_tuning(self.h, g, *[C.byref(x) for x in v]))
+            out.append(dict(tile_cap=v[0].value, ept0=v[1].value, bpc=v[2].value,
+                            ncub=v[3].value))
+        return out
+
+    def close(self):
+        if self.h:
+            lib().wv_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -27,8 +27,8 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
 DEVICE = ["--offload-arch=" + ARCH, "-munsafe-fp-atomics"]
 
 SOURCES = ["wv_dwt.hip", "wv_select.hip", "wv_quant.hip", "wv_lloyd.hip",
-           "wv_batch.hip", "wv_api.cpp", "wv_compress.cpp", "wv_params.cpp",
-           "wv_huffman.cpp", "wv_serve.cpp"]
+           "wv_api.cpp", "wv_compress.cpp", "wv_params.cpp", "wv_huffman.cpp",
+           "wv_batch.cpp", "wv_serve.cpp"]
 
 
 def _compile(src):

@@ -45,29 +45,26 @@ static unsigned ceil_log2(unsigned x) {   // dwt_cpu.cc:75-78
   return r;
 }
 
-// Pass sequence of the forward transform.  The reference visits x, rotates,
-// y, rotates, z (cdf97_3d dwt_cpu.cc:943-965); here every intermediate
-// stays in x,y,z layout and only the last pass writes the rotated z,x,y
-// layout.  With depth 1 the z,x,y and x,y,z memory orders coincide, so no
-// z pass is needed unless it has levels.
-struct PassPlan {
-  int axis[3];
-  int levels[3];
-  int n;
-};
 
-static PassPlan forward_plan(const wv_geom *g) {
-  PassPlan p{};
-  const int *S = g->steps;
-  const bool any = S[0] || S[1] || S[2];
-  if (!any) {            // no transform, no rotation: conversion only
-    p.axis[0] = 0; p.levels[0] = 0; p.n = 1;
-    return p;
+// host parameter block -> the device form the kernels read
+QuantDev quant_dev_of(const wv_quant_params *p) {
+  QuantDev q{};
+  q.alg = p->alg;
+  q.bin_count = p->bin_count;
+  q.zero_bin = p->zero_bin;
+  q.threshold = p->threshold;
+  q.neg_scale = p->neg_scale;
+  q.neg_offset = p->neg_offset;
+  q.pos_scale = p->pos_scale;
+  q.pos_offset = p->pos_offset;
+  if (p->alg == WV_QUANT_LOG) {
+    q.n_pos = p->n_pos_edges;
+    q.n_neg = p->n_neg_edges;
+    q.pos_clamp = p->pos_clamp;
   }
-  if (S[0]) { p.axis[p.n] = 0; p.levels[p.n++] = S[0]; }
-  if (S[1]) { p.axis[p.n] = 1; p.levels[p.n++] = S[1]; }
-  if (S[2] || g->padded[2] > 1) { p.axis[p.n] = 2; p.levels[p.n++] = S[2]; }
-  return p;
+  else if (p->alg == WV_QUANT_LLOYD) { q.n_pos = p->n_boundaries; q.n_neg = 0; }
+  q.table = p->table_d;
+  return q;
 }
 
 }  // namespace wv
@@ -291,25 +288,6 @@ int wv_quant_params_upload(wv_quant_params *p, void *table_d, void *stream) {
   return WV_OK;
 }
 
-static QuantDev to_dev(const wv_quant_params *p) {
-  QuantDev q{};
-  q.alg = p->alg;
-  q.bin_count = p->bin_count;
-  q.zero_bin = p->zero_bin;
-  q.threshold = p->threshold;
-  q.neg_scale = p->neg_scale;
-  q.neg_offset = p->neg_offset;
-  q.pos_scale = p->pos_scale;
-  q.pos_offset = p->pos_offset;
-  if (p->alg == WV_QUANT_LOG) {
-    q.n_pos = p->n_pos_edges;
-    q.n_neg = p->n_neg_edges;
-    q.pos_clamp = p->pos_clamp;
-  }
-  else if (p->alg == WV_QUANT_LLOYD) { q.n_pos = p->n_boundaries; q.n_neg = 0; }
-  q.table = p->table_d;
-  return q;
-}
 
 int wv_quantize_hist(const float *coef_d, int64_t n, const wv_quant_params *p,
                      int32_t *bins_d, uint32_t *counts_d, void *stream) {
@@ -318,7 +296,7 @@ int wv_quantize_hist(const float *coef_d, int64_t n, const wv_quant_params *p,
     set_error("parameter tables not uploaded");
     return WV_ERR_STATE;
   }
-  HIPCHK(quantize_hist(coef_d, n, to_dev(p), bins_d, counts_d, (hipStream_t)stream),
+  HIPCHK(quantize_hist(coef_d, n, quant_dev_of(p), bins_d, counts_d, (hipStream_t)stream),
          "quantize");
   return WV_OK;
 }

@@ -74,51 +74,34 @@ __device__ inline int opaque_tid() { return opaque((int)threadIdx.x); }
 struct Tile {
   int nvalid;      // valid lines in this tile (>= 1)
   int a, b;        // axis0: first row; axis1: x0, z; axis2: x0, y
+  const void *src; // this tile's cubelet
+  float *dst;
 };
 
-// Position of a block in its tile sequence: tile index and, for the y / z
-// passes, (x tile, other coordinate), advanced without divisions.
-struct Cursor {
-  int tile, xt, ob;
-};
-
-__device__ inline Cursor cursor_at(const DwtPass &P, int tile) {
-  Cursor c;
-  c.tile = tile;
-  c.xt = 0;
-  c.ob = 0;
-  if (P.axis != 0) {
-    c.xt = tile % P.ntiles_x;
-    c.ob = tile / P.ntiles_x;
-  }
-  return c;
-}
-
-struct Step {
-  int g, gx, gb;   // grid size and its split into (x tiles, rows)
-};
-
-__device__ inline void cursor_advance(const DwtPass &P, const Step &st, Cursor &c) {
-  c.tile += st.g;
-  if (P.axis != 0) {
-    c.xt += st.gx;
-    c.ob += st.gb;
-    if (c.xt >= P.ntiles_x) {
-      c.xt -= P.ntiles_x;
-      c.ob++;
-    }
-  }
-}
-
-__device__ inline Tile cursor_tile(const DwtPass &P, int T, const Cursor &c) {
+// Global tile index -> cubelet and tile.  Divisors pass through opaque()
+// so no reciprocal is hoisted out of the tile loop.
+__device__ inline Tile tile_at(const DwtPass &P, int T, int tile) {
   Tile t;
+  int c = 0;
+  if (P.src_tab) {
+    const int n1 = opaque(P.ntiles1);
+    c = tile / n1;
+    tile -= c * n1;
+    t.src = P.src_tab[c];
+    t.dst = P.dst_tab[c];
+  } else {
+    t.src = P.src;
+    t.dst = P.dst;
+  }
   if (P.axis == 0) {
-    t.a = c.tile * T;
+    t.a = tile * T;
     t.b = 0;
     t.nvalid = min(T, P.Y * P.Z - t.a);
   } else {
-    t.a = c.xt * T;
-    t.b = c.ob;
+    const int ntx = opaque(P.ntiles_x);
+    const int ob = tile / ntx;
+    t.a = (tile - ob * ntx) * T;
+    t.b = ob;
     t.nvalid = min(T, P.X - t.a);
   }
   return t;
@@ -267,7 +250,7 @@ template <typename SRC, bool CLAMP, int TT, int EPT>
 __device__ inline void load_regs(const DwtPass &P, const Tile &t, const size_t *lb,
                                  float (&v)[EPT]) {
   const int tid = opaque_tid();
-  const SRC *src = (const SRC *)P.src;
+  const SRC *src = (const SRC *)t.src;
   if (exact_tile<TT, EPT>(P, t)) {
     if (line_major_load(P)) {
       constexpr int LPW = lm_fast<TT, EPT>() ? TT / 8 : 1;
@@ -369,14 +352,14 @@ __device__ inline void store_tile(const DwtPass &P, const Tile &t, const float *
       const float *b = tout + lane * LS + w * LPW;
 #pragma unroll
       for (int li = 0; li < LPW; li++) {
-        float *base = P.dst + uniform64(lb[w * LPW + li]);
+        float *base = t.dst + uniform64(lb[w * LPW + li]);
 #pragma unroll
         for (int j = 0; j < JN; j++) base[j * 64 + lane] = b[j * 64 * LS + li];
       }
     } else {
       constexpr int PSTEP = WV_DBLOCK / (TT > 0 ? TT : 1);
       const PosMap m = pos_map_store(P, t);
-      char *base = (char *)(P.dst + m.rowbase + t.a);
+      char *base = (char *)(t.dst + m.rowbase + t.a);
       constexpr int TW = TT > 0 ? TT : 1;
       const unsigned voff = (unsigned)(((tid / TW) * (unsigned)m.pstride + tid % TW) * 4u);
       const size_t ustep = (size_t)PSTEP * m.pstride * 4;
@@ -394,7 +377,7 @@ __device__ inline void store_tile(const DwtPass &P, const Tile &t, const float *
       int l, p;
       lm_index<TT, EPT>(P, tid, u, l, p);
       p = min(p, plast);
-      P.dst[lb[l] + p] = tout[p * LS + l];
+      t.dst[lb[l] + p] = tout[p * LS + l];
     }
   } else {
     const PosMap m = pos_map_store(P, t);
@@ -403,7 +386,7 @@ __device__ inline void store_tile(const DwtPass &P, const Tile &t, const float *
       int l, p;
       pm_index<TT, EPT>(P, tid, u, l, p);
       p = min(p, plast);
-      P.dst[m.rowbase + (size_t)p * m.pstride + min(t.a + l, m.xcap)] = tout[p * LS + l];
+      t.dst[m.rowbase + (size_t)p * m.pstride + min(t.a + l, m.xcap)] = tout[p * LS + l];
     }
   }
 }
@@ -662,31 +645,28 @@ dwt_pass_kernel(DwtPass P, int ntiles) {
   const int T = TT > 0 ? TT : P.T;
   float *tile = smem + WV_HALO * (T + 1);
   const float *tout = tile + (P.levels > 0 ? GAP : 0) * (T + 1);
-  Cursor c = cursor_at(P, blockIdx.x);
-  if (c.tile >= ntiles) return;
-  Step st;
-  st.g = gridDim.x;
-  st.gx = P.axis != 0 ? st.g % P.ntiles_x : 0;
-  st.gb = P.axis != 0 ? st.g / P.ntiles_x : 0;
-  Tile t = cursor_tile(P, T, c);
+  int g = blockIdx.x;
+  if (g >= ntiles) return;
+  const int G = gridDim.x;
+  Tile t = tile_at(P, T, g);
   set_bases<CLAMP>(P, t, line_major_load(P) ? lb_ld[0] : nullptr, &t, nullptr);
   __syncthreads();
   float v[EPT];
   load_regs<SRC, CLAMP, TT, EPT>(P, t, lb_ld[0], v);
   int buf = 0;
-  cursor_advance(P, st, c);
-  if (c.tile < ntiles) {
-    Tile tn = cursor_tile(P, T, c);
+  g += G;
+  if (g < ntiles) {
+    Tile tn = tile_at(P, T, g);
     pipe_step<WAVE, INV, SRC, CLAMP, TT, EPT>(P, tile, tout, t, tn, lb_ld[1], lb_st, v);
     t = tn;
     buf = 1;
-    cursor_advance(P, st, c);
-    while (c.tile < ntiles) {
-      tn = cursor_tile(P, T, c);
+    g += G;
+    while (g < ntiles) {
+      tn = tile_at(P, T, g);
       pipe_step<WAVE, INV, SRC, CLAMP, TT, EPT>(P, tile, tout, t, tn, lb_ld[buf ^ 1], lb_st, v);
       t = tn;
       buf ^= 1;
-      cursor_advance(P, st, c);
+      g += G;
     }
   }
   // last tile: nothing left to prefetch
@@ -741,7 +721,13 @@ static hipError_t launch_one(const DwtPass &P, int ntiles, hipStream_t s) {
     attr_set = true;
   }
   // persistent grid: as many blocks as stay resident at once
-  int per_cu = bpc_override();
+  int per_cu = P.bpc > 0 ? P.bpc : bpc_override();
+  int occ = 0;
+  {
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, WV_DBLOCK, lds);
+    if (e != hipSuccess) return e;
+  }
+  if (per_cu > occ) per_cu = occ;
   if (!per_cu) {
     hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, WV_DBLOCK, lds);
     if (e != hipSuccess) return e;
@@ -831,8 +817,11 @@ int plan_pass(DwtPass &P, int axis) {
   P.L = dims[axis];
   if (P.L > WV_MAX_LINE) return -1;
   P.ept = WV_EPT;
-  if (axis == 0 && P.L % 64 == 0 && P.L >= 64 && P.L <= 512) P.ept = ept_axis0();
-  int T = tile_cap();
+  if (axis == 0 && P.L % 64 == 0 && P.L >= 64 && P.L <= 512)
+    P.ept = P.ept0 == 16 || P.ept0 == 32 ? P.ept0 : ept_axis0();
+  int T = P.tcap > 0 ? P.tcap : tile_cap();
+  if (T > WV_MAXT) T = WV_MAXT;
+  while (T & (T - 1)) T &= T - 1;
   while (T > 1 && (long)P.L * T > (long)WV_DBLOCK * P.ept) T >>= 1;
   if (axis != 0) {
     while (T > 1 && T / 2 >= P.X) T >>= 1;
@@ -841,12 +830,17 @@ int plan_pass(DwtPass &P, int axis) {
   P.log2T = ilog2(T);
   P.log2dblock_T = ilog2(WV_DBLOCK / T);
   P.magic_L = (unsigned)(((1ull << 32) + (unsigned)P.L - 1) / (unsigned)P.L);
+  int n1;
   if (axis == 0) {
     const int rows = P.Y * P.Z;
-    return (rows + T - 1) / T;
+    n1 = (rows + T - 1) / T;
+  } else {
+    P.ntiles_x = (P.X + T - 1) / T;
+    n1 = P.ntiles_x * (axis == 1 ? P.Z : P.Y);
   }
-  P.ntiles_x = (P.X + T - 1) / T;
-  return P.ntiles_x * (axis == 1 ? P.Z : P.Y);
+  P.ntiles1 = n1;
+  const long long total = (long long)n1 * (P.src_tab ? (P.ncub > 0 ? P.ncub : 0) : 1);
+  return total > 0x7fffffff ? -1 : (int)total;
 }
 
 hipError_t launch_pass(const DwtPass &P, int ntiles, int wavelet, bool inverse,

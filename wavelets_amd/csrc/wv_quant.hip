@@ -76,12 +76,15 @@ __device__ inline int quant_one(const QuantDev &q, const float *tab, float x) {
   }
 }
 
+// One block's share (block bidx of nblk) of the fused quantize + histogram
+// pass over coef[0, n).
+// tab / h are the caller's LDS arrays of WV_LDS_TABLE / WV_LDS_HIST entries
+// (declared once per kernel, so instantiations for several quantizers in
+// one kernel share them).
 template <int ALG>
-__global__ void __launch_bounds__(WV_BLOCK)
-quant_hist_kernel(const float *coef, int64_t n, QuantDev q,
-                  int32_t *bins, uint32_t *__restrict__ counts) {
-  __shared__ float tab[WV_LDS_TABLE];
-  __shared__ uint32_t h[WV_LDS_HIST];
+__device__ void quant_hist_body(const float *coef, int64_t n, const QuantDev &q,
+                                int32_t *bins, uint32_t *__restrict__ counts,
+                                int bidx, int nblk, float *tab, uint32_t *h) {
   const int ntab = q.n_pos + (ALG == WV_QUANT_LOG ? q.n_neg : 0);
   const bool tab_in_lds = ntab <= WV_LDS_TABLE;
   if (ALG != WV_QUANT_UNIFORM && tab_in_lds)
@@ -109,8 +112,8 @@ quant_hist_kernel(const float *coef, int64_t n, QuantDev q,
   const float4 *c4 = (const float4 *)coef;
   int4 *b4 = (int4 *)bins;
   // four independent 16-byte loads per thread in flight per iteration
-  const int64_t stride = (int64_t)gridDim.x * WV_BLOCK;
-  int64_t i0 = (int64_t)blockIdx.x * WV_BLOCK + threadIdx.x;
+  const int64_t stride = (int64_t)nblk * WV_BLOCK;
+  int64_t i0 = (int64_t)bidx * WV_BLOCK + threadIdx.x;
   for (; i0 + 3 * stride < nvec; i0 += 4 * stride) {
     float4 vv[4];
 #pragma unroll
@@ -138,8 +141,8 @@ quant_hist_kernel(const float *coef, int64_t n, QuantDev q,
     count(r.x); count(r.y); count(r.z); count(r.w);
   }
   // tail
-  for (int64_t i = nvec * 4 + (int64_t)blockIdx.x * WV_BLOCK + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * WV_BLOCK) {
+  for (int64_t i = nvec * 4 + (int64_t)bidx * WV_BLOCK + threadIdx.x; i < n;
+       i += (int64_t)nblk * WV_BLOCK) {
     const int r = quant_one<ALG>(q, t, coef[i]);
     bins[i] = r;
     count(r);
@@ -154,6 +157,15 @@ quant_hist_kernel(const float *coef, int64_t n, QuantDev q,
   if (hist_in_lds)
     for (int i = threadIdx.x; i < q.bin_count; i += WV_BLOCK)
       if (h[i]) atomicAdd(&counts[i], h[i]);
+}
+
+template <int ALG>
+__global__ void __launch_bounds__(WV_BLOCK)
+quant_hist_kernel(const float *coef, int64_t n, QuantDev q,
+                  int32_t *bins, uint32_t *__restrict__ counts) {
+  __shared__ float tab[WV_LDS_TABLE];
+  __shared__ uint32_t h[WV_LDS_HIST];
+  quant_hist_body<ALG>(coef, n, q, bins, counts, blockIdx.x, gridDim.x, tab, h);
 }
 
 __global__ void zero_u32(uint32_t *p, int n) {
@@ -311,6 +323,71 @@ hipError_t dequantize_table(const int32_t *bins, int64_t n, const float *table,
                             int bin_count, float *out, hipStream_t s) {
   hipLaunchKernelGGL(dequant_kernel, dim3(grid_for(n * 4)), dim3(WV_BLOCK), 0, s,
                      bins, n, table, bin_count, out);
+  return hipGetLastError();
+}
+
+// ---- batches -------------------------------------------------------------
+__device__ inline int find_qjob(const int *blk0, int njobs, int b) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (blk0[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// one block per job: counts[0 .. bin_count] = 0, and the uniform
+// quantizer's parameters from the job's threshold result (the float
+// expressions of QuantUniform::init, quant.h:227-239)
+__global__ void bquant_prepare(const BQuantJob *J) {
+  const BQuantJob &j = J[blockIdx.x];
+  for (int i = threadIdx.x; i <= j.bin_count; i += blockDim.x) j.counts[i] = 0;
+  if (threadIdx.x == 0 && j.alg == WV_QUANT_UNIFORM) {
+    const wv_thresh_result *res = j.res;
+    const float threshold = res->threshold, maxv = res->max_abs;
+    const int bins = j.bin_count;
+    const int negBins = (bins - 1) / 2, posBins = bins / 2;
+    QuantDev o;
+    o.alg = WV_QUANT_UNIFORM;
+    o.bin_count = bins;
+    o.zero_bin = (bins - 1) >> 1;
+    o.threshold = threshold;
+    o.neg_scale = (float)(negBins - 1) / (maxv - threshold);
+    o.pos_scale = (float)(posBins - 1) / (maxv - threshold);
+    o.neg_offset = 1 - threshold * o.neg_scale;
+    o.pos_offset = 1 - threshold * o.pos_scale;
+    o.n_pos = o.n_neg = 0;
+    o.pos_clamp = 0.f;
+    o.table = nullptr;
+    *j.q = o;
+  }
+}
+
+__global__ void __launch_bounds__(WV_BLOCK)
+bquant_hist(const BQuantJob *J, const int *blk0, int njobs) {
+  const int jb = find_qjob(blk0, njobs, blockIdx.x);
+  __shared__ float tab[WV_LDS_TABLE];
+  __shared__ uint32_t h[WV_LDS_HIST];
+  const BQuantJob &j = J[jb];
+  const QuantDev q = *j.q;
+  const int bidx = (int)blockIdx.x - j.blk0;
+  if (q.alg == WV_QUANT_UNIFORM)
+    quant_hist_body<WV_QUANT_UNIFORM>(j.coef, j.n, q, j.bins, j.counts, bidx, j.nblk, tab, h);
+  else if (q.alg == WV_QUANT_LOG)
+    quant_hist_body<WV_QUANT_LOG>(j.coef, j.n, q, j.bins, j.counts, bidx, j.nblk, tab, h);
+  else
+    quant_hist_body<WV_QUANT_LLOYD>(j.coef, j.n, q, j.bins, j.counts, bidx, j.nblk, tab, h);
+}
+
+int quant_blocks_for(int64_t n) { return (int)grid_for(n); }
+
+hipError_t quantize_hist_batch(const BQuantJob *jobs_d, const int *blk0_d, int njobs,
+                               int total_blocks, hipStream_t s) {
+  if (njobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(bquant_prepare, dim3(njobs), dim3(256), 0, s, jobs_d);
+  hipLaunchKernelGGL(bquant_hist, dim3(total_blocks), dim3(WV_BLOCK), 0, s, jobs_d, blk0_d,
+                     njobs);
   return hipGetLastError();
 }
 

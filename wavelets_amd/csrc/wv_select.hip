@@ -48,7 +48,8 @@ struct SelState {
   uint32_t crank;       // rank inside the candidate list
   uint32_t b1, b2, b3;
   uint32_t min_ord, max_ord, max_key;
-  uint32_t pad[2];
+  uint32_t overflow;    // the candidate list outgrew its capacity
+  uint32_t pad;
   uint32_t hist1[WV_H1_BINS];
   uint32_t hist2[WV_H2_BINS];
   uint32_t hist3[WV_H3_BINS];
@@ -173,8 +174,8 @@ __device__ void find_bucket(const uint32_t *cnt, uint32_t rank, uint32_t *out_b,
 }
 
 // ---- 1. sample -------------------------------------------------------------
-__global__ void __launch_bounds__(1024)
-sel_sample(const float *__restrict__ coef, int64_t n, float frac, SelState *S) {
+__device__ void sel_sample_dev(const float *__restrict__ coef, int64_t n, float frac,
+                               SelState *S) {
   __shared__ uint32_t keys[SEL_SAMPLE];
   __shared__ uint32_t h[WV_H1_BINS];
   __shared__ uint32_t res_b[2], res_r[2];
@@ -250,7 +251,13 @@ sel_sample(const float *__restrict__ coef, int64_t n, float frac, SelState *S) {
     S->min_ord = 0xffffffffu;
     S->max_ord = 0u;
     S->max_key = 0u;
+    S->overflow = 0;
   }
+}
+
+__global__ void __launch_bounds__(1024)
+sel_sample(const float *__restrict__ coef, int64_t n, float frac, SelState *S) {
+  sel_sample_dev(coef, n, frac, S);
 }
 
 // ---- 2. collect ------------------------------------------------------------
@@ -258,9 +265,8 @@ sel_sample(const float *__restrict__ coef, int64_t n, float frac, SelState *S) {
 // SEL_CHUNK keys (16-byte loads, 16 keys per thread).  Per-block results go
 // to S->part (no same-address global atomics); the candidate list is
 // reserved with one atomic per LDS-buffer flush.
-__global__ void __launch_bounds__(WV_BLOCK)
-sel_collect(const float *__restrict__ coef, int64_t n, SelState *S,
-            uint32_t *cand) {
+__device__ void sel_collect_dev(const float *__restrict__ coef, int64_t n, SelState *S,
+                                uint32_t *cand, uint32_t cap, int bidx, int nblk) {
   __shared__ uint32_t buf[SEL_CAND_CAP];
   __shared__ uint32_t cnt, base;
   __shared__ uint32_t red[4][WV_BLOCK / 64];
@@ -269,8 +275,8 @@ sel_collect(const float *__restrict__ coef, int64_t n, SelState *S,
   __syncthreads();
   const uint32_t klo = S->key_lo, khi = S->key_hi;
   const int64_t nchunk = (n + SEL_CHUNK - 1) / SEL_CHUNK;
-  const int64_t c0 = nchunk * blockIdx.x / gridDim.x;
-  const int64_t c1 = nchunk * (blockIdx.x + 1) / gridDim.x;
+  const int64_t c0 = nchunk * bidx / nblk;
+  const int64_t c1 = nchunk * (bidx + 1) / nblk;
   uint32_t nb = 0, mn = 0xffffffffu, mx = 0, mk = 0;
   for (int64_t c = c0; c < c1; c++) {
     const int64_t start = c * SEL_CHUNK;
@@ -314,9 +320,13 @@ sel_collect(const float *__restrict__ coef, int64_t n, SelState *S,
     // flush when the next sub-chunk might not fit
     if (cnt > SEL_CAND_CAP - SEL_CHUNK || c == c1 - 1) {
       const uint32_t k = cnt;
-      if (tid == 0 && k) base = atomicAdd(&S->ncand, k);
+      if (tid == 0 && k) {
+        base = atomicAdd(&S->ncand, k);
+        if (base + k > cap) S->overflow = 1;
+      }
       __syncthreads();
-      for (uint32_t i = tid; i < k; i += WV_BLOCK) cand[base + i] = buf[i];
+      if (base + k <= cap)
+        for (uint32_t i = tid; i < k; i += WV_BLOCK) cand[base + i] = buf[i];
       __syncthreads();
       if (tid == 0) cnt = 0;
       __syncthreads();
@@ -337,15 +347,20 @@ sel_collect(const float *__restrict__ coef, int64_t n, SelState *S,
       mx = max(mx, red[2][w]);
       mk = max(mk, red[3][w]);
     }
-    S->part[blockIdx.x][0] = nb;
-    S->part[blockIdx.x][1] = mn;
-    S->part[blockIdx.x][2] = mx;
-    S->part[blockIdx.x][3] = mk;
+    S->part[bidx][0] = nb;
+    S->part[bidx][1] = mn;
+    S->part[bidx][2] = mx;
+    S->part[bidx][3] = mk;
   }
 }
 
+__global__ void __launch_bounds__(WV_BLOCK)
+sel_collect(const float *__restrict__ coef, int64_t n, SelState *S, uint32_t *cand) {
+  sel_collect_dev(coef, n, S, cand, 0xffffffffu, blockIdx.x, gridDim.x);
+}
+
 // ---- 3. verify -------------------------------------------------------------
-__global__ void __launch_bounds__(1024) sel_verify(SelState *S, int nparts) {
+__device__ void sel_verify_dev(SelState *S, int nparts) {
   __shared__ uint32_t r[4][16];
   const int t = threadIdx.x;
   uint32_t nb = 0, mn = 0xffffffffu, mx = 0, mk = 0;
@@ -376,12 +391,16 @@ __global__ void __launch_bounds__(1024) sel_verify(SelState *S, int nparts) {
   S->max_key = mk;
   if (S->special) { S->crank = 0; return; }
   const uint32_t idx = (uint32_t)S->idx;
-  if (S->below <= idx && idx < S->below + S->ncand) {
+  if (!S->overflow && S->below <= idx && idx < S->below + S->ncand) {
     S->crank = idx - S->below;
   } else {
     S->fallback = 1;
     S->ncand = 0;
   }
+}
+
+__global__ void __launch_bounds__(1024) sel_verify(SelState *S, int nparts) {
+  sel_verify_dev(S, nparts);
 }
 
 // ---- fallback: exact bucket of the rank from a full histogram --------------
@@ -491,8 +510,7 @@ __global__ void __launch_bounds__(1024) sel_cscan(SelState *S) {
   }
 }
 
-__global__ void sel_result(int64_t n, const SelState *S, wv_thresh_result *res) {
-  if (threadIdx.x != 0) return;
+__device__ void sel_result_dev(int64_t n, const SelState *S, wv_thresh_result *res) {
   wv_thresh_result o;
   // Traverse starts min at FLT_MAX and max at FLT_MIN (thresh_cpu.cc:67-68)
   const float mn = wv_ord2f(S->min_ord), mx = wv_ord2f(S->max_ord);
@@ -520,6 +538,10 @@ __global__ void sel_result(int64_t n, const SelState *S, wv_thresh_result *res) 
   o.n_candidates = (int32_t)(S->ncand & 0x7fffffffu) |
                    (S->fallback ? (int32_t)0x80000000 : 0);
   *res = o;
+}
+
+__global__ void sel_result(int64_t n, const SelState *S, wv_thresh_result *res) {
+  if (threadIdx.x == 0) sel_result_dev(n, S, res);
 }
 
 hipError_t threshold_select(const float *coef, int64_t n, float frac,
@@ -550,6 +572,91 @@ hipError_t threshold_select(const float *coef, int64_t n, float frac,
   hipLaunchKernelGGL(sel_chist<3>, dim3(256), dim3(WV_BLOCK), 0, s, cand, S);
   hipLaunchKernelGGL(sel_cscan<3>, dim3(1), dim3(1024), 0, s, S);
   hipLaunchKernelGGL(sel_result, dim3(1), dim3(64), 0, s, n, S, res);
+  return hipGetLastError();
+}
+
+// ---- batches: one select per cubelet, all cubelets in three launches -----
+// Block b of the collect launch serves the last job j with blk0[j] <= b.
+__device__ inline int find_job(const int *blk0, int njobs, int b) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (blk0[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(1024) bsel_sample(const BSelJob *J) {
+  const BSelJob &j = J[blockIdx.x];
+  sel_sample_dev(j.coef, j.n, j.frac, (SelState *)j.state);
+}
+
+__global__ void __launch_bounds__(WV_BLOCK)
+bsel_collect(const BSelJob *J, const int *blk0, int njobs) {
+  const int jb = find_job(blk0, njobs, blockIdx.x);
+  const BSelJob &j = J[jb];
+  sel_collect_dev(j.coef, j.n, (SelState *)j.state, j.cand, j.cap,
+                  (int)blockIdx.x - j.blk0, j.nblk);
+}
+
+// Verify the window, then a three-digit radix select inside the block over
+// the candidates -- or, when the window missed the rank or the candidate
+// list overflowed, over every key of the cubelet -- and the result record.
+__global__ void __launch_bounds__(1024) bsel_finish(const BSelJob *J) {
+  __shared__ uint32_t h[WV_H1_BINS];
+  __shared__ uint32_t rb, rr;
+  const BSelJob &j = J[blockIdx.x];
+  SelState *S = (SelState *)j.state;
+  sel_verify_dev(S, j.nblk);
+  __syncthreads();
+  if (!S->special) {
+    const bool fb = S->fallback != 0;
+    const uint32_t *src = fb ? (const uint32_t *)j.coef : j.cand;
+    const int64_t cnt = fb ? j.n : (int64_t)S->ncand;
+    uint32_t rank = fb ? (uint32_t)S->idx : S->crank;
+    uint32_t prefix = 0, pmask = 0;
+    for (int d = 0; d < 3; d++) {
+      const int shift = d == 0 ? WV_H1_SHIFT : d == 1 ? WV_H2_SHIFT : 0;
+      const uint32_t nb = d == 2 ? WV_H3_BINS : WV_H1_BINS;
+      for (int i = threadIdx.x; i < WV_H1_BINS; i += 1024) h[i] = 0;
+      if (threadIdx.x == 0) { rb = 0; rr = 0; }
+      __syncthreads();
+      for (int64_t i0 = 0; i0 < cnt; i0 += 1024) {
+        const int64_t i = i0 + threadIdx.x;
+        const uint32_t key = i < cnt ? (src[i] & 0x7fffffffu) : 0u;
+        const bool ok = i < cnt && (key & pmask) == prefix;
+        wv_wave_hist_add<WV_H1_BITS>(h, (key >> shift) & (nb - 1), ok);
+      }
+      __syncthreads();
+      find_bucket<WV_H1_BINS>(h, rank, &rb, &rr);
+      const uint32_t b = rb;
+      rank = rr;
+      prefix |= b << shift;
+      pmask |= (nb - 1) << shift;
+      if (threadIdx.x == 0) {
+        if (d == 0) S->b1 = b;
+        else if (d == 1) S->b2 = b;
+        else S->b3 = b;
+      }
+      __syncthreads();
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) sel_result_dev(j.n, S, j.res);
+}
+
+int64_t sel_state_bytes() { return (int64_t)((sizeof(SelState) + 255) & ~(size_t)255); }
+int sel_max_collect_blocks() { return SEL_GRID; }
+int64_t sel_collect_chunk() { return SEL_CHUNK; }
+
+hipError_t threshold_select_batch(const BSelJob *jobs_d, const int *blk0_d, int njobs,
+                                  int total_blocks, hipStream_t s) {
+  if (njobs <= 0) return hipSuccess;
+  hipLaunchKernelGGL(bsel_sample, dim3(njobs), dim3(1024), 0, s, jobs_d);
+  hipLaunchKernelGGL(bsel_collect, dim3(total_blocks), dim3(WV_BLOCK), 0, s, jobs_d, blk0_d,
+                     njobs);
+  hipLaunchKernelGGL(bsel_finish, dim3(njobs), dim3(1024), 0, s, jobs_d);
   return hipGetLastError();
 }
 
