@@ -1,15 +1,26 @@
 """Benchmark of the MI355X Wavelets compression hot path.
 
-Default workload (one GPU): the 512^3 float32 cubelet (40 ellipsoids +
-N(0,20^2) noise), CDF 9/7 at 3,3,3 steps, keep the largest 5% of |coef|
-(-thresh 0.95), uniform quantizer with 1024 bins.  One step is the whole
-device-resident pipeline on that cubelet with the input already in HBM:
-fused-axis-pass DWT (threshold statistics fused into the last pass), radix
-select of the threshold, quantization fused with the bin histogram.
+Default workload (one GPU): the batched launch -- a mixed stream of 2000
+cubelets, sizes drawn from seed from {32^3, 64^3, 128^3, 256x256x64,
+512x512x16}, dtypes uint8 / int16 / float32, CDF 9/7 or Haar (50/50) at
+min(3, log2 dim) steps per axis, log quantizer with 256 bins, threshold
+fraction 0.9, contents a smooth blob field plus noise generated on the
+device.  One step is the whole device-resident batch (transform, threshold
+select, quantize + histogram) with the inputs already in HBM; value is the
+stream's input bytes per second.  Beside it, on the same stream:
 
-Prints one JSON line: input GB/s, the DWT pass roofline (algorithmic bytes /
-average pass duration from HIP events recorded on the pipeline's own stream)
-and the CPU oracle timed on a bounded sample on this host.
+  plain     the single-cubelet drop-in (wv_compress_device) run cubelet by
+            cubelet over the first cubelets of the stream, and whether its
+            bins equal the batch's;
+  single    the 512^3 float32 cubelet (40 ellipsoids + N(0,20^2) noise),
+            CDF 9/7 3,3,3, keep 5%, uniform 1024 bins, whose fused-axis
+            DWT pass is the roofline kernel: its algorithmic bytes (read 4 B
+            + write 4 B per sample) over the average pass duration from HIP
+            events recorded on the pipeline's own stream;
+  cpu       the oracle (single-threaded C restatement of the reference CPU
+            path) on a bounded sample of the 512^3 cubelet.
+
+Prints one JSON line.
 """
 import argparse
 import ctypes as C
@@ -24,18 +35,27 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+ITEMSIZE = {"uint8": 1, "int16": 2, "float32": 4}
 
 
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["batched", "single"], default="batched")
+    ap.add_argument("--count", type=int, default=2000, help="cubelets in the mixed stream")
+    ap.add_argument("--plain-count", type=int, default=400,
+                    help="cubelets of the stream timed through the plain path")
+    ap.add_argument("--size", type=int, default=512, help="edge of the single cubelet")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--extras", default="e2e",
-                    help="comma list of extra measurements: e2e,batched,serving")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--extras", default="e2e", help="comma list: e2e (single-cubelet one-call)")
     return ap.parse_args()
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
 
 
 def cpu_baseline(vol_sample):
@@ -52,50 +72,33 @@ def cpu_baseline(vol_sample):
     nbytes = vol_sample.nbytes
     return {"value": round(nbytes / t / 1e9, 4), "unit": "GB/s", "cores": 1,
             "kind": "port",
-            "sample": "%dx%dx%d float32 sub-volume of the same cubelet, CDF97 3,3,3, "
+            "sample": "%dx%dx%d float32 sub-volume of the 512^3 cubelet, CDF97 3,3,3, "
                       "thresh 0.95, uniform 1024; DWT+threshold(sort)+quantize "
                       "= %.2f s single-threaded (Huffman %.2f s not counted)" % (
                           vol_sample.shape[2], vol_sample.shape[1], vol_sample.shape[0],
                           t, r.t_huff)}
 
 
-def main():
-    args = parse()
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(cubelets shard across GPUs with no exchange; see DESIGN.md)",
-              file=sys.stderr)
-        sys.exit(2)
-    import torch
-    import wavelets_amd as wv
+def single_cubelet(args, wv, torch, dev, steps, warmup):
+    """The 512^3 single-cubelet device pipeline and the DWT pass roofline."""
     from wavelets_amd import synth
     L = wv.lib()
-    dev = torch.device("cuda:0")
-    torch.cuda.set_device(dev)
-
     n = args.size
     t0 = time.time()
     vol = synth.ellipsoids_f32(n=n, seed=2)
-    gen_s = time.time() - t0
-    print("generated %d^3 float32 in %.1f s" % (n, gen_s), file=sys.stderr, flush=True)
+    log("generated %d^3 float32 in %.1f s" % (n, time.time() - t0))
     vol_d = torch.from_numpy(vol).to(dev)
     N = vol.size
-    in_bytes = vol.nbytes
-
     g = wv.geom([n, n, n], [3, 3, 3], wv.CDF97, wv.FLOAT32)
     coef = torch.empty(g.count, dtype=torch.float32, device=dev)
-    ws = torch.empty(max(g.ws_floats, wv.lib().wv_threshold_ws_bytes(g.count) // 4 + 1),
+    ws = torch.empty(max(g.ws_floats, L.wv_threshold_ws_bytes(g.count) // 4 + 1),
                      dtype=torch.float32, device=dev)
-    stats = torch.empty(wv.STATS_BYTES, dtype=torch.uint8, device=dev)
     res_d = torch.zeros(64, dtype=torch.uint8, device=dev)
     bins = torch.empty(g.count, dtype=torch.int32, device=dev)
     counts = torch.empty(1024 + 128, dtype=torch.int32, device=dev)
     stream = torch.cuda.current_stream()
     sp = C.c_void_p(stream.cuda_stream)
     npass = L.wv_dwt3d_num_passes(C.byref(g))
-
-    def make_events():
-        return [wv.Event() for _ in range(npass + 1)]
 
     def step(evs):
         arr = (C.c_void_p * (npass + 1))(*[e.h for e in evs])
@@ -109,39 +112,42 @@ def main():
             C.c_void_p(coef.data_ptr()), g.count, 1024, C.c_void_p(res_d.data_ptr()),
             C.c_void_p(bins.data_ptr()), C.c_void_p(counts.data_ptr()), sp))
 
-    for _ in range(args.warmup):
-        step(make_events())
+    for _ in range(warmup):
+        step([wv.Event() for _ in range(npass + 1)])
     torch.cuda.synchronize()
-    evsets = [make_events() for _ in range(args.steps)]
+    evsets = [[wv.Event() for _ in range(npass + 1)] for _ in range(steps)]
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
-    t_wall = time.time()
     start.record(stream)
-    for k in range(args.steps):
+    for k in range(steps):
         step(evsets[k])
     stop.record(stream)
     torch.cuda.synchronize()
-    t_wall = time.time() - t_wall
-    ms_total = start.elapsed_time(stop)
-    ms_step = ms_total / args.steps
+    ms_step = start.elapsed_time(stop) / steps
     pass_ms = np.zeros(npass)
     for evs in evsets:
         for i in range(npass):
             pass_ms[i] += evs[i].elapsed_ms(evs[i + 1])
-    pass_ms /= args.steps
-    # algorithmic bytes per DWT pass launch: read 4 B + write 4 B per element
-    # (float32 input on the first pass too)
-    pass_bytes = 8.0 * N
+    pass_ms /= steps
+    pass_bytes = 8.0 * N        # read 4 B + write 4 B per sample (float32 input)
     avg_pass_ms = float(pass_ms.mean())
     achieved = pass_bytes / (avg_pass_ms * 1e-3) / 1e9
-    value = in_bytes / (ms_step * 1e-3) / 1e9
-
-    _, r = wv.threshold(coef, 0.95, stats)   # final result for the record
-    extra = {}
+    _, r = wv.threshold(coef, 0.95)
+    out = {
+        "workload": "512^3 float32 cubelet (40 ellipsoids + N(0,20^2)), CDF97 3,3,3, keep 5%% "
+                    "(thresh 0.95), uniform 1024 bins" if n == 512 else "%d^3 float32" % n,
+        "gbps_input": round(vol.nbytes / (ms_step * 1e-3) / 1e9, 3),
+        "ms_per_step": round(ms_step, 4),
+        "threshold": float(r.threshold), "max_abs": float(r.max_abs),
+        "select_fallback": bool(r.n_candidates < 0),
+    }
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "dwt_pass_kernel (fused axis pass) on the 512^3 cubelet",
+            "pass_ms": [round(float(v), 4) for v in pass_ms],
+            "algorithmic_bytes_per_launch": pass_bytes}
     if "e2e" in args.extras:
-        # plain drop-in end to end: wv_compress (device stages, D2H of bins
-        # and counts, host Huffman + .cube image)
         opt = wv.Options(wv.CDF97, wv.UNIFORM, 1024, (C.c_int * 3)(3, 3, 3),
                          C.c_float(0.95), 1, 30, 1)
         orig = (C.c_int * 3)(n, n, n)
@@ -153,49 +159,164 @@ def main():
         wv.check(L.wv_compress(C.c_void_p(vol_d.data_ptr()), wv.FLOAT32, orig, C.byref(opt),
                                C.c_void_p(work.data_ptr()), wb, None,
                                fbuf.ctypes.data_as(C.c_void_p), cap, C.byref(info), sp))
-        extra["e2e_plain"] = {
-            "gbps_input": round(in_bytes / (info.ms_total * 1e-3) / 1e9, 3),
+        out["e2e_one_call"] = {
+            "gbps_input": round(vol.nbytes / (info.ms_total * 1e-3) / 1e9, 3),
             "ms_total": round(info.ms_total, 3),
-            "ms_device_incl_host_roundtrip": round(info.ms_device, 3),
             "file_bytes": int(info.file_bytes),
-            "note": "includes bins D2H (4 B/elt) and single-threaded host Huffman"}
+            "note": "wv_compress: device stages, bins D2H (4 B/sample), single-threaded "
+                    "host Huffman and .cube image"}
         del work
+    return out, roof, vol
+
+
+def mixed_stream(args, wv, torch, dev, steps, warmup):
+    """The batched launch on the mixed stream, and the plain path beside it."""
+    from wavelets_amd import synth
+    L = wv.lib()
+    stream = synth.mixed_stream(args.count, seed=4)
+    t0 = time.time()
+    cubes, in_bytes = [], 0
+    for i, (size, dt, wave) in enumerate(stream):
+        vol = synth.cubelet_device(size, dt, 1000 + i, device=dev)
+        in_bytes += vol.numel() * ITEMSIZE[dt]
+        cubes.append(dict(vol=vol, wavelet=wave, steps=synth.stream_steps(size),
+                          alg=wv.LOG, bins=256, frac=0.9))
+    torch.cuda.synchronize()
+    log("generated %d cubelets (%.2f GB) in %.1f s" % (len(cubes), in_bytes / 1e9,
+                                                      time.time() - t0))
+    t0 = time.time()
+    flags = 0 if args.no_autotune else wv.BATCH_AUTOTUNE
+    b = wv.Batch(cubes, flags=flags)
+    log("planned batch (%d groups%s) in %.1f s" % (
+        len(b.groups()), "" if args.no_autotune else ", autotuned", time.time() - t0))
+    for _ in range(max(1, warmup)):
+        b.submit()
+        b.wait()
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for _ in range(steps):
+        b.submit()
+    b.wait()
+    torch.cuda.synchronize()
+    ms_batch = (time.time() - t0) * 1e3 / steps
+    batched_gbps = in_bytes / (ms_batch * 1e-3) / 1e9
+    log("batched: %.3f ms per stream, %.1f GB/s" % (ms_batch, batched_gbps))
+
+    # plain path: one cubelet at a time through the drop-in device stages
+    P = min(args.plain_count, len(cubes))
+    opt_of = {}
+    work_bytes = 0
+    for c in cubes[:P]:
+        v = c["vol"]
+        orig = (C.c_int * 3)(v.shape[2], v.shape[1], v.shape[0])
+        opt = wv.Options(c["wavelet"], c["alg"], c["bins"], (C.c_int * 3)(*c["steps"]),
+                         C.c_float(c["frac"]), 1, 30, 1)
+        opt_of[id(c)] = (orig, opt)
+        work_bytes = max(work_bytes, L.wv_compress_work_bytes(orig, C.byref(opt)))
+    work = torch.empty(work_bytes, dtype=torch.uint8, device=dev)
+    sp = C.c_void_p(torch.cuda.current_stream().cuda_stream)
+    info = wv.CompressInfo()
+
+    def plain_one(c):
+        orig, opt = opt_of[id(c)]
+        wv.check(L.wv_compress_device(C.c_void_p(c["vol"].data_ptr()), wv.dtype_id(c["vol"]),
+                                      orig, C.byref(opt), C.c_void_p(work.data_ptr()),
+                                      work_bytes, C.byref(info), sp))
+
+    for c in cubes[:min(P, 20)]:
+        plain_one(c)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    for c in cubes[:P]:
+        plain_one(c)
+    torch.cuda.synchronize()
+    ms_plain = (time.time() - t0) * 1e3
+    plain_bytes = sum(c["vol"].numel() * c["vol"].element_size() for c in cubes[:P])
+    plain_gbps = plain_bytes / (ms_plain * 1e-3) / 1e9
+    log("plain: %d cubelets in %.1f ms, %.1f GB/s" % (P, ms_plain, plain_gbps))
+    # identical results: the plain bins of a spread of cubelets against the batch's
+    same = True
+    for i in list(range(0, P, max(1, P // 25))):
+        plain_one(cubes[i])
+        torch.cuda.synchronize()
+        n = b.bins[i].numel()
+        pb = work[:n * 4].view(torch.int32)
+        if not torch.equal(pb, b.bins[i]):
+            same = False
+    out = {
+        "workload": "mixed stream: %d cubelets {32^3, 64^3, 128^3, 256x256x64, 512x512x16}, "
+                    "u8/i16/f32, CDF97|Haar, min(3,log2 dim) steps, log 256 bins, "
+                    "thresh 0.9" % len(cubes),
+        "cubelets": len(cubes), "input_bytes": int(in_bytes),
+        "samples": int(sum(c["vol"].numel() for c in cubes)),
+        "batched_ms_per_stream": round(ms_batch, 3),
+        "batched_gbps": round(batched_gbps, 3),
+        "plain_gbps": round(plain_gbps, 3),
+        "plain_cubelets_timed": P,
+        "batched_over_plain": round(batched_gbps / plain_gbps, 3),
+        "plain_bins_equal_batched": same,
+        "groups": len(b.groups()),
+    }
+    b.close()
+    del cubes, work
+    torch.cuda.empty_cache()
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(cubelets shard across GPUs with no exchange; see DESIGN.md)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import wavelets_amd as wv
+    wv.lib()
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+
+    single, roof, vol = single_cubelet(args, wv, torch, dev,
+                                       args.steps if args.workload == "single" else 10,
+                                       args.warmup)
+    batched = None
+    if args.workload == "batched":
+        batched = mixed_stream(args, wv, torch, dev, args.steps, args.warmup)
 
     cpu = None
     if not args.no_cpu:
-        q = max(32, n // 2)
-        sample = np.ascontiguousarray(vol[:q, :q, :])   # [z, y, x] = 512 x q x q
+        q = max(32, args.size // 2)
+        sample = np.ascontiguousarray(vol[:q, :q, :])
         cpu = cpu_baseline(sample)
 
+    if batched is not None:
+        value, ms_step = batched["batched_gbps"], batched["batched_ms_per_stream"]
+        workload = batched["workload"]
+        dtype = "f32"
+    else:
+        value, ms_step = single["gbps_input"], single["ms_per_step"]
+        workload = single["workload"]
+        dtype = "f32"
     line = {
-        "metric": "input GB/s compressed (device-resident pipeline, plain single cubelet)",
-        "select_fallback": bool(r.n_candidates & 0x80000000 if r.n_candidates >= 0 else True),
-        "value": round(value, 3),
+        "metric": "input GB/s compressed, batched vs plain; p50/p99 latency at fixed req/s",
+        "value": value,
         "unit": "GB/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": ms_step,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded 40-ellipsoid volume + Gaussian noise)",
-        "config": {"workload": "512^3 float32 cubelet, CDF97 3,3,3, keep 5% "
-                               "(thresh 0.95), uniform 1024 bins" if n == 512 else
-                   "%d^3 float32 cubelet" % n,
-                   "stages": "dwt(3 fused axis passes)+sampled-window select+quantize/histogram",
-                   "threshold": float(r.threshold), "max_abs": float(r.max_abs)},
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": None,
-                     "kernel": "dwt_pass_kernel (fused axis pass)",
-                     "pass_ms": [round(float(v), 4) for v in pass_ms],
-                     "algorithmic_bytes_per_launch": pass_bytes},
+        "dtype": dtype,
+        "data": "synthetic (seeded; generated on the host / device, inputs resident in HBM)",
+        "config": {"workload": workload},
+        "roofline": roof,
         "cpu_baseline": cpu,
-        "wall_s": round(t_wall, 3),
+        "single_cubelet": single,
     }
-    line.update(extra)
+    if batched is not None:
+        line["batched"] = batched
     print(json.dumps(line), flush=True)
 
 

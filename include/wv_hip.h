@@ -276,6 +276,16 @@ int wv_compress(const void *in_d, int dtype, const int orig[3],
                 wv_compress_info *info, void *stream);
 /* device bytes wv_compress needs in work_d */
 int64_t wv_compress_work_bytes(const int orig[3], const wv_options *opt);
+/* The device-resident part of wv_compress alone (compressFile's GPU
+   stages, test_compress_gpu.cu:180-402, without the bins D2H and the host
+   coder): on return the quantize + histogram may still run on stream; bins
+   (int32, padded count) are at offset 0 of work_d and the bin_count + 1
+   counts at wv_compress_counts_offset.  Synchronises once, for the
+   threshold result. */
+int wv_compress_device(const void *in_d, int dtype, const int orig[3],
+                       const wv_options *opt, void *work_d, int64_t work_bytes,
+                       wv_compress_info *info, void *stream);
+int64_t wv_compress_counts_offset(const int orig[3], const wv_options *opt);
 
 /* ------------------------------------------------------------------ *
  * Batched compression of many cubelets of uneven size.  No counterpart in

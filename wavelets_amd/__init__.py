@@ -127,6 +127,10 @@ def _declare(L):
                                   P, I64, C.POINTER(CompressInfo), P]
         L.wv_compress_work_bytes.restype = I64
         L.wv_compress_work_bytes.argtypes = [P, C.POINTER(Options)]
+        L.wv_compress_device.argtypes = [P, C.c_int, P, C.POINTER(Options), P, I64,
+                                         C.POINTER(CompressInfo), P]
+        L.wv_compress_counts_offset.restype = I64
+        L.wv_compress_counts_offset.argtypes = [P, C.POINTER(Options)]
     if hasattr(L, "wv_batch_plan"):
         L.wv_batch_plan.argtypes = [C.POINTER(BatchItem), C.c_int, C.c_int,
                                     C.POINTER(C.c_void_p)]

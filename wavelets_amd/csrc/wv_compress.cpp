@@ -59,59 +59,113 @@ extern "C" int64_t wv_compress_work_bytes(const int orig[3], const wv_options *o
     }                                                             \
   } while (0)
 
+namespace {
+
+// The device-resident stages of one cubelet: transform, threshold select
+// (one synchronisation for its result), [Lloyd], quantizer tables, fused
+// quantize + histogram into the work buffer (bins over the coefficients,
+// counts at L.counts).  The quantize is left running on the stream.
+struct Stages {
+  wv_geom g;
+  Layout L;
+  wv_thresh_result res;
+  wv_quant_params qp;
+  bool have_qp = false;
+  std::vector<float> bounds, values;
+  int lloyd_iters = 0;
+  ~Stages() {
+    if (have_qp) wv_quant_params_free(&qp);
+  }
+};
+
+int device_stages(const void *in_d, int dtype, const int orig[3], const wv_options *opt,
+                  void *work_d, int64_t work_bytes, hipStream_t s, Stages &st) {
+  if (!in_d || !orig || !opt || !work_d) { set_error("null argument"); return WV_ERR_ARG; }
+  if (opt->bin_count < 2 || opt->bin_count > WV_MAX_BINS) { set_error("bin count"); return WV_ERR_ARG; }
+  CHK(wv_geom_init(&st.g, orig, opt->steps, opt->wavelet, dtype));
+  const wv_geom &g = st.g;
+  st.L = layout_for(g, opt);
+  const Layout &L = st.L;
+  if (work_bytes < L.total) {
+    set_error("work buffer %lld < %lld", (long long)work_bytes, (long long)L.total);
+    return WV_ERR_NOMEM;
+  }
+  char *w = (char *)work_d;
+  float *coef = (float *)(w + L.coef);
+  float *ws = (float *)(w + L.ws);
+  wv_thresh_result *res_d = (wv_thresh_result *)(w + L.result);
+  uint32_t *counts_d = (uint32_t *)(w + L.counts);
+  void *table_d = w + L.table;
+
+  CHK(wv_dwt3d_forward(&g, in_d, coef, ws, nullptr, s));
+  // the threshold select needs no precomputed statistics
+  CHK(wv_threshold(coef, g.count, opt->thresh_frac, nullptr, ws, res_d, s));
+  HCHK(hipMemcpyAsync(&st.res, res_d, sizeof st.res, hipMemcpyDeviceToHost, s), "D2H threshold");
+  HCHK(hipStreamSynchronize(s), "sync");
+  const wv_thresh_result &res = st.res;
+  if (opt->quant_alg == WV_QUANT_LLOYD) {
+    st.bounds.resize(opt->bin_count - 1);
+    st.values.resize(opt->bin_count);
+    CHK(wv_lloyd(coef, g.count, &res, opt->bin_count, opt->lloyd_max_iters, w + L.lloyd,
+                 st.bounds.data(), st.values.data(), &st.lloyd_iters, s));
+    CHK(wv_quant_params_init(&st.qp, WV_QUANT_LLOYD, opt->bin_count, res.threshold, 0,
+                             st.bounds.data(), st.values.data()));
+  } else {
+    // QuantUniform / QuantLog with maxValue = max |x| (test_compress_cpu.cc:262-265)
+    CHK(wv_quant_params_init(&st.qp, opt->quant_alg, opt->bin_count, res.threshold,
+                             res.max_abs, nullptr, nullptr));
+  }
+  st.have_qp = true;
+  CHK(wv_quant_params_upload(&st.qp, table_d, s));
+  CHK(wv_quantize_hist(coef, g.count, &st.qp, (int32_t *)coef, counts_d, s));
+  return WV_OK;
+}
+
+}  // namespace
+
+extern "C" int64_t wv_compress_counts_offset(const int orig[3], const wv_options *opt) {
+  wv_geom g;
+  if (wv_geom_init(&g, orig, opt->steps, opt->wavelet, WV_DTYPE_FLOAT32) != WV_OK) return -1;
+  return layout_for(g, opt).counts;
+}
+
+extern "C" int wv_compress_device(const void *in_d, int dtype, const int orig[3],
+                                  const wv_options *opt, void *work_d, int64_t work_bytes,
+                                  wv_compress_info *info, void *stream) {
+  if (!info) { set_error("null argument"); return WV_ERR_ARG; }
+  std::memset(info, 0, sizeof *info);
+  Stages st;
+  CHK(device_stages(in_d, dtype, orig, opt, work_d, work_bytes, (hipStream_t)stream, st));
+  info->thresh = st.res;
+  std::memcpy(info->out_size, st.g.out_size, sizeof st.g.out_size);
+  std::memcpy(info->steps, st.g.steps, sizeof st.g.steps);
+  info->zero_bin = st.qp.zero_bin;
+  info->lloyd_iters = st.lloyd_iters;
+  return WV_OK;
+}
+
 extern "C" int wv_compress(const void *in_d, int dtype, const int orig[3],
                            const wv_options *opt, void *work_d,
                            int64_t work_bytes, int32_t *bins_host,
                            uint8_t *file_buf, int64_t file_cap,
                            wv_compress_info *info, void *stream) {
-  if (!in_d || !orig || !opt || !work_d || !info) { set_error("null argument"); return WV_ERR_ARG; }
-  if (opt->bin_count < 2 || opt->bin_count > WV_MAX_BINS) { set_error("bin count"); return WV_ERR_ARG; }
+  if (!info) { set_error("null argument"); return WV_ERR_ARG; }
   const auto t0 = std::chrono::steady_clock::now();
   hipStream_t s = (hipStream_t)stream;
   std::memset(info, 0, sizeof *info);
-  wv_geom g;
-  CHK(wv_geom_init(&g, orig, opt->steps, opt->wavelet, dtype));
-  const Layout L = layout_for(g, opt);
-  if (work_bytes < L.total) { set_error("work buffer %lld < %lld", (long long)work_bytes, (long long)L.total); return WV_ERR_NOMEM; }
+  Stages st;
+  CHK(device_stages(in_d, dtype, orig, opt, work_d, work_bytes, s, st));
+  const wv_geom &g = st.g;
+  const Layout &L = st.L;
   char *w = (char *)work_d;
   float *coef = (float *)(w + L.coef);
-  float *ws = (float *)(w + L.ws);
-  void *stats = nullptr;   // the threshold select needs no precomputed statistics
-  wv_thresh_result *res_d = (wv_thresh_result *)(w + L.result);
   uint32_t *counts_d = (uint32_t *)(w + L.counts);
-  void *table_d = w + L.table;
-
-  hipEvent_t e0, e1;
-  HCHK(hipEventCreate(&e0), "event");
-  HCHK(hipEventCreate(&e1), "event");
-  HCHK(hipEventRecord(e0, s), "event record");
-
-  CHK(wv_dwt3d_forward(&g, in_d, coef, ws, nullptr, s));
-  CHK(wv_threshold(coef, g.count, opt->thresh_frac, stats, ws, res_d, s));
-  wv_thresh_result res;
-  HCHK(hipMemcpyAsync(&res, res_d, sizeof res, hipMemcpyDeviceToHost, s), "D2H threshold");
+  const wv_thresh_result &res = st.res;
+  const std::vector<float> &bounds = st.bounds, &values = st.values;
+  const int lloyd_iters = st.lloyd_iters;
+  const int zero_bin = st.qp.zero_bin;
   HCHK(hipStreamSynchronize(s), "sync");
-
-  std::vector<float> bounds, values;
-  int lloyd_iters = 0;
-  wv_quant_params qp;
-  if (opt->quant_alg == WV_QUANT_LLOYD) {
-    bounds.resize(opt->bin_count - 1);
-    values.resize(opt->bin_count);
-    CHK(wv_lloyd(coef, g.count, &res, opt->bin_count, opt->lloyd_max_iters, w + L.lloyd,
-                 bounds.data(), values.data(), &lloyd_iters, s));
-    CHK(wv_quant_params_init(&qp, WV_QUANT_LLOYD, opt->bin_count, res.threshold, 0,
-                             bounds.data(), values.data()));
-  } else {
-    // QuantUniform / QuantLog with maxValue = max |x| (test_compress_cpu.cc:262-265)
-    CHK(wv_quant_params_init(&qp, opt->quant_alg, opt->bin_count, res.threshold,
-                             res.max_abs, nullptr, nullptr));
-  }
-  int rc = wv_quant_params_upload(&qp, table_d, s);
-  if (rc == WV_OK) rc = wv_quantize_hist(coef, g.count, &qp, (int32_t *)coef, counts_d, s);
-  if (rc != WV_OK) { wv_quant_params_free(&qp); return rc; }
-  const int zero_bin = qp.zero_bin;
-  HCHK(hipEventRecord(e1, s), "event record");
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 
   std::vector<uint32_t> counts(opt->bin_count);
   std::vector<int32_t> own_bins;
@@ -120,10 +174,6 @@ extern "C" int wv_compress(const void *in_d, int dtype, const int orig[3],
   HCHK(hipMemcpyAsync(counts.data(), counts_d, counts.size() * 4, hipMemcpyDeviceToHost, s), "D2H counts");
   HCHK(hipMemcpyAsync(bins, coef, g.count * 4, hipMemcpyDeviceToHost, s), "D2H bins");
   HCHK(hipStreamSynchronize(s), "sync");
-  float ms = 0;
-  HCHK(hipEventElapsedTime(&ms, e0, e1), "elapsed");
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
 
   info->thresh = res;
   std::memcpy(info->out_size, g.out_size, sizeof g.out_size);
@@ -151,11 +201,10 @@ extern "C" int wv_compress(const void *in_d, int dtype, const int orig[3],
     d.with_footer = opt->with_footer;
     int64_t pay = 0;
     const int64_t nb = wv_write_cube(&d, bins, g.count, counts.data(), file_buf, file_cap, &pay);
-    if (nb < 0) { wv_quant_params_free(&qp); set_error("file buffer too small"); return (int)nb; }
+    if (nb < 0) { set_error("file buffer too small"); return (int)nb; }
     info->file_bytes = nb;
     info->payload_bytes = pay;
   }
-  wv_quant_params_free(&qp);
   info->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return WV_OK;
 }

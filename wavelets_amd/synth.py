@@ -134,3 +134,40 @@ def cubelet(size_xyz, dtype, seed):
     if dtype == "int16":
         return np.clip(np.rint(v * 3000 + 300), 0, 4095).astype(np.int16)
     return (v * 4000).astype(np.float32)
+
+
+def cubelet_device(size_xyz, dtype, seed, device="cuda"):
+    """cubelet() generated on the device: the blob parameters come from the
+    same numpy generator, the noise from torch's generator seeded with seed
+    (deterministic on one platform).  Used for the mixed stream, whose 2000
+    cubelets hold about 4.3 G samples."""
+    import torch
+    rng = np.random.default_rng(seed)
+    x, y, z = size_xyz
+    g = [torch.arange(n, dtype=torch.float32, device=device) for n in (z, y, x)]
+    v = torch.zeros((z, y, x), dtype=torch.float32, device=device)
+    for _ in range(4):
+        c = [rng.uniform(0, n) for n in (z, y, x)]
+        s = rng.uniform(4, max(8, max(x, y, z) / 3))
+        w = [torch.exp(-((g[k] - float(c[k])) ** 2) / float(2 * s * s)) for k in range(3)]
+        v += float(rng.uniform(0.2, 1.0)) * (w[0][:, None, None] * w[1][None, :, None] *
+                                             w[2][None, None, :])
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    v += torch.randn(v.shape, generator=gen, device=device) * 0.03
+    if dtype == "uint8":
+        return torch.clamp(torch.round(v * 200 + 20), 0, 255).to(torch.uint8)
+    if dtype == "int16":
+        return torch.clamp(torch.round(v * 3000 + 300), 0, 4095).to(torch.int16)
+    return v * 4000
+
+
+def stream_steps(size_xyz):
+    """min(3, ceilLog2(dim)) steps per axis."""
+    out = []
+    for d in size_xyz:
+        m = 0
+        while (1 << m) < d:
+            m += 1
+        out.append(min(3, m))
+    return out
