@@ -8,10 +8,11 @@
 //
 //  uniform : the float expressions of QuantUniform::quant (quant.h:244-276),
 //            contraction off, so bins equal the host's;
-//  log     : bins are assigned by comparing |x| with the host-tabulated bin
-//            edges (the smallest float of every bin under the host logf), a
-//            binary search in LDS; this reproduces QuantLog::quant
-//            (quant.h:374-406) exactly without evaluating logf on the device;
+//  log     : the bin is the number of host-tabulated bin edges (the
+//            smallest float of every bin under the host logf) at or below
+//            |x|, found in LDS from a device-logarithm guess; this reproduces
+//            QuantLog::quant (quant.h:374-406) exactly, whatever the device
+//            logarithm's rounding;
 //  codebook: lower_bound over the boundaries plus the lastBoundary rule of
 //            the host QuantCodebook::quant (quant.h:515-526).
 #include "wv_device.h"
@@ -22,15 +23,15 @@ namespace wv {
 #define WV_LDS_TABLE 4096    // floats of edge / boundary table kept in LDS
 #define WV_LDS_HIST 4096     // bins counted in LDS (more: global atomics)
 
-// number of entries of e[0..m) that are <= x  (e ascending)
-__device__ inline int count_le(const float *e, int m, float x) {
-  int lo = 0, len = m;
-  while (len > 0) {
-    const int half = len >> 1;
-    if (e[lo + half] <= x) { lo += half + 1; len -= half + 1; }
-    else len = half;
-  }
-  return lo;
+// number of entries of e[0..m) that are <= x (e ascending), from a
+// starting guess g: steps down while e[g-1] > x and up
+// while e[g] <= x.  The result depends only on the table (ascending), not on
+// the guess; a guess within a bin or two costs two or three LDS reads.
+__device__ inline int count_le_from(const float *e, int m, float x, int g) {
+  g = g < 0 ? 0 : (g > m ? m : g);
+  while (g > 0 && e[g - 1] > x) g--;
+  while (g < m && e[g] <= x) g++;
+  return g;
 }
 
 // first index with e[i] >= x (std::lower_bound)
@@ -60,14 +61,20 @@ __device__ inline int quant_one(const QuantDev &q, const float *tab, float x) {
       return (int)x;
     }
   } else if (ALG == WV_QUANT_LOG) {
+    // The bin is the number of host-tabulated edges <= |x| (each edge is the
+    // smallest float of its bin under the host logf); the device logarithm
+    // only supplies the starting guess, scale*ln|x| + offset, of that count.
+    constexpr float LN2 = 0.69314718055994531f;
     if (x < 0) {
       const float a = -x;
       if (a <= q.threshold) return zb;
-      return zb - count_le(tab + q.n_pos, q.n_neg, a);
+      const int g = (int)(q.neg_scale * (__log2f(a) * LN2) + q.neg_offset);
+      return zb - count_le_from(tab + q.n_pos, q.n_neg, a, g);
     } else {
       if (x <= q.threshold) return zb;
       if (x >= q.pos_clamp) return q.bin_count - 1;
-      return zb + count_le(tab, q.n_pos, x);
+      const int g = (int)(q.pos_scale * (__log2f(x) * LN2) + q.pos_offset + 0.5f);
+      return zb + count_le_from(tab, q.n_pos, x, g);
     }
   } else {
     const int m = q.n_pos;   // boundaries

@@ -46,13 +46,13 @@ struct SelState {
   uint32_t ncand;
   int32_t fallback;
   uint32_t crank;       // rank inside the candidate list
-  uint32_t b1, b2, b3;
+  uint32_t base, bits;  // the key lies in [base, base + 2^bits) (radix rounds)
+  uint32_t pad0;
   uint32_t min_ord, max_ord, max_key;
   uint32_t overflow;    // the candidate list outgrew its capacity
   uint32_t pad;
   uint32_t hist1[WV_H1_BINS];
-  uint32_t hist2[WV_H2_BINS];
-  uint32_t hist3[WV_H3_BINS];
+  uint32_t hist2[WV_H1_BINS];   // radix-round histogram
   uint32_t part[SEL_GRID][4];   // per collect block: below, min, max, max key
 };
 
@@ -236,8 +236,7 @@ __device__ void sel_sample_dev(const float *__restrict__ coef, int64_t n, float 
     else key_hi = (b << WV_H1_SHIFT) | (sub << WV_H2_SHIFT) | ((1u << WV_H2_SHIFT) - 1);
   }
   for (int i = t; i < WV_H1_BINS; i += 1024) S->hist1[i] = 0;
-  for (int i = t; i < WV_H2_BINS; i += 1024) S->hist2[i] = 0;
-  for (int i = t; i < WV_H3_BINS; i += 1024) S->hist3[i] = 0;
+  for (int i = t; i < WV_H1_BINS; i += 1024) S->hist2[i] = 0;
   if (t == 0) {
     S->idx = (int32_t)(special ? n : idx);
     S->special = special;
@@ -247,7 +246,9 @@ __device__ void sel_sample_dev(const float *__restrict__ coef, int64_t n, float 
     S->ncand = 0;
     S->fallback = 0;
     S->crank = 0;
-    S->b1 = S->b2 = S->b3 = 0;
+    // the radix rounds start from the window: bits covers key_hi - key_lo
+    S->base = key_lo;
+    S->bits = key_hi > key_lo ? 32u - (uint32_t)__clz((int)(key_hi - key_lo)) : 0u;
     S->min_ord = 0xffffffffu;
     S->max_ord = 0u;
     S->max_key = 0u;
@@ -414,6 +415,8 @@ __global__ void __launch_bounds__(1024) sel_fb_scan(SelState *S) {
     S->key_lo = b << WV_H1_SHIFT;
     S->key_hi = (b << WV_H1_SHIFT) | ((1u << WV_H1_SHIFT) - 1);
     S->crank = r;
+    S->base = b << WV_H1_SHIFT;
+    S->bits = WV_H1_SHIFT;
   }
   for (int i = threadIdx.x; i < WV_H1_BINS; i += 1024) S->hist1[i] = 0;
 }
@@ -445,67 +448,65 @@ sel_fb_collect(const float *__restrict__ coef, int64_t n, SelState *S,
 }
 
 // ---- 4. radix select over the candidates -----------------------------------
-// digit d of the candidates whose higher digits match; few blocks, so the
-// global histogram takes few atomics
-template <int DIGIT>
+// The key of rank crank among the candidates lies in [base, base + 2^bits).
+// A round histograms the next d = min(bits, 11) bits of (key - base) over
+// the candidates in that range, then narrows base / bits / crank to the
+// rank's bucket.  Starting from the sampled window (bits ~ 20) two rounds
+// finish; the exact fallback starts from one top-digit bucket (bits = 20).
+__device__ inline bool in_range(uint32_t key, uint32_t base, uint32_t bits) {
+  return key >= base && (uint64_t)(key - base) < (1ull << bits);
+}
+
+__device__ inline uint32_t round_digit(uint32_t key, uint32_t base, uint32_t bits) {
+  const uint32_t d = bits < 11u ? bits : 11u;
+  return ((key - base) >> (bits - d)) & ((1u << d) - 1u);
+}
+
+// few blocks, so the global histogram takes few atomics; the digits of a
+// narrowed range spread over the bins, so plain LDS atomics suffice
 __global__ void __launch_bounds__(WV_BLOCK)
-sel_chist(const uint32_t *cand, SelState *S) {
-  constexpr int NB = DIGIT == 3 ? WV_H3_BINS : 2048;
-  constexpr int BITS = DIGIT == 3 ? 9 : 11;
-  __shared__ uint32_t h[NB];
-  if (S->special) return;
+sel_rhist(const uint32_t *cand, SelState *S) {
+  __shared__ uint32_t h[WV_H1_BINS];
+  if (S->special || S->bits == 0) return;
   const uint32_t nc = S->ncand;
   const uint32_t per = (nc + gridDim.x - 1) / gridDim.x;
   const uint32_t a = blockIdx.x * per, e = min(nc, a + per);
   if (a >= e) return;
-  for (int i = threadIdx.x; i < NB; i += WV_BLOCK) h[i] = 0;
+  for (int i = threadIdx.x; i < WV_H1_BINS; i += WV_BLOCK) h[i] = 0;
   __syncthreads();
-  const uint32_t b1 = S->b1, b2 = S->b2;
+  const uint32_t base = S->base, bits = S->bits;
   for (uint32_t i0 = a; i0 < e; i0 += WV_BLOCK * 4) {
     uint32_t kk[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const uint32_t i = i0 + u * WV_BLOCK + threadIdx.x;
-      kk[u] = i < e ? cand[i] : 0u;
+      kk[u] = i < e ? cand[i] : 0xffffffffu;
     }
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const uint32_t i = i0 + u * WV_BLOCK + threadIdx.x;
-      bool ok = i < e;
       const uint32_t key = kk[u];
-      uint32_t d;
-      if (DIGIT == 1) {
-        d = key >> WV_H1_SHIFT;
-      } else if (DIGIT == 2) {
-        ok = ok && (key >> WV_H1_SHIFT) == b1;
-        d = (key >> WV_H2_SHIFT) & (WV_H2_BINS - 1);
-      } else {
-        ok = ok && (key >> WV_H1_SHIFT) == b1 &&
-             ((key >> WV_H2_SHIFT) & (WV_H2_BINS - 1)) == b2;
-        d = key & (WV_H3_BINS - 1);
-      }
-      wv_wave_hist_add<BITS>(h, d, ok);
+      if (key != 0xffffffffu && in_range(key, base, bits))
+        atomicAdd(&h[round_digit(key, base, bits)], 1u);
     }
   }
   __syncthreads();
-  uint32_t *gh = DIGIT == 1 ? S->hist1 : DIGIT == 2 ? S->hist2 : S->hist3;
-  for (int i = threadIdx.x; i < NB; i += WV_BLOCK)
-    if (h[i]) atomicAdd(&gh[i], h[i]);
+  for (int i = threadIdx.x; i < WV_H1_BINS; i += WV_BLOCK)
+    if (h[i]) atomicAdd(&S->hist2[i], h[i]);
 }
 
-template <int DIGIT>
-__global__ void __launch_bounds__(1024) sel_cscan(SelState *S) {
-  if (S->special) return;
-  constexpr int NB = DIGIT == 3 ? WV_H3_BINS : 2048;
+__global__ void __launch_bounds__(1024) sel_rscan(SelState *S) {
+  if (S->special || S->bits == 0) return;
   __shared__ uint32_t b, r;
   if (threadIdx.x == 0) { b = 0; r = 0; }
   __syncthreads();
-  const uint32_t *gh = DIGIT == 1 ? S->hist1 : DIGIT == 2 ? S->hist2 : S->hist3;
-  find_bucket<NB>(gh, S->crank, &b, &r);
+  find_bucket<WV_H1_BINS>(S->hist2, S->crank, &b, &r);
+  __syncthreads();
+  for (int i = threadIdx.x; i < WV_H1_BINS; i += 1024) S->hist2[i] = 0;
   if (threadIdx.x == 0) {
-    if (DIGIT == 1) S->b1 = b;
-    else if (DIGIT == 2) S->b2 = b;
-    else S->b3 = b;
+    const uint32_t bits = S->bits;
+    const uint32_t d = bits < 11u ? bits : 11u;
+    S->base += b << (bits - d);
+    S->bits = bits - d;
     S->crank = r;
   }
 }
@@ -526,7 +527,7 @@ __device__ void sel_result_dev(int64_t n, const SelState *S, wv_thresh_result *r
     o.thresh_idx = (int32_t)n;
     o.nonzero_count = 0;
   } else {
-    const uint32_t key = (S->b1 << WV_H1_SHIFT) | (S->b2 << WV_H2_SHIFT) | S->b3;
+    const uint32_t key = S->base;
     float t = __builtin_bit_cast(float, key);
     o.threshold_key = key;
     if (t == 0.f) t = 1e-16f;    // thresh_cpu.cc:98
@@ -564,13 +565,12 @@ hipError_t threshold_select(const float *coef, int64_t n, float frac,
   hipLaunchKernelGGL(sel_fb_scan, dim3(1), dim3(1024), 0, s, S);
   hipLaunchKernelGGL(sel_fb_collect, dim3((unsigned)fb), dim3(WV_BLOCK), 0, s, coef, n, S,
                      cand);
-  // radix select over the candidates
-  hipLaunchKernelGGL(sel_chist<1>, dim3(256), dim3(WV_BLOCK), 0, s, cand, S);
-  hipLaunchKernelGGL(sel_cscan<1>, dim3(1), dim3(1024), 0, s, S);
-  hipLaunchKernelGGL(sel_chist<2>, dim3(256), dim3(WV_BLOCK), 0, s, cand, S);
-  hipLaunchKernelGGL(sel_cscan<2>, dim3(1), dim3(1024), 0, s, S);
-  hipLaunchKernelGGL(sel_chist<3>, dim3(256), dim3(WV_BLOCK), 0, s, cand, S);
-  hipLaunchKernelGGL(sel_cscan<3>, dim3(1), dim3(1024), 0, s, S);
+  // radix rounds over the candidates (at most three: 11 + 11 + 9 bits;
+  // the sampled window usually needs two, and a spent round returns at once)
+  for (int round = 0; round < 3; round++) {
+    hipLaunchKernelGGL(sel_rhist, dim3(256), dim3(WV_BLOCK), 0, s, cand, S);
+    hipLaunchKernelGGL(sel_rscan, dim3(1), dim3(1024), 0, s, S);
+  }
   hipLaunchKernelGGL(sel_result, dim3(1), dim3(64), 0, s, n, S, res);
   return hipGetLastError();
 }
@@ -615,31 +615,26 @@ __global__ void __launch_bounds__(1024) bsel_finish(const BSelJob *J) {
     const uint32_t *src = fb ? (const uint32_t *)j.coef : j.cand;
     const int64_t cnt = fb ? j.n : (int64_t)S->ncand;
     uint32_t rank = fb ? (uint32_t)S->idx : S->crank;
-    uint32_t prefix = 0, pmask = 0;
-    for (int d = 0; d < 3; d++) {
-      const int shift = d == 0 ? WV_H1_SHIFT : d == 1 ? WV_H2_SHIFT : 0;
-      const uint32_t nb = d == 2 ? WV_H3_BINS : WV_H1_BINS;
+    uint32_t base = fb ? 0u : S->base, bits = fb ? 31u : S->bits;
+    while (bits > 0) {
       for (int i = threadIdx.x; i < WV_H1_BINS; i += 1024) h[i] = 0;
       if (threadIdx.x == 0) { rb = 0; rr = 0; }
       __syncthreads();
-      for (int64_t i0 = 0; i0 < cnt; i0 += 1024) {
-        const int64_t i = i0 + threadIdx.x;
-        const uint32_t key = i < cnt ? (src[i] & 0x7fffffffu) : 0u;
-        const bool ok = i < cnt && (key & pmask) == prefix;
-        wv_wave_hist_add<WV_H1_BITS>(h, (key >> shift) & (nb - 1), ok);
+      for (int64_t i = threadIdx.x; i < cnt; i += 1024) {
+        const uint32_t key = src[i] & 0x7fffffffu;
+        if (in_range(key, base, bits)) atomicAdd(&h[round_digit(key, base, bits)], 1u);
       }
       __syncthreads();
       find_bucket<WV_H1_BINS>(h, rank, &rb, &rr);
-      const uint32_t b = rb;
+      const uint32_t d = bits < 11u ? bits : 11u;
+      base += rb << (bits - d);
+      bits -= d;
       rank = rr;
-      prefix |= b << shift;
-      pmask |= (nb - 1) << shift;
-      if (threadIdx.x == 0) {
-        if (d == 0) S->b1 = b;
-        else if (d == 1) S->b2 = b;
-        else S->b3 = b;
-      }
       __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      S->base = base;
+      S->bits = 0;
     }
   }
   __syncthreads();
