@@ -50,7 +50,12 @@ def parse():
     ap.add_argument("--size", type=int, default=512, help="edge of the single cubelet")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--extras", default="e2e", help="comma list: e2e (single-cubelet one-call)")
+    ap.add_argument("--extras", default="e2e,serving",
+                    help="comma list: e2e (single-cubelet one-call), serving")
+    ap.add_argument("--serve-count", type=int, default=300, help="requests of the serving run")
+    ap.add_argument("--serve-slots", type=int, default=8, help="serving device workers")
+    ap.add_argument("--serve-load", type=float, default=0.7,
+                    help="offered rate as a fraction of the measured saturation rate")
     return ap.parse_args()
 
 
@@ -221,7 +226,7 @@ def mixed_stream(args, wv, torch, dev, steps, warmup):
         orig, opt = opt_of[id(c)]
         wv.check(L.wv_compress_device(C.c_void_p(c["vol"].data_ptr()), wv.dtype_id(c["vol"]),
                                       orig, C.byref(opt), C.c_void_p(work.data_ptr()),
-                                      work_bytes, C.byref(info), sp))
+                                      work_bytes, None, C.byref(info), sp))
 
     for c in cubes[:min(P, 20)]:
         plain_one(c)
@@ -263,6 +268,68 @@ def mixed_stream(args, wv, torch, dev, steps, warmup):
     return out
 
 
+def serving(args, wv, torch, dev):
+    """The serving loop on a Poisson request stream at a fixed offered rate.
+    Requests: edges lognormal (median 128, sigma 0.6) clipped to [32, 512],
+    depth lognormal (median 48, sigma 1.0) clipped to [1, 512], dtypes
+    u8 / i16 / f32, Haar at 4,4,4 steps, Lloyd 256 bins (30 iterations),
+    kept fraction uniform on [0.02, 0.2] (thresh = 1 - kept).  Latency is
+    submit -> bins, counts and .cube image on the host."""
+    from wavelets_amd import synth
+    rng = np.random.default_rng(5)
+    N = args.serve_count
+    reqs, in_bytes = [], 0
+    t0 = time.time()
+    for i in range(N):
+        ex = int(np.clip(np.round(np.exp(rng.normal(np.log(128), 0.6))), 32, 512))
+        ey = int(np.clip(np.round(np.exp(rng.normal(np.log(128), 0.6))), 32, 512))
+        ez = int(np.clip(np.round(np.exp(rng.normal(np.log(48), 1.0))), 1, 512))
+        dt = ("uint8", "int16", "float32")[int(rng.integers(3))]
+        keep = float(rng.uniform(0.02, 0.2))
+        vol = synth.cubelet_device((ex, ey, ez), dt, 5000 + i, device=dev).cpu().numpy()
+        in_bytes += vol.nbytes
+        reqs.append((vol, 1.0 - keep))
+    log("generated %d serving requests (%.2f GB) in %.1f s" % (N, in_bytes / 1e9, time.time() - t0))
+
+    def run(rate):
+        srv = wv.Server(slots=args.serve_slots, hbm_budget=200 << 30, encode=True)
+        arrivals = (np.cumsum(rng.exponential(1.0 / rate, N)) if rate else np.zeros(N))
+        done = []
+        start = time.time()
+        for i, (vol, thr) in enumerate(reqs):
+            wait = start + arrivals[i] - time.time()
+            if wait > 0:
+                time.sleep(wait)
+            srv.submit(vol, wv.HAAR, [4, 4, 4], wv.LLOYD, 256, thr, lloyd_iters=30)
+            if i % 16 == 0:
+                done += srv.poll()
+        done += srv.drain()
+        st = srv.stats()
+        srv.close()
+        assert len(done) == N and all(d["status"] == 0 for d in done)
+        lat = np.array([d["t_done"] - d["t_submit"] for d in done]) * 1e3
+        span = max(d["t_done"] for d in done) - min(d["t_submit"] for d in done)
+        return dict(p50_ms=round(float(np.percentile(lat, 50)), 3),
+                    p99_ms=round(float(np.percentile(lat, 99)), 3),
+                    gbps=round(in_bytes / span / 1e9, 3), rps=round(N / span, 2),
+                    peak_inflight=st["peak_inflight"],
+                    peak_device_gb=round(st["peak_bytes"] / 1e9, 3))
+
+    sat = run(None)                    # all requests at once: capacity
+    rate = args.serve_load * sat["rps"]
+    fixed = run(rate)
+    log("serving: saturation %.1f req/s %.2f GB/s; at %.1f req/s p50 %.2f ms p99 %.2f ms" % (
+        sat["rps"], sat["gbps"], rate, fixed["p50_ms"], fixed["p99_ms"]))
+    return {"workload": "%d requests, edges lognormal [32,512], depth [1,512], u8/i16/f32, "
+                        "Haar 4,4,4, Lloyd 256 (30 it), keep U[0.02,0.2]; %d device workers, "
+                        "200 GB HBM budget; latency = submit -> bins+counts+.cube on host"
+                        % (N, args.serve_slots),
+            "input_bytes": int(in_bytes),
+            "saturation": sat,
+            "offered_rps": round(rate, 2),
+            "at_offered_rate": fixed}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -282,6 +349,10 @@ def main():
     batched = None
     if args.workload == "batched":
         batched = mixed_stream(args, wv, torch, dev, args.steps, args.warmup)
+
+    serve = None
+    if "serving" in args.extras and args.workload == "batched":
+        serve = serving(args, wv, torch, dev)
 
     cpu = None
     if not args.no_cpu:
@@ -317,6 +388,8 @@ def main():
     }
     if batched is not None:
         line["batched"] = batched
+    if serve is not None:
+        line["serving"] = serve
     print(json.dumps(line), flush=True)
 
 

@@ -281,10 +281,12 @@ int64_t wv_compress_work_bytes(const int orig[3], const wv_options *opt);
    coder): on return the quantize + histogram may still run on stream; bins
    (int32, padded count) are at offset 0 of work_d and the bin_count + 1
    counts at wv_compress_counts_offset.  Synchronises once, for the
-   threshold result. */
+   threshold result (and, with Lloyd, per iteration).  codebook_out
+   (optional host, 2*bin_count - 1 floats) receives the Lloyd boundaries
+   then values. */
 int wv_compress_device(const void *in_d, int dtype, const int orig[3],
                        const wv_options *opt, void *work_d, int64_t work_bytes,
-                       wv_compress_info *info, void *stream);
+                       float *codebook_out, wv_compress_info *info, void *stream);
 int64_t wv_compress_counts_offset(const int orig[3], const wv_options *opt);
 
 /* ------------------------------------------------------------------ *
@@ -331,6 +333,52 @@ int wv_batch_groups(const wv_batch *b);
 int wv_batch_group_tuning(const wv_batch *b, int g, int *tile_cap, int *ept0, int *bpc,
                           int *ncub);
 int wv_batch_destroy(wv_batch *b);
+
+/* ------------------------------------------------------------------ *
+ * Serving loop.  No counterpart in the reference (one file per run of
+ * test_compress_gpu).  Requests of mixed shapes and dtypes are submitted
+ * as they arrive; device workers (one stream each) admit the head of the
+ * FIFO when a worker is free and the device bytes of admitted requests
+ * plus its footprint fit hbm_budget_bytes, run the device-resident stages
+ * (wv_compress_device), return bins and counts, and release the device
+ * memory; encoder threads write the .cube images.  Every result equals the
+ * single-cubelet path's for that request.
+ * ------------------------------------------------------------------ */
+typedef struct {
+  int slots;                 /* device workers (concurrent requests) */
+  int64_t hbm_budget_bytes;  /* device bytes admitted requests may hold */
+  int encode;                /* 1: host Huffman + .cube image per request */
+  int host_threads;          /* encoder threads (0: = slots) */
+} wv_serve_config;
+
+typedef struct {
+  const void *in_host;       /* input samples, valid until the result is polled */
+  int dtype;
+  int orig[3];
+  wv_options opt;
+  int32_t *bins_out;         /* optional host: padded count bins */
+  uint32_t *counts_out;      /* optional host: bin_count + 1 counts */
+  uint8_t *file_out;         /* optional host .cube image (encode) */
+  int64_t file_cap;
+} wv_serve_request;
+
+typedef struct {
+  int64_t id;
+  int status;                /* WV_OK or a WV_ERR_* */
+  double t_submit, t_admit, t_done;   /* seconds since the server started */
+  int64_t in_bytes, device_bytes, file_bytes;
+  wv_thresh_result thresh;
+  int zero_bin, lloyd_iters;
+} wv_serve_result;
+
+typedef struct wv_server wv_server;
+int wv_serve_create(const wv_serve_config *cfg, wv_server **out);
+int wv_serve_submit(wv_server *s, const wv_serve_request *r, int64_t *id);
+/* moves up to max completed results into out; returns how many */
+int wv_serve_poll(wv_server *s, wv_serve_result *out, int max);
+int wv_serve_drain(wv_server *s);   /* until every submitted request completed */
+int wv_serve_stats(wv_server *s, int64_t *peak_bytes, int *peak_inflight, double *now_s);
+int wv_serve_destroy(wv_server *s);  /* drains, then stops the threads */
 
 #ifdef __cplusplus
 }

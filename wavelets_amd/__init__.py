@@ -87,8 +87,26 @@ class BatchItem(C.Structure):
 
 BATCH_AUTOTUNE, BATCH_NO_GRAPH = 1, 2
 
+
+class ServeConfig(C.Structure):
+    _fields_ = [("slots", C.c_int), ("hbm_budget_bytes", C.c_int64), ("encode", C.c_int),
+                ("host_threads", C.c_int)]
+
 P = C.c_void_p
 I64 = C.c_int64
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [("in_host", C.c_void_p), ("dtype", C.c_int), ("orig", C.c_int * 3),
+                ("opt", Options), ("bins_out", C.c_void_p), ("counts_out", C.c_void_p),
+                ("file_out", C.c_void_p), ("file_cap", C.c_int64)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("id", C.c_int64), ("status", C.c_int), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double), ("in_bytes", C.c_int64),
+                ("device_bytes", C.c_int64), ("file_bytes", C.c_int64),
+                ("thresh", ThreshResult), ("zero_bin", C.c_int), ("lloyd_iters", C.c_int)]
 
 
 def _declare(L):
@@ -127,7 +145,7 @@ def _declare(L):
                                   P, I64, C.POINTER(CompressInfo), P]
         L.wv_compress_work_bytes.restype = I64
         L.wv_compress_work_bytes.argtypes = [P, C.POINTER(Options)]
-        L.wv_compress_device.argtypes = [P, C.c_int, P, C.POINTER(Options), P, I64,
+        L.wv_compress_device.argtypes = [P, C.c_int, P, C.POINTER(Options), P, I64, P,
                                          C.POINTER(CompressInfo), P]
         L.wv_compress_counts_offset.restype = I64
         L.wv_compress_counts_offset.argtypes = [P, C.POINTER(Options)]
@@ -145,6 +163,14 @@ def _declare(L):
                                             C.POINTER(C.c_int), C.POINTER(C.c_int),
                                             C.POINTER(C.c_int)]
         L.wv_batch_destroy.argtypes = [P]
+    if hasattr(L, "wv_serve_create"):
+        L.wv_serve_create.argtypes = [C.POINTER(ServeConfig), C.POINTER(C.c_void_p)]
+        L.wv_serve_submit.argtypes = [P, C.POINTER(ServeRequest), C.POINTER(C.c_int64)]
+        L.wv_serve_poll.argtypes = [P, C.POINTER(ServeResult), C.c_int]
+        L.wv_serve_drain.argtypes = [P]
+        L.wv_serve_stats.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_double)]
+        L.wv_serve_destroy.argtypes = [P]
     if hasattr(L, "wv_lloyd"):
         L.wv_lloyd_ws_bytes.restype = I64
         L.wv_lloyd_ws_bytes.argtypes = [I64, C.c_int]
@@ -433,6 +459,81 @@ class Batch:
     def close(self):
         if self.h:
             lib().wv_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+NP_DTYPE_ID = {"uint8": UINT8, "int16": INT16, "int32": INT32, "float32": FLOAT32}
+
+
+class Server:
+    """The serving loop (wv_serve_*): requests are host arrays [z, y, x];
+    results come back with the bins, counts and .cube image of each."""
+
+    def __init__(self, slots=8, hbm_budget=200 << 30, encode=True, host_threads=0):
+        cfg = ServeConfig(slots, hbm_budget, 1 if encode else 0, host_threads)
+        self.h = C.c_void_p()
+        check(lib().wv_serve_create(C.byref(cfg), C.byref(self.h)))
+        self.live = {}
+        self.encode = encode
+
+    def submit(self, vol, wavelet, steps, alg, bins, frac, compress_zeros=True,
+               lloyd_iters=30, footer=True, want_bins=False):
+        vol = np.ascontiguousarray(vol)
+        orig = [vol.shape[2], vol.shape[1], vol.shape[0]]
+        g = Geom()
+        check(lib().wv_geom_init(C.byref(g), _i3(orig), _i3(steps), wavelet,
+                                 NP_DTYPE_ID[str(vol.dtype)]))
+        out_bins = np.empty(g.count, np.int32) if want_bins else None
+        counts = np.empty(bins + 1, np.uint32)
+        cap = g.count * 5 + (1 << 20)
+        fbuf = np.empty(cap, np.uint8) if self.encode else None
+        r = ServeRequest()
+        r.in_host = vol.ctypes.data
+        r.dtype = NP_DTYPE_ID[str(vol.dtype)]
+        r.orig = _i3(orig)
+        r.opt = Options(wavelet, alg, bins, _i3(steps), C.c_float(frac),
+                        1 if compress_zeros else 0, lloyd_iters, 1 if footer else 0)
+        r.bins_out = out_bins.ctypes.data if out_bins is not None else None
+        r.counts_out = counts.ctypes.data
+        r.file_out = fbuf.ctypes.data if fbuf is not None else None
+        r.file_cap = cap if fbuf is not None else 0
+        rid = C.c_int64()
+        check(lib().wv_serve_submit(self.h, C.byref(r), C.byref(rid)))
+        self.live[rid.value] = (vol, out_bins, counts, fbuf)
+        return rid.value
+
+    def poll(self, max_results=1024):
+        arr = (ServeResult * max_results)()
+        n = check(lib().wv_serve_poll(self.h, arr, max_results))
+        out = []
+        for i in range(n):
+            r = arr[i]
+            vol, b, counts, fbuf = self.live.pop(r.id)
+            out.append(dict(id=r.id, status=r.status, t_submit=r.t_submit, t_admit=r.t_admit,
+                            t_done=r.t_done, in_bytes=r.in_bytes, device_bytes=r.device_bytes,
+                            thresh=r.thresh, zero_bin=r.zero_bin, lloyd_iters=r.lloyd_iters,
+                            bins=b, counts=counts,
+                            file=None if fbuf is None else fbuf[:r.file_bytes].tobytes()))
+        return out
+
+    def drain(self):
+        check(lib().wv_serve_drain(self.h))
+        return self.poll(1 << 16)
+
+    def stats(self):
+        pb, pi, now = C.c_int64(), C.c_int(), C.c_double()
+        check(lib().wv_serve_stats(self.h, C.byref(pb), C.byref(pi), C.byref(now)))
+        return dict(peak_bytes=pb.value, peak_inflight=pi.value, now=now.value)
+
+    def close(self):
+        if self.h:
+            lib().wv_serve_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):

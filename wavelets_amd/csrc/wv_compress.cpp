@@ -131,11 +131,17 @@ extern "C" int64_t wv_compress_counts_offset(const int orig[3], const wv_options
 
 extern "C" int wv_compress_device(const void *in_d, int dtype, const int orig[3],
                                   const wv_options *opt, void *work_d, int64_t work_bytes,
-                                  wv_compress_info *info, void *stream) {
+                                  float *codebook_out, wv_compress_info *info,
+                                  void *stream) {
   if (!info) { set_error("null argument"); return WV_ERR_ARG; }
   std::memset(info, 0, sizeof *info);
   Stages st;
   CHK(device_stages(in_d, dtype, orig, opt, work_d, work_bytes, (hipStream_t)stream, st));
+  if (codebook_out && opt->quant_alg == WV_QUANT_LLOYD) {
+    std::memcpy(codebook_out, st.bounds.data(), st.bounds.size() * sizeof(float));
+    std::memcpy(codebook_out + st.bounds.size(), st.values.data(),
+                st.values.size() * sizeof(float));
+  }
   info->thresh = st.res;
   std::memcpy(info->out_size, st.g.out_size, sizeof st.g.out_size);
   std::memcpy(info->steps, st.g.steps, sizeof st.g.steps);

@@ -37,6 +37,7 @@
 // Arithmetic follows the reference CPU code operation by operation with
 // contraction disabled (-ffp-contract=off), so coefficients are
 // bit-identical to cdf97 / haar (dwt_cpu.cc:139-203,683-799).
+#include <atomic>
 #include <cstdlib>
 #include "wv_device.h"
 #include "wv_internal.h"
@@ -680,27 +681,26 @@ dwt_pass_kernel(DwtPass P, int ntiles) {
 // ---- host side ------------------------------------------------------------
 // resident blocks per CU override (autotuning hook): WV_DWT_BPC
 static int bpc_override() {
-  static int v = -1;
-  if (v < 0) {
+  static const int v = [] {
     const char *e = getenv("WV_DWT_BPC");
-    v = e ? atoi(e) : 0;
-    if (v < 0 || v > 16) v = 0;
-  }
+    const int x = e ? atoi(e) : 0;
+    return (x < 0 || x > 16) ? 0 : x;
+  }();
   return v;
 }
 
 static int cu_count() {
-  static int cus[64] = {0};
+  static std::atomic<int> cus[64];
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
-  if (!cus[dev]) {
-    int n = 0;
+  int n = cus[dev].load(std::memory_order_relaxed);
+  if (!n) {
     if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
         n <= 0)
       n = 256;
-    cus[dev] = n;
+    cus[dev].store(n, std::memory_order_relaxed);
   }
-  return cus[dev];
+  return n;
 }
 
 static size_t tile_lds_bytes(const DwtPass &P) {
@@ -711,15 +711,11 @@ template <int WAVE, bool INV, typename SRC, bool CLAMP, int TT, int EPT>
 static hipError_t launch_one(const DwtPass &P, int ntiles, hipStream_t s) {
   auto k = dwt_pass_kernel<WAVE, INV, SRC, CLAMP, TT, EPT>;
   const size_t lds = tile_lds_bytes(P);
-  static bool attr_set = false;
-  if (!attr_set) {
-    // 160 KB of LDS per workgroup on gfx950, minus the kernel's static LDS
-    hipError_t e = hipFuncSetAttribute((const void *)k,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       152 * 1024);
-    if (e != hipSuccess) return e;
-    attr_set = true;
-  }
+  // 160 KB of LDS per workgroup on gfx950, minus the kernel's static LDS;
+  // set once per instantiation (thread-safe)
+  static const hipError_t attr = hipFuncSetAttribute(
+      (const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 152 * 1024);
+  if (attr != hipSuccess) return attr;
   // persistent grid: as many blocks as stay resident at once
   int per_cu = P.bpc > 0 ? P.bpc : bpc_override();
   int occ = 0;
@@ -783,23 +779,21 @@ static int ilog2(int v) { int r = 0; while ((1 << (r + 1)) <= v) r++; return r; 
 
 // tile width cap (autotuning hook): WV_DWT_TILE in the environment
 static int tile_cap() {
-  static int cap = -1;
-  if (cap < 0) {
+  static const int cap = [] {
     const char *e = getenv("WV_DWT_TILE");
-    cap = e ? atoi(e) : WV_MAXT;
-    if (cap < 1 || cap > WV_MAXT || (cap & (cap - 1))) cap = WV_MAXT;
-  }
+    const int c = e ? atoi(e) : WV_MAXT;
+    return (c < 1 || c > WV_MAXT || (c & (c - 1))) ? WV_MAXT : c;
+  }();
   return cap;
 }
 
 // elements per thread for x-line passes (autotuning hook): WV_DWT_EPT0
 static int ept_axis0() {
-  static int v = -1;
-  if (v < 0) {
+  static const int v = [] {
     const char *e = getenv("WV_DWT_EPT0");
-    v = e ? atoi(e) : 16;
-    if (v != 16 && v != 32) v = 16;
-  }
+    const int x = e ? atoi(e) : 16;
+    return (x != 16 && x != 32) ? 16 : x;
+  }();
   return v;
 }
 

@@ -229,56 +229,68 @@ __global__ void ll_clear_runs(LloydState *S, int32_t *nruns, int32_t *first,
 }
 
 // one wavefront per chain: chains 0..cs-1 are the cells, chain cs the
-// signed distortion (only when DIST).  Sequential float order, lane-wise
-// loads of 64 points then 64 dependent adds in lane order.
+// signed distortion (only when DIST).  The float sums must follow the
+// points' order exactly (lloyds.cpp:56-64,124-132), so each chain is one
+// dependent add per point.  The loop is wave-uniform: every lane walks the
+// same indices, so the loads are scalar (read through the scalar cache, 16
+// points per batch, issued ahead of the adds) and a step costs one
+// dependent VALU add instead of a cross-lane shuffle.
+__device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 template <bool DIST>
 __global__ void __launch_bounds__(64)
-ll_chains(const float *pts, const int32_t *grp, const float *cb,
-          LloydState *S, const int32_t *first, const int32_t *last,
-          const int32_t *nruns, float *sum, uint32_t *count, float *dist_out) {
+ll_chains(const float *__restrict__ pts, const int32_t *__restrict__ grp,
+          const float *__restrict__ cb, LloydState *S, const int32_t *__restrict__ first,
+          const int32_t *__restrict__ last, const int32_t *__restrict__ nruns,
+          float *sum, uint32_t *count, float *dist_out) {
   if (S->done) return;
   const int chain = blockIdx.x;
-  const int cs = S->cs;
-  const int lane = threadIdx.x;
-  const int psize = S->psize;
+  const int cs = uni(S->cs);
+  const int psize = uni(S->psize);
   if (chain < cs) {
-    const int nr = nruns[chain];
+    const int nr = uni(nruns[chain]);
     float s = 0.f;
     uint32_t c = 0;
     if (nr == 1) {
-      const int a = first[chain], b = last[chain] + 1;
+      const int a = uni(first[chain]), b = uni(last[chain] + 1);
       c = (uint32_t)(b - a);
-      for (int base = a; base < b; base += 64) {
-        const int i = base + lane;
-        const float v = i < b ? pts[i] : 0.f;
-        const int m = min(64, b - base);
-        for (int k = 0; k < m; k++) s += __shfl(v, k);
+      int i = a;
+      for (; i + 16 <= b; i += 16) {
+        float v[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) v[k] = pts[i + k];
+#pragma unroll
+        for (int k = 0; k < 16; k++) s += v[k];
       }
+      for (; i < b; i++) s += pts[i];
     } else if (nr > 1) {
-      for (int base = 0; base < psize; base += 64) {
-        const int i = base + lane;
-        const bool mine = i < psize && grp[i] == chain;
-        const float v = mine ? pts[i] : 0.f;
-        uint64_t m = __ballot(mine);
-        c += (uint32_t)__popcll(m);
-        while (m) {
-          const int k = __ffsll((unsigned long long)m) - 1;
-          s += __shfl(v, k);
-          m &= m - 1;
-        }
+      int i = 0;
+      for (; i + 16 <= psize; i += 16) {
+        float v[16];
+        int g[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) { v[k] = pts[i + k]; g[k] = grp[i + k]; }
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+          if (g[k] == chain) { s += v[k]; c++; }
       }
+      for (; i < psize; i++)
+        if (grp[i] == chain) { s += pts[i]; c++; }
     }
-    if (lane == 0) { sum[chain] = s; count[chain] = c; }
+    if (threadIdx.x == 0) { sum[chain] = s; count[chain] = c; }
   } else if (DIST) {
     // dist += codebook[idx] - p over all points in order (lloyds.cpp:64,132)
     float d = 0.f;
-    for (int base = 0; base < psize; base += 64) {
-      const int i = base + lane;
-      const float v = i < psize ? cb[grp[i]] - pts[i] : 0.f;
-      const int m = min(64, psize - base);
-      for (int k = 0; k < m; k++) d += __shfl(v, k);
+    int i = 0;
+    for (; i + 16 <= psize; i += 16) {
+      float t[16];
+#pragma unroll
+      for (int k = 0; k < 16; k++) t[k] = cb[grp[i + k]] - pts[i + k];
+#pragma unroll
+      for (int k = 0; k < 16; k++) d += t[k];
     }
-    if (lane == 0) *dist_out = d;
+    for (; i < psize; i++) d += cb[grp[i]] - pts[i];
+    if (threadIdx.x == 0) *dist_out = d;
   }
 }
 
