@@ -1,0 +1,96 @@
+"""GPU tests of the drop-in driver wv_compress_gpu (test_compress_gpu's
+compress path and flags): the .cube it writes for an input .cube equals the
+oracle's file byte for byte, and its -err round trip (dequantize + inverse
+transform on the GPU) gives the oracle's PSNR within 0.01 dB."""
+import os
+import re
+import struct
+import subprocess
+
+import numpy as np
+import pytest
+
+import _cubeproto as P
+import _oracle as O
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "wavelets_amd", "wv_compress_gpu")
+PROTO_DTYPE = {np.uint8: 1, np.int32: 2, np.float32: 3}
+
+
+def _write_input_cube(path, vol, max_value=0):
+    """An uncompressed cubelet file: the 64-byte header, then one record
+    (cubelet_file.cc:259-300)."""
+    m = P.messages()
+    cb = m["CubeletBuffer"]()
+    cb.width, cb.height, cb.depth = vol.shape[2], vol.shape[1], vol.shape[0]
+    cb.data_type = PROTO_DTYPE[vol.dtype.type]
+    cb.byte_count = vol.nbytes
+    if max_value:
+        cb.maximum_value = max_value
+    meta = cb.SerializeToString()
+    header = b"SCU cubelets 1.0\n".ljust(24, b"\0") + b"\0" * 40
+    with open(path, "wb") as f:
+        f.write(header + struct.pack("<I", len(meta)) + meta + vol.tobytes())
+
+
+def _vol(shape_zyx, dtype, seed):
+    rng = np.random.default_rng(seed)
+    z, y, x = shape_zyx
+    g = np.mgrid[0:z, 0:y, 0:x].astype(np.float32)
+    v = (np.sin(g[2] * 0.23) * 50 + np.cos(g[1] * 0.09) * 40 + g[0] * 0.8 +
+         rng.standard_normal(shape_zyx).astype(np.float32) * 10 + 120)
+    if dtype == np.uint8:
+        return np.clip(v, 0, 255).astype(np.uint8)
+    if dtype == np.int32:
+        return np.clip(v * 12, 0, 4095).astype(np.int32)
+    return v.astype(np.float32)
+
+
+CASES = [
+    ((16, 32, 48), np.uint8, "cdf97", "log", 256, "-1", 0.9, 0),
+    ((24, 24, 24), np.float32, "haar", "uniform", 1024, "3", 0.95, 4096),
+    ((20, 36, 28), np.int32, "cdf97", "log", 256, "3,2", 0.8, 4095),
+    ((16, 16, 32), np.int32, "haar", "lloyd", 64, "4,4,4", 0.9, 4095),
+]
+
+
+@pytest.mark.parametrize("shape,dtype,wave,qalg,bins,steps,frac,maxv", CASES)
+def test_driver_file_matches_oracle(tmp_path, shape, dtype, wave, qalg, bins, steps, frac, maxv):
+    assert os.path.exists(DRIVER), "driver not built"
+    vol = _vol(shape, dtype, 3)
+    src, dst = str(tmp_path / "in.cube"), str(tmp_path / "out.cube")
+    _write_input_cube(src, vol, maxv)
+    args = [DRIVER, "-steps", steps, "-thresh", str(frac), "-qcount", str(bins),
+            "-wave", wave, "-qalg", qalg, "-err"]
+    if qalg == "lloyd":
+        args += ["-lloyd-iters", "30"]
+    r = subprocess.run(args + [src, dst], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = open(dst, "rb").read()
+    st = [int(s) for s in steps.split(",")]
+    st = st * 3 if len(st) == 1 else (st + [0] if len(st) == 2 else st)
+    wv_id = {"haar": O.HAAR, "cdf97": O.CDF97}[wave]
+    q_id = {"uniform": O.UNIFORM, "log": O.LOG, "lloyd": O.LLOYD}[qalg]
+    want = O.compress(vol, wv_id, q_id, bins, st, frac, lloyd_max_iters=30,
+                      max_possible=maxv, want_bins=True)
+    assert got == want["file"]
+    # -err: the oracle's round trip and PSNR formula on the same bins
+    res = want["res"]
+    if q_id == O.LLOYD:
+        cbuf = P.parse_cube_file(want["file"])[1][0][0].compression_parameters
+        q = O.quantizer(q_id, bins, res.threshold, 0.0,
+                        np.array(cbuf.quant_bin_boundaries, np.float32),
+                        np.array(cbuf.quant_bin_values, np.float32))
+    else:
+        q = O.quantizer(q_id, bins, res.threshold, res.max_abs)
+    coef = O.dequantize(q, want["bins"])
+    back, size = O.dwt3d(coef, list(res.size), wv_id, list(res.steps), inverse=True)
+    psnr, mse = O.error_psnr(vol, back, size, maxv)
+    m = re.search(r"PSNR ([0-9.]+) dB", r.stdout)
+    assert m, r.stdout
+    assert abs(float(m.group(1)) - psnr) <= 0.01, (m.group(1), psnr)
