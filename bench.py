@@ -50,6 +50,10 @@ def parse():
     ap.add_argument("--size", type=int, default=512, help="edge of the single cubelet")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run a plain device copy of 4 N bytes (N = samples of the "
+                         "single cubelet) a few times: the reference kernel for scaling "
+                         "FETCH_SIZE / WRITE_SIZE counters")
     ap.add_argument("--extras", default="e2e,serving",
                     help="comma list: e2e (single-cubelet one-call), serving")
     ap.add_argument("--serve-count", type=int, default=300, help="requests of the serving run")
@@ -84,6 +88,19 @@ def cpu_baseline(vol_sample):
                           t, r.t_huff)}
 
 
+def measured_traffic():
+    """HBM bytes per DWT pass launch on the 512^3 cubelet, from the two
+    rocprofv3 --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled by a copy of known
+    size, recorded with their commands in profiles/."""
+    import re
+    path = os.path.join(ROOT, "profiles", "dwt_pass_traffic_512cube_cdf97.md")
+    try:
+        m = re.search(r"traffic_bytes_per_launch = (\d+)", open(path).read())
+        return int(m.group(1)) if m else None
+    except OSError:
+        return None
+
+
 def single_cubelet(args, wv, torch, dev, steps, warmup):
     """The 512^3 single-cubelet device pipeline and the DWT pass roofline."""
     from wavelets_amd import synth
@@ -94,6 +111,12 @@ def single_cubelet(args, wv, torch, dev, steps, warmup):
     log("generated %d^3 float32 in %.1f s" % (n, time.time() - t0))
     vol_d = torch.from_numpy(vol).to(dev)
     N = vol.size
+    if args.calibrate_copy:
+        dst = torch.empty_like(vol_d)
+        for _ in range(3):
+            dst.copy_(vol_d)        # reads 4 N bytes, writes 4 N bytes
+        torch.cuda.synchronize()
+        del dst
     g = wv.geom([n, n, n], [3, 3, 3], wv.CDF97, wv.FLOAT32)
     coef = torch.empty(g.count, dtype=torch.float32, device=dev)
     ws = torch.empty(max(g.ws_floats, L.wv_threshold_ws_bytes(g.count) // 4 + 1),
@@ -147,8 +170,11 @@ def single_cubelet(args, wv, torch, dev, steps, warmup):
         "threshold": float(r.threshold), "max_abs": float(r.max_abs),
         "select_fallback": bool(r.n_candidates < 0),
     }
+    traffic = measured_traffic() if n == 512 else None
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "traffic_source": "profiles/dwt_pass_traffic_512cube_cdf97.md (rocprofv3 FETCH_SIZE "
+                              "+ WRITE_SIZE, scaled by a 512 MiB copy)" if traffic else None,
             "kernel": "dwt_pass_kernel (fused axis pass) on the 512^3 cubelet",
             "pass_ms": [round(float(v), 4) for v in pass_ms],
             "algorithmic_bytes_per_launch": pass_bytes}
