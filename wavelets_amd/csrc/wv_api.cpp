@@ -118,13 +118,10 @@ int wv_geom_init(wv_geom *g, const int orig[3], const int steps[3],
   } else {
     std::memcpy(g->out_size, g->padded, sizeof g->padded);
   }
-  const PassPlan p = forward_plan(g);
-  // the z pass cannot run in place (it changes layout): earlier passes then
-  // work in the workspace
-  const bool z_last = p.axis[p.n - 1] == 2;
-  g->ws_floats = (z_last && p.n >= 2) ? count : 0;
-  // the inverse runs z first (out of place), then in place
-  if (g->ws_floats < count) g->ws_floats = count;
+  // intermediates need one workspace of the padded size (forward: the
+  // passes before the last layout change; inverse: none, but callers size
+  // the workspace the same way)
+  g->ws_floats = count;
   return WV_OK;
 }
 
@@ -150,31 +147,27 @@ int wv_dwt3d_forward_ev(const wv_geom *g, const void *in_d, float *out_d,
                         void *stream) {
   if (!g || !in_d || !out_d) { set_error("null argument"); return WV_ERR_ARG; }
   hipStream_t s = (hipStream_t)stream;
-  const PassPlan plan = forward_plan(g);
-  const bool z_last = plan.axis[plan.n - 1] == 2;
-  if (z_last && plan.n >= 2 && !ws_d) { set_error("workspace required"); return WV_ERR_NOMEM; }
-  float *inter = (z_last && plan.n >= 2) ? ws_d : out_d;
+  PassSpec spec[3];
+  const int n = forward_specs(g->padded, g->steps, spec);
+  int dst_out[3];
+  pass_buffers(spec, n, dst_out);
+  for (int i = 0; i < n; i++)
+    if (!dst_out[i] && !ws_d) { set_error("workspace required"); return WV_ERR_NOMEM; }
   const void *src = in_d;
-  bool clamp = true;
-  for (int i = 0; i < plan.n; i++) {
+  for (int i = 0; i < n; i++) {
     DwtPass P{};
     P.X = g->padded[0]; P.Y = g->padded[1]; P.Z = g->padded[2];
     P.ox = g->orig[0]; P.oy = g->orig[1]; P.oz = g->orig[2];
-    P.levels = plan.levels[i];
-    const bool last = i == plan.n - 1;
     P.src = src;
-    P.dst = last ? out_d : inter;
-    P.store_zxy = (last && plan.axis[i] == 2 && (g->steps[0] || g->steps[1] || g->steps[2])) ? 1 : 0;
-    P.load_zxy = 0;
-    P.stats = nullptr;
-    const int ntiles = plan_pass(P, plan.axis[i]);
-    if (ntiles < 0) { set_error("line too long"); return WV_ERR_UNSUPPORTED; }
+    P.dst = dst_out[i] ? out_d : ws_d;
+    const bool clamp = i == 0;
+    const int ntiles = plan_pass(P, spec[i], clamp);
+    if (ntiles < 0) { set_error("pass %d: unsupported geometry", i); return WV_ERR_UNSUPPORTED; }
     if (events && i == 0) HIPCHK(hipEventRecord((hipEvent_t)events[0], s), "event");
     HIPCHK(launch_pass(P, ntiles, g->wavelet, false, clamp ? g->dtype : WV_DTYPE_FLOAT32,
-                       clamp, P.stats != nullptr, s), "dwt pass");
+                       clamp, false, s), "dwt pass");
     if (events) HIPCHK(hipEventRecord((hipEvent_t)events[i + 1], s), "event");
     src = P.dst;
-    clamp = false;
   }
   if (stats_d) HIPCHK(stats_compute(out_d, g->count, stats_d, s), "stats");
   return WV_OK;
@@ -184,45 +177,36 @@ int wv_dwt3d_inverse(const wv_geom *g, const float *coef_d, float *out_d,
                      float *ws_d, void *stream) {
   if (!g || !coef_d || !out_d) { set_error("null argument"); return WV_ERR_ARG; }
   hipStream_t s = (hipStream_t)stream;
-  const int *S = g->steps;
-  if (!(S[0] || S[1] || S[2])) {
+  PassSpec spec[3];
+  const int n = inverse_specs(g->padded, g->steps, spec);
+  if (n == 0) {
     HIPCHK(hipMemcpyAsync(out_d, coef_d, g->count * 4, hipMemcpyDeviceToDevice, s),
            "copy");
     return WV_OK;
   }
-  // reverse order: z (from z,x,y layout), y, x -- all writing x,y,z layout
-  struct { int axis, levels, load_zxy; } seq[3];
-  int n = 0;
-  if (S[2] || g->padded[2] > 1) seq[n++] = {2, S[2], 1};
-  if (S[1]) seq[n++] = {1, S[1], 0};
-  if (S[0]) seq[n++] = {0, S[0], 0};
+  int dst_out[3];
+  pass_buffers(spec, n, dst_out);
   const float *src = coef_d;
   for (int i = 0; i < n; i++) {
+    if (!dst_out[i] && !ws_d) { set_error("workspace required"); return WV_ERR_NOMEM; }
     DwtPass P{};
     P.X = g->padded[0]; P.Y = g->padded[1]; P.Z = g->padded[2];
     P.ox = P.X; P.oy = P.Y; P.oz = P.Z;
-    P.levels = seq[i].levels;
-    P.load_zxy = seq[i].load_zxy;
-    P.store_zxy = 0;
     P.src = src;
-    P.dst = out_d;   // the z pass reads coef_d, later passes run in place
-    P.stats = nullptr;
-    const int ntiles = plan_pass(P, seq[i].axis);
-    if (ntiles < 0) { set_error("line too long"); return WV_ERR_UNSUPPORTED; }
+    P.dst = dst_out[i] ? out_d : ws_d;
+    const int ntiles = plan_pass(P, spec[i], false);
+    if (ntiles < 0) { set_error("inverse pass %d: unsupported geometry", i); return WV_ERR_UNSUPPORTED; }
     HIPCHK(launch_pass(P, ntiles, g->wavelet, true, WV_DTYPE_FLOAT32, false, false, s),
            "inverse pass");
-    src = out_d;
+    src = P.dst;
   }
-  if (n > 0 && seq[0].axis != 2) {
-    // depth-1 cube whose first inverse pass is in place: the data must be
-    // in out_d before it runs
-    return WV_OK;
-  }
-  (void)ws_d;
   return WV_OK;
 }
 
-int wv_dwt3d_num_passes(const wv_geom *g) { return forward_plan(g).n; }
+int wv_dwt3d_num_passes(const wv_geom *g) {
+  PassSpec spec[3];
+  return forward_specs(g->padded, g->steps, spec);
+}
 
 int wv_event_create(void **ev) {
   hipEvent_t e;

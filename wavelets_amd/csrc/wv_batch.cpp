@@ -54,7 +54,9 @@ struct Group {
   int padded[3], orig[3], steps[3];
   int dtype, wavelet;
   std::vector<int> items;
-  PassPlan plan;
+  PassSpec spec[3];
+  int npass;
+  int dst_out[3];              // per pass: 1 writes the coefficients, 0 the bins buffer
   const void **src_tab_d[3];   // per pass: device pointer tables
   float **dst_tab_d[3];
   Tuning tune;
@@ -112,23 +114,18 @@ namespace wv {
 static int enqueue_group(wv_batch *b, Group &G, hipStream_t s, int ncub_limit) {
   const int ncub = ncub_limit > 0 ? std::min<int>(ncub_limit, (int)G.items.size())
                                   : (int)G.items.size();
-  for (int i = 0; i < G.plan.n; i++) {
+  for (int i = 0; i < G.npass; i++) {
     DwtPass P{};
     P.X = G.padded[0]; P.Y = G.padded[1]; P.Z = G.padded[2];
     P.ox = G.orig[0]; P.oy = G.orig[1]; P.oz = G.orig[2];
-    P.levels = G.plan.levels[i];
-    const bool last = i == G.plan.n - 1;
-    P.store_zxy = (last && G.plan.axis[i] == 2 &&
-                   (G.steps[0] || G.steps[1] || G.steps[2])) ? 1 : 0;
-    P.load_zxy = 0;
     P.src_tab = G.src_tab_d[i];
     P.dst_tab = G.dst_tab_d[i];
     P.ncub = ncub;
     P.tcap = G.tune.tcap;
     P.ept0 = G.tune.ept0;
     P.bpc = G.tune.bpc;
-    const int ntiles = plan_pass(P, G.plan.axis[i]);
-    if (ntiles < 0) { set_error("batch: line too long"); return WV_ERR_UNSUPPORTED; }
+    const int ntiles = plan_pass(P, G.spec[i], i == 0);
+    if (ntiles < 0) { set_error("batch: unsupported pass geometry"); return WV_ERR_UNSUPPORTED; }
     const bool clamp = i == 0;
     hipError_t e = launch_pass(P, ntiles, G.wavelet, false,
                                clamp ? G.dtype : WV_DTYPE_FLOAT32, clamp, false, s);
@@ -358,7 +355,8 @@ int wv_batch_plan(const wv_batch_item *items, int n, int flags, wv_batch **out) 
       std::memcpy(G.steps, g.steps, sizeof G.steps);
       G.dtype = it.dtype;
       G.wavelet = it.wavelet;
-      G.plan = forward_plan(&g);
+      G.npass = forward_specs(g.padded, g.steps, G.spec);
+      pass_buffers(G.spec, G.npass, G.dst_out);
       gidx[k] = (int)b->groups.size();
       b->groups.push_back(G);
       f = gidx.find(k);
@@ -380,7 +378,7 @@ int wv_batch_plan(const wv_batch_item *items, int n, int flags, wv_batch **out) 
   const size_t o_res = off; off = up(off + sizeof(wv_thresh_result) * n);
   const size_t o_tabs = off;
   size_t tab_bytes = 0;
-  for (auto &G : b->groups) tab_bytes += up(sizeof(void *) * G.items.size()) * 2 * G.plan.n;
+  for (auto &G : b->groups) tab_bytes += up(sizeof(void *) * G.items.size()) * 2 * G.npass;
   off = up(off + tab_bytes);
   const size_t state_bytes = (size_t)sel_state_bytes();
   const size_t o_state = off; off = up(off + state_bytes * n);
@@ -455,19 +453,20 @@ int wv_batch_plan(const wv_batch_item *items, int n, int flags, wv_batch **out) 
   BCHK(hipMemcpy(b->qblk_d, qblk.data(), sizeof(int) * n, hipMemcpyHostToDevice), "H2D");
   BCHK(hipMemcpy(b->qdev_d, b->qdev_h, sizeof(QuantDev) * n, hipMemcpyHostToDevice), "H2D");
 
-  // pass pointer tables: pass 0 reads the input; intermediates live in the
-  // cubelet's bins buffer; the last pass writes the coefficients
+  // pass pointer tables: pass 0 reads the input; a pass writes the
+  // coefficients or, as intermediate, the cubelet's bins buffer
   char *tp = ar + o_tabs;
   for (auto &G : b->groups) {
     const int nc = (int)G.items.size();
-    for (int p = 0; p < G.plan.n; p++) {
+    for (int p = 0; p < G.npass; p++) {
       std::vector<const void *> src(nc);
       std::vector<float *> dst(nc);
       for (int c = 0; c < nc; c++) {
         const wv_batch_item &it = b->items[G.items[c]];
         float *inter = (float *)it.bins_d;
-        src[c] = p == 0 ? it.in_d : (const void *)inter;
-        dst[c] = p == G.plan.n - 1 ? it.coef_d : inter;
+        src[c] = p == 0 ? it.in_d
+                        : (const void *)(G.dst_out[p - 1] ? it.coef_d : inter);
+        dst[c] = G.dst_out[p] ? it.coef_d : inter;
       }
       G.src_tab_d[p] = (const void **)tp;
       BCHK(hipMemcpy(tp, src.data(), sizeof(void *) * nc, hipMemcpyHostToDevice), "H2D");

@@ -57,7 +57,7 @@ constexpr float H0 = -.85269867900940f, H1 = .418092273222210f,
                 H4 = -.037828455506995f;
 constexpr float INV_SQRT2 = 0.70710678118654752440f;   // dwt_cpu.cc:11
 
-#define WV_EPT (2 * WV_MAXP)   // most tile elements per thread (EPT): L*T <= WV_DBLOCK*EPT
+#define WV_EPT (2 * WV_MAXP)   // most tile elements per thread (EPT): L*T <= NT*EPT
 #define WV_HALO 4              // spare rows before and after a line
 
 template <typename S> __device__ inline float to_f(S v) { return (float)v; }
@@ -72,9 +72,11 @@ __device__ inline int opaque(int v) {
 __device__ inline int opaque_tid() { return opaque((int)threadIdx.x); }
 
 // ---- tiles ----------------------------------------------------------------
+// A pass's lines form a grid of Nu neighbouring lines (u) by Nv rows (v); a
+// tile is T consecutive u of one row.
 struct Tile {
   int nvalid;      // valid lines in this tile (>= 1)
-  int a, b;        // axis0: first row; axis1: x0, z; axis2: x0, y
+  int a, b;        // first u, row v
   const void *src; // this tile's cubelet
   float *dst;
 };
@@ -83,10 +85,9 @@ struct Tile {
 // so no reciprocal is hoisted out of the tile loop.
 __device__ inline Tile tile_at(const DwtPass &P, int T, int tile) {
   Tile t;
-  int c = 0;
   if (P.src_tab) {
     const int n1 = opaque(P.ntiles1);
-    c = tile / n1;
+    const int c = tile / n1;
     tile -= c * n1;
     t.src = P.src_tab[c];
     t.dst = P.dst_tab[c];
@@ -94,102 +95,40 @@ __device__ inline Tile tile_at(const DwtPass &P, int T, int tile) {
     t.src = P.src;
     t.dst = P.dst;
   }
-  if (P.axis == 0) {
-    t.a = tile * T;
-    t.b = 0;
-    t.nvalid = min(T, P.Y * P.Z - t.a);
-  } else {
-    const int ntx = opaque(P.ntiles_x);
-    const int ob = tile / ntx;
-    t.a = (tile - ob * ntx) * T;
-    t.b = ob;
-    t.nvalid = min(T, P.X - t.a);
-  }
+  const int ntu = opaque(P.ntu);
+  const int v = tile / ntu;
+  t.a = (tile - v * ntu) * T;
+  t.b = v;
+  t.nvalid = min(T, P.Nu - t.a);
   return t;
 }
 
-// ---- line-major addressing -------------------------------------------------
-// x lines, and z lines of the rotated z,x,y layout: a line is contiguous.
-__device__ inline bool line_major_load(const DwtPass &P) {
-  return P.axis == 0 || (P.axis == 2 && P.load_zxy);
-}
-__device__ inline bool line_major_store(const DwtPass &P) {
-  return P.axis == 0 || (P.axis == 2 && P.store_zxy);
-}
+__device__ inline bool line_major_load(const DwtPass &P) { return P.ld_line != 0; }
+__device__ inline bool line_major_store(const DwtPass &P) { return P.st_line != 0; }
 
 // global offset of (line l, position 0); lines past the edge map onto the
-// last valid line
-template <bool CLAMP>
-__device__ inline size_t line_base_load(const DwtPass &P, const Tile &t, int l) {
-  l = min(l, t.nvalid - 1);
-  if (P.axis == 0) {
-    const int r = t.a + l;
-    const int Yo = opaque(P.Y);
-    int y = r % Yo, z = r / Yo;
-    if (CLAMP) {
-      y = min(y, P.oy - 1);
-      z = min(z, P.oz - 1);
-      return ((size_t)z * P.oy + y) * P.ox;
-    }
-    return ((size_t)z * P.Y + y) * P.X;
-  }
-  return ((size_t)t.b * P.X + (t.a + l)) * P.Z;     // z,x,y layout
-}
-__device__ inline size_t line_base_store(const DwtPass &P, const Tile &t, int l) {
-  l = min(l, t.nvalid - 1);
-  if (P.axis == 0) return (size_t)(t.a + l) * P.X;
-  return ((size_t)t.b * P.X + (t.a + l)) * P.Z;
+// last valid line, coordinates past the source onto its last sample
+__device__ inline size_t line_base(const LineMap &m, const Tile &t, int l) {
+  const int u = min(t.a + min(l, t.nvalid - 1), m.ucap);
+  const int v = min(t.b, m.vcap);
+  return (size_t)((int64_t)u * m.su + (int64_t)v * m.sv);
 }
 
-// ---- position-major addressing ---------------------------------------------
-// y lines and z lines of the x,y,z layout: T neighbouring lines are T
-// neighbouring x.  Element (line l, position p) sits at
-// rowbase + min(p, pcap) * pstride + min(a + l, xcap).
+// Position-major access (neighbouring lines adjacent in memory, su == 1):
+// element (line l, position p) at rowbase + min(p, pcap) * pstride +
+// min(a + l, xcap).
 struct PosMap {
   size_t rowbase, pstride;
   int pcap, xcap;
 };
 
-template <bool CLAMP>
-__device__ inline PosMap pos_map_load(const DwtPass &P, const Tile &t) {
-  PosMap m;
-  const int xlast = t.a + t.nvalid - 1;
-  if (CLAMP) {
-    m.xcap = min(xlast, P.ox - 1);
-    if (P.axis == 1) {
-      m.rowbase = (size_t)min(t.b, P.oz - 1) * P.oy * P.ox;
-      m.pstride = P.ox;
-      m.pcap = P.oy - 1;
-    } else {
-      m.rowbase = (size_t)min(t.b, P.oy - 1) * P.ox;
-      m.pstride = (size_t)P.oy * P.ox;
-      m.pcap = P.oz - 1;
-    }
-  } else {
-    m.xcap = xlast;
-    m.pcap = P.L - 1;
-    if (P.axis == 1) {
-      m.rowbase = (size_t)t.b * P.Y * P.X;
-      m.pstride = P.X;
-    } else {
-      m.rowbase = (size_t)t.b * P.X;
-      m.pstride = (size_t)P.Y * P.X;
-    }
-  }
-  return m;
-}
-__device__ inline PosMap pos_map_store(const DwtPass &P, const Tile &t) {
-  PosMap m;
-  m.xcap = t.a + t.nvalid - 1;
-  m.pcap = P.L - 1;
-  if (P.axis == 1) {
-    m.rowbase = (size_t)t.b * P.Y * P.X;
-    m.pstride = P.X;
-  } else {
-    m.rowbase = (size_t)t.b * P.X;
-    m.pstride = (size_t)P.Y * P.X;
-  }
-  return m;
+__device__ inline PosMap pos_map(const LineMap &m, const Tile &t) {
+  PosMap r;
+  r.rowbase = (size_t)((int64_t)min(t.b, m.vcap) * m.sv);
+  r.pstride = (size_t)m.sp;
+  r.pcap = m.pcap;
+  r.xcap = min(t.a + t.nvalid - 1, m.ucap);
+  return r;
 }
 
 // ---- element <-> thread mappings ------------------------------------------
@@ -197,45 +136,47 @@ __device__ inline PosMap pos_map_store(const DwtPass &P, const Tile &t) {
 //   line-major, compile-time TT >= 8 with L % 64 == 0: wave w owns lines
 //     w*TT/8 .. w*TT/8 + TT/8 - 1; element u = li*JN + j of a lane is line
 //     w*TT/8 + li at position j*64 + lane (JN = EPT / (TT/8));
-//   line-major, generic: element u is flat index tid + u*WV_DBLOCK of the
+//   line-major, generic: element u is flat index tid + u*NT of the
 //     line-major tile (division by L through magic_L);
 //   position-major: element u is line tid % T at position
-//     tid / T + u * (WV_DBLOCK / T).
-// Exact tiles (compile-time TT, L*T == WV_DBLOCK*EPT, no padding, all T
+//     tid / T + u * (NT / T).
+// Exact tiles (compile-time TT, L*T == NT*EPT, no padding, all T
 // lines valid) use the same element order with no clamping, and address
 // global memory as a wave-uniform base plus a 32-bit lane offset.
 // compile-time line-major mapping: whole 64-position segments per wave
 // instruction, at most EPT lines per wave
-template <int TT, int EPT> constexpr bool lm_fast() { return TT >= 8 && TT / 8 <= EPT; }
+template <int TT, int EPT, int NT> constexpr bool lm_fast() {
+  return TT >= NT / 64 && TT / (NT / 64) <= EPT;
+}
 
-template <int TT, int EPT>
+template <int TT, int EPT, int NT>
 __device__ inline void lm_index(const DwtPass &P, int tid, int u, int &l, int &p) {
-  if (lm_fast<TT, EPT>()) {
-    constexpr int LPW = lm_fast<TT, EPT>() ? TT / 8 : 1;
+  if (lm_fast<TT, EPT, NT>()) {
+    constexpr int LPW = lm_fast<TT, EPT, NT>() ? TT / (NT / 64) : 1;
     constexpr int JN = EPT / LPW;
     l = (tid >> 6) * LPW + u / JN;
     p = (u % JN) * 64 + (tid & 63);
   } else {
-    const int i = min(tid + u * WV_DBLOCK, P.T * P.L - 1);
+    const int i = min(tid + u * NT, P.T * P.L - 1);
     l = (int)__umulhi((unsigned)i, P.magic_L);
     p = i - l * P.L;
   }
 }
 
-template <int TT, int EPT>
+template <int TT, int EPT, int NT>
 __device__ inline void pm_index(const DwtPass &P, int tid, int u, int &l, int &p) {
   if (TT > 0) {
     l = tid % TT;
-    p = tid / TT + u * (WV_DBLOCK / TT);
+    p = tid / TT + u * (NT / TT);
   } else {
     l = tid & (P.T - 1);
     p = (tid >> P.log2T) + (u << (P.log2dblock_T));
   }
 }
 
-template <int TT, int EPT>
+template <int TT, int EPT, int NT>
 __device__ inline bool exact_tile(const DwtPass &P, const Tile &t) {
-  return lm_fast<TT, EPT>() && P.exact && t.nvalid == TT;
+  return lm_fast<TT, EPT, NT>() && P.exact && t.nvalid == TT;
 }
 
 // 64-bit value that is the same in every lane, moved to scalar registers
@@ -247,14 +188,14 @@ __device__ inline size_t uniform64(size_t v) {
 
 // Issues the loads of this thread's elements of tile t into v[] without
 // waiting for them.  Every path issues exactly EPT loads.
-template <typename SRC, bool CLAMP, int TT, int EPT>
+template <typename SRC, bool CLAMP, int TT, int EPT, int NT>
 __device__ inline void load_regs(const DwtPass &P, const Tile &t, const size_t *lb,
                                  float (&v)[EPT]) {
   const int tid = opaque_tid();
   const SRC *src = (const SRC *)t.src;
-  if (exact_tile<TT, EPT>(P, t)) {
+  if (exact_tile<TT, EPT, NT>(P, t)) {
     if (line_major_load(P)) {
-      constexpr int LPW = lm_fast<TT, EPT>() ? TT / 8 : 1;
+      constexpr int LPW = lm_fast<TT, EPT, NT>() ? TT / (NT / 64) : 1;
       constexpr int JN = EPT / LPW;
       const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
       const int lane = tid & 63;
@@ -265,8 +206,8 @@ __device__ inline void load_regs(const DwtPass &P, const Tile &t, const size_t *
         for (int j = 0; j < JN; j++) v[li * JN + j] = to_f(base[j * 64 + lane]);
       }
     } else {
-      constexpr int PSTEP = WV_DBLOCK / (TT > 0 ? TT : 1);
-      const PosMap m = pos_map_load<CLAMP>(P, t);
+      constexpr int PSTEP = NT / (TT > 0 ? TT : 1);
+      const PosMap m = pos_map(P.ld, t);
       const char *base = (const char *)(src + m.rowbase + t.a);
       constexpr int TW = TT > 0 ? TT : 1;
       const unsigned voff = (unsigned)(((tid / TW) * (unsigned)m.pstride + tid % TW) * sizeof(SRC));
@@ -278,32 +219,32 @@ __device__ inline void load_regs(const DwtPass &P, const Tile &t, const size_t *
     return;
   }
   if (line_major_load(P)) {
-    const int pmax = (CLAMP && P.axis == 0) ? P.ox - 1 : P.L - 1;
+    const int pmax = P.ld.pcap;
 #pragma unroll
     for (int u = 0; u < EPT; u++) {
       int l, p;
-      lm_index<TT, EPT>(P, tid, u, l, p);
+      lm_index<TT, EPT, NT>(P, tid, u, l, p);
       v[u] = to_f(src[lb[l] + min(p, pmax)]);
     }
   } else {
-    const PosMap m = pos_map_load<CLAMP>(P, t);
+    const PosMap m = pos_map(P.ld, t);
 #pragma unroll
     for (int u = 0; u < EPT; u++) {
       int l, p;
-      pm_index<TT, EPT>(P, tid, u, l, p);
+      pm_index<TT, EPT, NT>(P, tid, u, l, p);
       v[u] = to_f(src[m.rowbase + (size_t)min(p, m.pcap) * m.pstride + min(t.a + l, m.xcap)]);
     }
   }
 }
 
-template <int TT, int EPT>
+template <int TT, int EPT, int NT>
 __device__ inline void regs_to_lds(const DwtPass &P, const Tile &t, float *tile,
                                    const float (&v)[EPT]) {
   const int LS = (TT > 0 ? TT : P.T) + 1;
   const int tid = opaque_tid();
-  if (exact_tile<TT, EPT>(P, t)) {
+  if (exact_tile<TT, EPT, NT>(P, t)) {
     if (line_major_load(P)) {
-      constexpr int LPW = lm_fast<TT, EPT>() ? TT / 8 : 1;
+      constexpr int LPW = lm_fast<TT, EPT, NT>() ? TT / (NT / 64) : 1;
       constexpr int JN = EPT / LPW;
       float *b = tile + (tid & 63) * LS + (tid >> 6) * LPW;
 #pragma unroll
@@ -311,7 +252,7 @@ __device__ inline void regs_to_lds(const DwtPass &P, const Tile &t, float *tile,
 #pragma unroll
         for (int j = 0; j < JN; j++) b[j * 64 * LS + li] = v[li * JN + j];
     } else {
-      constexpr int PSTEP = WV_DBLOCK / (TT > 0 ? TT : 1);
+      constexpr int PSTEP = NT / (TT > 0 ? TT : 1);
       constexpr int TW = TT > 0 ? TT : 1;
       float *b = tile + (tid / TW) * LS + tid % TW;
 #pragma unroll
@@ -324,14 +265,14 @@ __device__ inline void regs_to_lds(const DwtPass &P, const Tile &t, float *tile,
 #pragma unroll
     for (int u = 0; u < EPT; u++) {
       int l, p;
-      lm_index<TT, EPT>(P, tid, u, l, p);
+      lm_index<TT, EPT, NT>(P, tid, u, l, p);
       tile[min(p, plast) * LS + l] = v[u];
     }
   } else {
 #pragma unroll
     for (int u = 0; u < EPT; u++) {
       int l, p;
-      pm_index<TT, EPT>(P, tid, u, l, p);
+      pm_index<TT, EPT, NT>(P, tid, u, l, p);
       tile[min(p, plast) * LS + l] = v[u];
     }
   }
@@ -339,14 +280,14 @@ __device__ inline void regs_to_lds(const DwtPass &P, const Tile &t, float *tile,
 
 // Stores tile t from LDS; tout is the row of output position 0 (forward
 // CDF 9/7 leaves its coefficients WV_HALO rows down).
-template <int TT, int EPT>
+template <int TT, int EPT, int NT>
 __device__ inline void store_tile(const DwtPass &P, const Tile &t, const float *tout,
                                   const size_t *lb) {
   const int LS = (TT > 0 ? TT : P.T) + 1;
   const int tid = opaque_tid();
-  if (exact_tile<TT, EPT>(P, t)) {
+  if (exact_tile<TT, EPT, NT>(P, t)) {
     if (line_major_store(P)) {
-      constexpr int LPW = lm_fast<TT, EPT>() ? TT / 8 : 1;
+      constexpr int LPW = lm_fast<TT, EPT, NT>() ? TT / (NT / 64) : 1;
       constexpr int JN = EPT / LPW;
       const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
       const int lane = tid & 63;
@@ -358,8 +299,8 @@ __device__ inline void store_tile(const DwtPass &P, const Tile &t, const float *
         for (int j = 0; j < JN; j++) base[j * 64 + lane] = b[j * 64 * LS + li];
       }
     } else {
-      constexpr int PSTEP = WV_DBLOCK / (TT > 0 ? TT : 1);
-      const PosMap m = pos_map_store(P, t);
+      constexpr int PSTEP = NT / (TT > 0 ? TT : 1);
+      const PosMap m = pos_map(P.st, t);
       char *base = (char *)(t.dst + m.rowbase + t.a);
       constexpr int TW = TT > 0 ? TT : 1;
       const unsigned voff = (unsigned)(((tid / TW) * (unsigned)m.pstride + tid % TW) * 4u);
@@ -376,16 +317,16 @@ __device__ inline void store_tile(const DwtPass &P, const Tile &t, const float *
 #pragma unroll
     for (int u = 0; u < EPT; u++) {
       int l, p;
-      lm_index<TT, EPT>(P, tid, u, l, p);
+      lm_index<TT, EPT, NT>(P, tid, u, l, p);
       p = min(p, plast);
       t.dst[lb[l] + p] = tout[p * LS + l];
     }
   } else {
-    const PosMap m = pos_map_store(P, t);
+    const PosMap m = pos_map(P.st, t);
 #pragma unroll
     for (int u = 0; u < EPT; u++) {
       int l, p;
-      pm_index<TT, EPT>(P, tid, u, l, p);
+      pm_index<TT, EPT, NT>(P, tid, u, l, p);
       p = min(p, plast);
       t.dst[m.rowbase + (size_t)p * m.pstride + min(t.a + l, m.xcap)] = tout[p * LS + l];
     }
@@ -395,14 +336,14 @@ __device__ inline void store_tile(const DwtPass &P, const Tile &t, const float *
 // ---- forward levels ---------------------------------------------------------
 // CDF 9/7 with the gap layout.  Level input: rows [0, n); the window of the
 // pair j spans rows 2j-4 .. 2j+4, i.e. rows -4 .. n+2 over the line.
-template <int TT, int EPT>
+template <int TT, int EPT, int NT>
 __device__ void forward97_levels(float *tile, const DwtPass &P) {
   const int T = TT > 0 ? TT : P.T;
   const int LS = T + 1;
   const int tid = opaque_tid();
   const int line = tid % T;
   const int k = tid / T;
-  const int tpl = WV_DBLOCK / T;
+  const int tpl = NT / T;
   float *col = tile + line;
   int n = P.L;
   for (int s = 0; s < P.levels; s++) {
@@ -469,14 +410,14 @@ __device__ void forward97_levels(float *tile, const DwtPass &P) {
   }
 }
 
-template <int TT, int EPT>
+template <int TT, int EPT, int NT>
 __device__ void forward_haar_levels(float *tile, const DwtPass &P) {
   const int T = TT > 0 ? TT : P.T;
   const int LS = T + 1;
   const int tid = opaque_tid();
   const int line = tid % T;
   const int k = tid / T;
-  const int tpl = WV_DBLOCK / T;
+  const int tpl = NT / T;
   float *col = tile + line;
   int n = P.L;
   for (int s = 0; s < P.levels; s++) {
@@ -521,14 +462,14 @@ __device__ inline float ldi(const float *col, int i, int n, int half, int LS) {
   return (i & 1) ? col[(half + (i >> 1)) * LS] : col[(i >> 1) * LS];
 }
 
-template <int WAVE, int TT, int EPT>
+template <int WAVE, int TT, int EPT, int NT>
 __device__ void inverse_levels(float *tile, const DwtPass &P) {
   const int T = TT > 0 ? TT : P.T;
   const int LS = T + 1;
   const int tid = opaque_tid();
   const int line = tid % T;
   const int k = tid / T;
-  const int tpl = WV_DBLOCK / T;
+  const int tpl = NT / T;
   float *col = tile + line;
   int n = P.L >> (P.levels - 1);
   for (int s = 0; s < P.levels; s++) {
@@ -598,37 +539,37 @@ __device__ void inverse_levels(float *tile, const DwtPass &P) {
 }
 
 // ---- the pass kernel ------------------------------------------------------
-template <int WAVE, bool INV, int TT, int EPT>
+template <int WAVE, bool INV, int TT, int EPT, int NT>
 __device__ inline void run_levels(float *tile, const DwtPass &P) {
   if (P.levels <= 0) return;
-  if (INV) inverse_levels<WAVE, TT, EPT>(tile, P);
-  else if (WAVE == WV_WAVELET_CDF97) forward97_levels<TT, EPT>(tile, P);
-  else forward_haar_levels<TT, EPT>(tile, P);
+  if (INV) inverse_levels<WAVE, TT, EPT, NT>(tile, P);
+  else if (WAVE == WV_WAVELET_CDF97) forward97_levels<TT, EPT, NT>(tile, P);
+  else forward_haar_levels<TT, EPT, NT>(tile, P);
 }
 
-template <bool CLAMP>
+template <bool CLAMP, int NT>
 __device__ inline void set_bases(const DwtPass &P, const Tile &t, size_t *lb_ld,
                                  const Tile *ts, size_t *lb_st) {
   const int T = P.T;
-  for (int l = threadIdx.x; l < T; l += WV_DBLOCK) {
-    if (lb_ld) lb_ld[l] = line_base_load<CLAMP>(P, t, l);
-    if (lb_st) lb_st[l] = line_base_store(P, *ts, l);
+  for (int l = threadIdx.x; l < T; l += NT) {
+    if (lb_ld) lb_ld[l] = line_base(P.ld, t, l);
+    if (lb_st) lb_st[l] = line_base(P.st, *ts, l);
   }
 }
 
 // One steady-state step: tile t goes from registers to LDS, the loads of
 // tile tn are issued, t is transformed and stored.
-template <int WAVE, bool INV, typename SRC, bool CLAMP, int TT, int EPT>
+template <int WAVE, bool INV, typename SRC, bool CLAMP, int TT, int EPT, int NT>
 __device__ __attribute__((always_inline)) inline void
 pipe_step(const DwtPass &P, float *tile, const float *tout, const Tile &t, const Tile &tn,
           size_t *lb_next, size_t *lb_st, float (&v)[EPT]) {
-  regs_to_lds<TT, EPT>(P, t, tile, v);
-  set_bases<CLAMP>(P, tn, line_major_load(P) ? lb_next : nullptr, &t,
+  regs_to_lds<TT, EPT, NT>(P, t, tile, v);
+  set_bases<CLAMP, NT>(P, tn, line_major_load(P) ? lb_next : nullptr, &t,
                    line_major_store(P) ? lb_st : nullptr);
   __syncthreads();
-  load_regs<SRC, CLAMP, TT, EPT>(P, tn, lb_next, v);
-  run_levels<WAVE, INV, TT, EPT>(tile, P);
-  store_tile<TT, EPT>(P, t, tout, lb_st);
+  load_regs<SRC, CLAMP, TT, EPT, NT>(P, tn, lb_next, v);
+  run_levels<WAVE, INV, TT, EPT, NT>(tile, P);
+  store_tile<TT, EPT, NT>(P, t, tout, lb_st);
   __syncthreads();
 }
 
@@ -636,8 +577,8 @@ pipe_step(const DwtPass &P, float *tile, const float *tout, const Tile &t, const
 // with the same memory operations in flight as at its back edge (the
 // previous tile's stores behind the next tile's loads); waits on the loads
 // then never cover those stores.
-template <int WAVE, bool INV, typename SRC, bool CLAMP, int TT, int EPT>
-__global__ void __launch_bounds__(WV_DBLOCK, EPT == 16 ? WV_DWAVES16 : WV_DWAVES)
+template <int WAVE, bool INV, typename SRC, bool CLAMP, int TT, int EPT, int NT>
+__global__ void __launch_bounds__(NT, EPT == 16 ? WV_DWAVES16 : WV_DWAVES)
 dwt_pass_kernel(DwtPass P, int ntiles) {
   constexpr int GAP = (WAVE == WV_WAVELET_CDF97 && !INV) ? WV_HALO : 0;
   extern __shared__ float smem[];
@@ -650,32 +591,32 @@ dwt_pass_kernel(DwtPass P, int ntiles) {
   if (g >= ntiles) return;
   const int G = gridDim.x;
   Tile t = tile_at(P, T, g);
-  set_bases<CLAMP>(P, t, line_major_load(P) ? lb_ld[0] : nullptr, &t, nullptr);
+  set_bases<CLAMP, NT>(P, t, line_major_load(P) ? lb_ld[0] : nullptr, &t, nullptr);
   __syncthreads();
   float v[EPT];
-  load_regs<SRC, CLAMP, TT, EPT>(P, t, lb_ld[0], v);
+  load_regs<SRC, CLAMP, TT, EPT, NT>(P, t, lb_ld[0], v);
   int buf = 0;
   g += G;
   if (g < ntiles) {
     Tile tn = tile_at(P, T, g);
-    pipe_step<WAVE, INV, SRC, CLAMP, TT, EPT>(P, tile, tout, t, tn, lb_ld[1], lb_st, v);
+    pipe_step<WAVE, INV, SRC, CLAMP, TT, EPT, NT>(P, tile, tout, t, tn, lb_ld[1], lb_st, v);
     t = tn;
     buf = 1;
     g += G;
     while (g < ntiles) {
       tn = tile_at(P, T, g);
-      pipe_step<WAVE, INV, SRC, CLAMP, TT, EPT>(P, tile, tout, t, tn, lb_ld[buf ^ 1], lb_st, v);
+      pipe_step<WAVE, INV, SRC, CLAMP, TT, EPT, NT>(P, tile, tout, t, tn, lb_ld[buf ^ 1], lb_st, v);
       t = tn;
       buf ^= 1;
       g += G;
     }
   }
   // last tile: nothing left to prefetch
-  regs_to_lds<TT, EPT>(P, t, tile, v);
-  set_bases<CLAMP>(P, t, nullptr, &t, line_major_store(P) ? lb_st : nullptr);
+  regs_to_lds<TT, EPT, NT>(P, t, tile, v);
+  set_bases<CLAMP, NT>(P, t, nullptr, &t, line_major_store(P) ? lb_st : nullptr);
   __syncthreads();
-  run_levels<WAVE, INV, TT, EPT>(tile, P);
-  store_tile<TT, EPT>(P, t, tout, lb_st);
+  run_levels<WAVE, INV, TT, EPT, NT>(tile, P);
+  store_tile<TT, EPT, NT>(P, t, tout, lb_st);
 }
 
 // ---- host side ------------------------------------------------------------
@@ -707,9 +648,9 @@ static size_t tile_lds_bytes(const DwtPass &P) {
   return (size_t)(P.L + 2 * WV_HALO) * (P.T + 1) * 4;
 }
 
-template <int WAVE, bool INV, typename SRC, bool CLAMP, int TT, int EPT>
+template <int WAVE, bool INV, typename SRC, bool CLAMP, int TT, int EPT, int NT>
 static hipError_t launch_one(const DwtPass &P, int ntiles, hipStream_t s) {
-  auto k = dwt_pass_kernel<WAVE, INV, SRC, CLAMP, TT, EPT>;
+  auto k = dwt_pass_kernel<WAVE, INV, SRC, CLAMP, TT, EPT, NT>;
   const size_t lds = tile_lds_bytes(P);
   // 160 KB of LDS per workgroup on gfx950, minus the kernel's static LDS;
   // set once per instantiation (thread-safe)
@@ -720,18 +661,14 @@ static hipError_t launch_one(const DwtPass &P, int ntiles, hipStream_t s) {
   int per_cu = P.bpc > 0 ? P.bpc : bpc_override();
   int occ = 0;
   {
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, WV_DBLOCK, lds);
+    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, NT, lds);
     if (e != hipSuccess) return e;
   }
   if (per_cu > occ) per_cu = occ;
-  if (!per_cu) {
-    hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, WV_DBLOCK, lds);
-    if (e != hipSuccess) return e;
-    if (per_cu < 1) per_cu = 1;
-  }
+  if (!per_cu) per_cu = occ > 0 ? occ : 1;
   const long cap = (long)per_cu * cu_count();
   const int grid = (int)(ntiles < cap ? ntiles : cap);
-  hipLaunchKernelGGL(k, dim3(grid), dim3(WV_DBLOCK), lds, s, P, ntiles);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(NT), lds, s, P, ntiles);
   return hipGetLastError();
 }
 
@@ -739,27 +676,28 @@ static hipError_t launch_one(const DwtPass &P, int ntiles, hipStream_t s) {
 // segments per wave instruction); anything else runs the generic mapping.
 template <int WAVE, bool INV, typename SRC, bool CLAMP>
 static hipError_t launch_tile(const DwtPass &P, int ntiles, hipStream_t s) {
-  if (P.L % 64 == 0 && P.ept == 16) {
+  if (P.L % 64 == 0 && P.nt == 512 && P.ept == 16) {
     switch (P.T) {
-      case 128: return launch_one<WAVE, INV, SRC, CLAMP, 128, 16>(P, ntiles, s);
-      case 64: return launch_one<WAVE, INV, SRC, CLAMP, 64, 16>(P, ntiles, s);
-      case 32: return launch_one<WAVE, INV, SRC, CLAMP, 32, 16>(P, ntiles, s);
-      case 16: return launch_one<WAVE, INV, SRC, CLAMP, 16, 16>(P, ntiles, s);
+      case 128: return launch_one<WAVE, INV, SRC, CLAMP, 128, 16, 512>(P, ntiles, s);
+      case 64: return launch_one<WAVE, INV, SRC, CLAMP, 64, 16, 512>(P, ntiles, s);
+      case 32: return launch_one<WAVE, INV, SRC, CLAMP, 32, 16, 512>(P, ntiles, s);
+      case 16: return launch_one<WAVE, INV, SRC, CLAMP, 16, 16, 512>(P, ntiles, s);
       default: break;
     }
   }
-  if (P.L % 64 == 0) {
+  if (P.L % 64 == 0 && P.nt == 512 && P.ept == 32) {
     switch (P.T) {
-      case 256: return launch_one<WAVE, INV, SRC, CLAMP, 256, 32>(P, ntiles, s);
-      case 128: return launch_one<WAVE, INV, SRC, CLAMP, 128, 32>(P, ntiles, s);
-      case 64: return launch_one<WAVE, INV, SRC, CLAMP, 64, 32>(P, ntiles, s);
-      case 32: return launch_one<WAVE, INV, SRC, CLAMP, 32, 32>(P, ntiles, s);
-      case 16: return launch_one<WAVE, INV, SRC, CLAMP, 16, 32>(P, ntiles, s);
-      case 8: return launch_one<WAVE, INV, SRC, CLAMP, 8, 32>(P, ntiles, s);
+      case 256: return launch_one<WAVE, INV, SRC, CLAMP, 256, 32, 512>(P, ntiles, s);
+      case 128: return launch_one<WAVE, INV, SRC, CLAMP, 128, 32, 512>(P, ntiles, s);
+      case 64: return launch_one<WAVE, INV, SRC, CLAMP, 64, 32, 512>(P, ntiles, s);
+      case 32: return launch_one<WAVE, INV, SRC, CLAMP, 32, 32, 512>(P, ntiles, s);
+      case 16: return launch_one<WAVE, INV, SRC, CLAMP, 16, 32, 512>(P, ntiles, s);
+      case 8: return launch_one<WAVE, INV, SRC, CLAMP, 8, 32, 512>(P, ntiles, s);
       default: break;
     }
   }
-  return launch_one<WAVE, INV, SRC, CLAMP, 0, 32>(P, ntiles, s);
+  // generic mapping (holds any tile with L*T <= 512*32)
+  return launch_one<WAVE, INV, SRC, CLAMP, 0, 32, 512>(P, ntiles, s);
 }
 
 template <int WAVE, bool INV>
@@ -797,43 +735,121 @@ static int ept_axis0() {
   return v;
 }
 
-// Fills the tile geometry of a pass; returns the number of tiles or -1 if
-// the line is longer than one LDS tile can hold.  Each thread carries EPT
-// tile elements (and owns at most EPT/2 pairs per level); T is the largest
-// power of two with L*T <= WV_DBLOCK*EPT, reduced for y / z passes over
-// narrow volumes so a tile does not run far past x.  x-line passes of
-// 64..512 samples use EPT = 16 (half-size tiles, twice the resident blocks
-// per CU); y / z passes keep EPT = 32, whose T-float segments across lines
+// element strides of the x, y and z axes in a layout
+static void layout_strides(int layout, int64_t X, int64_t Y, int64_t Z, int64_t st[3]) {
+  if (layout == WV_LAYOUT_YXZ) { st[0] = Y; st[1] = 1; st[2] = X * Y; }
+  else if (layout == WV_LAYOUT_ZXY) { st[0] = Z; st[1] = X * Z; st[2] = 1; }
+  else { st[0] = 1; st[1] = X; st[2] = X * Y; }
+}
+
+int forward_specs(const int padded[3], const int steps[3], PassSpec s[3]) {
+  const int *S = steps;
+  const int Z = padded[2];
+  int n = 0;
+  if (!S[0] && !S[1] && !S[2]) {            // no transform, no rotation
+    s[n++] = {0, 0, WV_LAYOUT_XYZ, WV_LAYOUT_XYZ};
+    return n;
+  }
+  if (S[0] && Z >= 32) {
+    if (S[1]) {
+      s[n++] = {0, S[0], WV_LAYOUT_XYZ, WV_LAYOUT_YXZ};
+      s[n++] = {1, S[1], WV_LAYOUT_YXZ, WV_LAYOUT_ZXY};
+    } else {
+      s[n++] = {0, S[0], WV_LAYOUT_XYZ, WV_LAYOUT_ZXY};
+    }
+    if (S[2]) s[n++] = {2, S[2], WV_LAYOUT_ZXY, WV_LAYOUT_ZXY};
+    return n;
+  }
+  if (S[0]) s[n++] = {0, S[0], WV_LAYOUT_XYZ, WV_LAYOUT_XYZ};
+  if (S[1]) s[n++] = {1, S[1], WV_LAYOUT_XYZ, WV_LAYOUT_XYZ};
+  if (S[2] || Z > 1) s[n++] = {2, S[2], WV_LAYOUT_XYZ, WV_LAYOUT_ZXY};
+  return n;
+}
+
+int inverse_specs(const int padded[3], const int steps[3], PassSpec s[3]) {
+  const int *S = steps;
+  int n = 0;
+  if (!S[0] && !S[1] && !S[2]) return 0;
+  if (S[2] || padded[2] > 1) s[n++] = {2, S[2], WV_LAYOUT_ZXY, WV_LAYOUT_XYZ};
+  if (S[1]) s[n++] = {1, S[1], WV_LAYOUT_XYZ, WV_LAYOUT_XYZ};
+  if (S[0]) s[n++] = {0, S[0], WV_LAYOUT_XYZ, WV_LAYOUT_XYZ};
+  return n;
+}
+
+void pass_buffers(const PassSpec *s, int n, int dst_out[3]) {
+  if (n <= 0) return;
+  dst_out[n - 1] = 1;
+  for (int i = n - 2; i >= 0; i--) {
+    const PassSpec &next = s[i + 1];
+    dst_out[i] = next.lin == next.lout ? dst_out[i + 1] : !dst_out[i + 1];
+  }
+}
+
+// Fills the tile geometry of a pass; returns the number of tiles or -1.
+// Lines run along s.axis; u is the axis of neighbouring lines -- the fastest
+// axis of whichever side is not line-contiguous, so its accesses are T-float
+// segments across lines -- and v the remaining axis.  When both sides are
+// line-contiguous and u, v are contiguous in both, they merge into one run
+// of lines.  Each thread carries EPT tile elements (and owns at most EPT/2
+// pairs per level); T is the largest power of two with L*T <= WV_DBLOCK*EPT,
+// reduced when u has fewer lines.  Passes that are line-contiguous on both
+// sides with 64..512 samples use EPT = 16 (half-size tiles, more resident
+// blocks per CU); the others keep EPT = 32, whose segments across lines
 // are twice as long.
-int plan_pass(DwtPass &P, int axis) {
+int plan_pass(DwtPass &P, const PassSpec &s, bool clamp) {
   const int dims[3] = {P.X, P.Y, P.Z};
-  P.axis = axis;
-  P.L = dims[axis];
+  const int odims[3] = {P.ox, P.oy, P.oz};
+  const int a = s.axis;
+  P.L = dims[a];
+  P.levels = s.levels;
   if (P.L > WV_MAX_LINE) return -1;
+  int64_t si[3], so[3];
+  if (clamp) layout_strides(s.lin, P.ox, P.oy, P.oz, si);
+  else layout_strides(s.lin, P.X, P.Y, P.Z, si);
+  layout_strides(s.lout, P.X, P.Y, P.Z, so);
+  P.ld_line = si[a] == 1;
+  P.st_line = so[a] == 1;
+  int b = -1, c = -1;
+  for (int k = 0; k < 3; k++)
+    if (k != a) { if (b < 0) b = k; else c = k; }
+  int u;
+  if (!P.ld_line) u = si[b] == 1 ? b : c;
+  else if (!P.st_line) u = so[b] == 1 ? b : c;
+  else u = si[b] <= si[c] ? b : c;
+  const int v = u == b ? c : b;
+  if (!P.ld_line && so[u] != 1 && !P.st_line) return -1;   // no common fast axis
+  const int *cap = clamp ? odims : dims;
+  P.Nu = dims[u];
+  P.Nv = dims[v];
+  P.ld = {si[u], si[v], si[a], cap[u] - 1, cap[v] - 1, cap[a] - 1};
+  P.st = {so[u], so[v], so[a], dims[u] - 1, dims[v] - 1, dims[a] - 1};
+  const bool unpadded = !clamp || (P.ox == P.X && P.oy == P.Y && P.oz == P.Z);
+  if (P.ld_line && P.st_line && unpadded && P.ld.su * P.Nu == P.ld.sv &&
+      P.st.su * P.Nu == P.st.sv) {
+    P.Nu *= P.Nv;
+    P.Nv = 1;
+    P.ld.ucap = P.st.ucap = P.Nu - 1;
+    P.ld.vcap = P.st.vcap = 0;
+  }
   P.ept = WV_EPT;
-  if (axis == 0 && P.L % 64 == 0 && P.L >= 64 && P.L <= 512)
+  P.nt = WV_DBLOCK;
+  if (P.ld_line && P.st_line && P.L % 64 == 0 && P.L >= 64 && P.L <= 512) {
     P.ept = P.ept0 == 16 || P.ept0 == 32 ? P.ept0 : ept_axis0();
+  }
   int T = P.tcap > 0 ? P.tcap : tile_cap();
   if (T > WV_MAXT) T = WV_MAXT;
   while (T & (T - 1)) T &= T - 1;
-  while (T > 1 && (long)P.L * T > (long)WV_DBLOCK * P.ept) T >>= 1;
-  if (axis != 0) {
-    while (T > 1 && T / 2 >= P.X) T >>= 1;
-  }
+  while (T > 1 && (long)P.L * T > (long)P.nt * P.ept) T >>= 1;
+  while (T > 1 && T / 2 >= P.Nu) T >>= 1;
   P.T = T;
   P.log2T = ilog2(T);
-  P.log2dblock_T = ilog2(WV_DBLOCK / T);
+  P.log2dblock_T = ilog2(P.nt / T);
   P.magic_L = (unsigned)(((1ull << 32) + (unsigned)P.L - 1) / (unsigned)P.L);
-  int n1;
-  if (axis == 0) {
-    const int rows = P.Y * P.Z;
-    n1 = (rows + T - 1) / T;
-  } else {
-    P.ntiles_x = (P.X + T - 1) / T;
-    n1 = P.ntiles_x * (axis == 1 ? P.Z : P.Y);
-  }
-  P.ntiles1 = n1;
-  const long long total = (long long)n1 * (P.src_tab ? (P.ncub > 0 ? P.ncub : 0) : 1);
+  P.ntu = (P.Nu + T - 1) / T;
+  const long long n1 = (long long)P.ntu * P.Nv;
+  if (n1 > 0x7fffffff) return -1;
+  P.ntiles1 = (int)n1;
+  const long long total = n1 * (P.src_tab ? (P.ncub > 0 ? P.ncub : 0) : 1);
   return total > 0x7fffffff ? -1 : (int)total;
 }
 
@@ -842,14 +858,16 @@ hipError_t launch_pass(const DwtPass &P, int ntiles, int wavelet, bool inverse,
   (void)stats;
   if (ntiles <= 0) return hipSuccess;
   DwtPass Q = P;
-  // exact tiles: a tile holds exactly WV_DBLOCK*ept elements, the source
-  // needs no clamping, and lane offsets (< WV_DBLOCK/T rows of the widest
+  // exact tiles: a tile holds exactly nt*ept elements, the source
+  // needs no clamping, and lane offsets (< nt/T rows of the widest
   // stride, plus T) fit 32 bits
   const bool unpadded = !clamp || (P.ox == P.X && P.oy == P.Y && P.oz == P.Z);
-  const long long stride = P.axis == 1 ? P.X : (long long)P.X * P.Y;
-  Q.exact = (long long)P.L * P.T == (long long)WV_DBLOCK * P.ept && P.T >= 8 &&
+  long long stride = 0;     // widest position stride of a position-major side
+  if (!P.ld_line) stride = P.ld.sp;
+  if (!P.st_line && P.st.sp > stride) stride = P.st.sp;
+  Q.exact = (long long)P.L * P.T == (long long)P.nt * P.ept && P.T >= 8 &&
             P.L % 64 == 0 && unpadded &&
-            ((long long)(WV_DBLOCK / P.T) * stride + P.T) * 4 < (1ll << 31);
+            ((long long)(P.nt / P.T) * stride + P.T) * 4 < (1ll << 31);
   if (wavelet == WV_WAVELET_CDF97)
     return inverse ? launch_wave<WV_WAVELET_CDF97, true>(Q, src_dtype, clamp, ntiles, s)
                    : launch_wave<WV_WAVELET_CDF97, false>(Q, src_dtype, clamp, ntiles, s);

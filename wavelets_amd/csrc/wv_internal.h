@@ -10,6 +10,15 @@
 
 namespace wv {
 
+// Where a pass reads or writes element (u, v, p): u*su + v*sv + p*sp, each
+// coordinate first clamped to its cap (REPEAT padding of the unpadded
+// source; lines past the volume edge).  p runs along the pass's axis, u
+// over neighbouring lines of a tile, v over tile rows.
+struct LineMap {
+  int64_t su, sv, sp;
+  int ucap, vcap, pcap;
+};
+
 // One fused axis pass over the whole (padded) volume.
 struct DwtPass {
   const void *src;
@@ -17,16 +26,18 @@ struct DwtPass {
   struct WvStats *stats;
   int X, Y, Z;        // padded volume dims (x, y, z)
   int ox, oy, oz;     // dims of the clamped source (the unpadded input)
-  int axis;           // 0: x lines, 1: y lines, 2: z lines
-  int load_zxy;       // source is in the rotated z,x,y layout (axis 2)
-  int store_zxy;      // destination is in the z,x,y layout (axis 2)
   int levels;
+  int L;              // line length (the pass axis)
+  int Nu, Nv;         // lines: Nu neighbouring lines (u) x Nv rows (v)
+  int ntu;            // tiles per row: ceil(Nu / T)
+  LineMap ld, st;     // load / store addressing
+  int ld_line, st_line;   // 1: lines are contiguous (sp == 1); 0: lines are
+                          // neighbours (su == 1), positions strided
   int T, log2T;       // lines per tile (a power of two <= WV_MAXT)
   int log2dblock_T;   // log2(threads per block / T)
-  int L;              // line length
   unsigned magic_L;   // ceil(2^32 / L) for exact division of i < 2^16
-  int ntiles_x;
   int ept;            // tile elements per thread (16 or 32)
+  int nt;             // threads per block (WV_DBLOCK)
   int exact;          // L*T fills a tile exactly, no padding, 32-bit lane offsets
   // batches: ncub cubelets of this same geometry, cubelet c reading
   // src_tab[c] and writing dst_tab[c] (src / dst unused then); ntiles1 tiles
@@ -35,36 +46,38 @@ struct DwtPass {
   float *const *dst_tab;
   int ncub, ntiles1;
   // launch overrides (0: defaults): tile width cap, elements per thread of
-  // x-line passes, resident blocks per CU
+  // line-contiguous passes, resident blocks per CU
   int tcap, ept0, bpc;
 };
 
-// Pass sequence of the forward transform.  The reference visits x, rotates,
-// y, rotates, z (cdf97_3d dwt_cpu.cc:943-965); here every intermediate
-// stays in x,y,z layout and only the last pass writes the rotated z,x,y
-// layout.  With depth 1 the z,x,y and x,y,z memory orders coincide, so no
-// z pass is needed unless it has levels.
-struct PassPlan {
-  int axis[3];
-  int levels[3];
-  int n;
+// Memory layouts of a cubelet, fastest axis first: x,y,z (the input and
+// intermediates), y,x,z, and the reference's final z,x,y.
+enum { WV_LAYOUT_XYZ = 0, WV_LAYOUT_YXZ = 1, WV_LAYOUT_ZXY = 2 };
+
+// One axis pass: levels along axis, reading layout lin, writing lout.
+struct PassSpec {
+  int axis, levels, lin, lout;
 };
 
-inline PassPlan forward_plan(const wv_geom *g) {
-  PassPlan p{};
-  const int *S = g->steps;
-  const bool any = S[0] || S[1] || S[2];
-  if (!any) {            // no transform, no rotation: conversion only
-    p.axis[0] = 0; p.levels[0] = 0; p.n = 1;
-    return p;
-  }
-  if (S[0]) { p.axis[p.n] = 0; p.levels[p.n++] = S[0]; }
-  if (S[1]) { p.axis[p.n] = 1; p.levels[p.n++] = S[1]; }
-  if (S[2] || g->padded[2] > 1) { p.axis[p.n] = 2; p.levels[p.n++] = S[2]; }
-  return p;
-}
+// The forward pass sequence; returns the number of passes.  The reference
+// visits x, rotates, y, rotates, z (cdf97_3d dwt_cpu.cc:943-965); here only
+// the data layouts change between passes.  With x and y levels and depth
+// >= 32 the x pass writes y,x,z and the y pass z,x,y, so every pass reads
+// whole lines; otherwise x and y stay in x,y,z and the z pass writes z,x,y
+// (with depth 1 the two orders coincide and no z pass runs unless it has
+// levels).  No levels at all: one conversion pass in x,y,z.
+int forward_specs(const int padded[3], const int steps[3], PassSpec s[3]);
+// inverse: z (from z,x,y into x,y,z), then y, then x
+int inverse_specs(const int padded[3], const int steps[3], PassSpec s[3]);
+// Which buffer each pass writes: 1 the output, 0 the workspace (the last
+// pass writes the output; a pass whose input and output layouts are equal
+// runs in place).
+void pass_buffers(const PassSpec *s, int n, int dst_out[3]);
+// Fills the geometry of P (X/Y/Z, ox/oy/oz and the batch fields set by the
+// caller) for spec s; clamp: the pass reads the unpadded source.  Returns
+// the tiles of one launch (all cubelets) or -1.
+int plan_pass(DwtPass &P, const PassSpec &s, bool clamp);
 
-int plan_pass(DwtPass &P, int axis);
 hipError_t launch_pass(const DwtPass &P, int ntiles, int wavelet, bool inverse,
                        int src_dtype, bool clamp, bool stats, hipStream_t s);
 
