@@ -20,8 +20,11 @@
 //            histogram of bits 30..20, then the keys of the rank's bucket)
 //            rebuilds the candidate list -- its kernels return at once when
 //            the window held the rank;
-//   radix    three histogram/scan rounds (bits 30..20, 19..9, 8..0) over
-//            the candidates give the exact key.
+//   radix    histogram/scan rounds over the candidates, relative to the
+//            window: each round splits [base, base + 2^bits) into up to
+//            2048 buckets and keeps the rank's bucket, so the window's
+//            ~20 bits take two rounds and the fallback's bucket at most
+//            two (a third launch returns at once when bits is 0).
 // Everything stays on the device; the result struct is written by the last
 // single-block kernel.
 #include "wv_device.h"
