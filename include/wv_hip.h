@@ -248,6 +248,44 @@ int64_t wv_write_cube(const wv_cube_desc *d, const int32_t *bins_host,
                       uint8_t *file_buf, int64_t cap, int64_t *payload_bytes);
 
 /* ------------------------------------------------------------------ *
+ * Huffman coding on the device.  Replaces the host half of writeQuantData
+ * after the int32 bins D2H (test_compress_gpu.cu:402): initHuffman +
+ * huffScanDupsOnly (test_compress_common.cc:927-973), computeHuffmanCoding
+ * (huffman.cc:42-97) and encodeToStream / dupEncodeString
+ * (huffman.h:241-296) into a BitStreamWriter (bit_stream.h:87-162).
+ * The zero-bin run statistics are computed on the device; the code book
+ * (a tree over bin_count + 1 leaves) is built on the host from them; the
+ * payload words are written on the device, bit for bit what the host
+ * coder writes.  Only the payload needs to cross PCIe.
+ * ------------------------------------------------------------------ */
+typedef struct wv_huff wv_huff;   /* code book + decode table of one payload */
+
+/* device workspace of wv_encode_device */
+int64_t wv_encode_ws_bytes(int64_t n, int bin_count);
+/* payload words that always suffice for n bins (the exact count is known
+   inside wv_encode_device) */
+int64_t wv_encode_words_bound(int64_t n, int bin_count);
+/* bins_d: n bins; counts_d: their histogram (bin_count + 1 entries, as
+   wv_quantize_hist writes it); dup_value: the zero bin, or -1 for no
+   duplicate-string code (-noz).  Writes the payload (LSB-first uint32
+   words) into words_d and its length into *nwords_out; when words_cap is
+   too small nothing is written, *nwords_out holds the size needed and
+   WV_ERR_NOMEM returns.  Synchronises the stream twice (run statistics,
+   code book); the payload kernels are left running on it.  *book_out
+   (optional; free with wv_huff_free) is needed to write the container. */
+int wv_encode_device(const int32_t *bins_d, int64_t n, const uint32_t *counts_d,
+                     int bin_count, int dup_value, void *ws_d, uint32_t *words_d,
+                     int64_t words_cap, int64_t *nwords_out, wv_huff **book_out,
+                     void *stream);
+int64_t wv_huff_payload_bits(const wv_huff *book);
+/* the .cube image around a device-coded payload (host copy of its words):
+   the same bytes wv_write_cube writes for the same bins */
+int64_t wv_write_cube_payload(const wv_cube_desc *d, const wv_huff *book,
+                              const uint32_t *words_host, int64_t nwords,
+                              uint8_t *file_buf, int64_t cap);
+void wv_huff_free(wv_huff *book);
+
+/* ------------------------------------------------------------------ *
  * One-call drop-in: the whole of compressFile (test_compress_gpu.cu:
  * 150-436) on a device-resident input.  Synchronises.  bins_host (may be
  * NULL) receives the bins; file_buf the .cube image.

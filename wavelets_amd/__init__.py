@@ -140,6 +140,19 @@ def _declare(L):
     L.wv_dequantize.argtypes = [P, I64, C.POINTER(QuantParams), P, P]
     L.wv_write_cube.restype = I64
     L.wv_write_cube.argtypes = [C.POINTER(CubeDesc), P, I64, P, P, I64, P]
+    if hasattr(L, "wv_encode_device"):
+        L.wv_encode_ws_bytes.restype = I64
+        L.wv_encode_ws_bytes.argtypes = [I64, C.c_int]
+        L.wv_encode_words_bound.restype = I64
+        L.wv_encode_words_bound.argtypes = [I64, C.c_int]
+        L.wv_encode_device.argtypes = [P, I64, P, C.c_int, C.c_int, P, P, I64,
+                                       C.POINTER(C.c_int64), C.POINTER(C.c_void_p), P]
+        L.wv_huff_payload_bits.restype = I64
+        L.wv_huff_payload_bits.argtypes = [P]
+        L.wv_write_cube_payload.restype = I64
+        L.wv_write_cube_payload.argtypes = [C.POINTER(CubeDesc), P, P, I64, P, I64]
+        L.wv_huff_free.restype = None
+        L.wv_huff_free.argtypes = [P]
     if hasattr(L, "wv_compress"):
         L.wv_compress.argtypes = [P, C.c_int, P, C.POINTER(Options), P, I64, P,
                                   P, I64, C.POINTER(CompressInfo), P]
@@ -370,10 +383,8 @@ def quantize_uniform_dev(coef, bin_count, res_dev, stream=None):
     return bins, counts[:bin_count]
 
 
-def write_cube(desc_kwargs, bins_host, counts_host):
-    """bins_host / counts_host: numpy int32 arrays.  Returns the .cube bytes."""
+def _cube_desc(desc_kwargs, keep):
     d = CubeDesc()
-    keep = []
     for k, v in desc_kwargs.items():
         if k in ("orig", "steps", "out_size"):
             setattr(d, k, _i3(v))
@@ -383,6 +394,13 @@ def write_cube(desc_kwargs, bins_host, counts_host):
             setattr(d, k, arr.ctypes.data_as(P))
         else:
             setattr(d, k, v)
+    return d
+
+
+def write_cube(desc_kwargs, bins_host, counts_host):
+    """bins_host / counts_host: numpy int32 arrays.  Returns the .cube bytes."""
+    keep = []
+    d = _cube_desc(desc_kwargs, keep)
     b = np.ascontiguousarray(bins_host, np.int32)
     c = np.ascontiguousarray(counts_host, np.uint32)
     cap = b.size * 8 + (1 << 20)
@@ -390,6 +408,59 @@ def write_cube(desc_kwargs, bins_host, counts_host):
     pay = C.c_int64()
     n = check(lib().wv_write_cube(C.byref(d), b.ctypes.data_as(P), b.size, c.ctypes.data_as(P),
                                   buf.ctypes.data_as(P), cap, C.byref(pay)))
+    return buf[:n].tobytes()
+
+
+class HuffBook:
+    """Code book of one device-coded payload (wv_huff)."""
+
+    def __init__(self, handle):
+        self.handle = handle
+
+    @property
+    def payload_bits(self):
+        return lib().wv_huff_payload_bits(self.handle)
+
+    def close(self):
+        if self.handle:
+            lib().wv_huff_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def encode_device(bins, counts, bin_count, dup_value, stream=None):
+    """Device Huffman coding of bins (int32 device tensor) with its histogram
+    counts (bin_count + 1 entries, device).  Returns (words, book): words is
+    an int32 device tensor of the LSB-first payload words."""
+    import torch
+    n = bins.numel()
+    dev = bins.device
+    ws = torch.empty(lib().wv_encode_ws_bytes(n, bin_count), dtype=torch.uint8, device=dev)
+    cap = lib().wv_encode_words_bound(n, bin_count)
+    words = torch.empty(max(cap, 1), dtype=torch.int32, device=dev)
+    nw = C.c_int64()
+    h = C.c_void_p()
+    check(lib().wv_encode_device(_ptr(bins), n, _ptr(counts), bin_count, dup_value, _ptr(ws),
+                                 _ptr(words), cap, C.byref(nw), C.byref(h), _stream(stream)))
+    book = HuffBook(h)
+    book._ws = ws     # the payload kernels may still run on the stream
+    return words[:nw.value], book
+
+
+def write_cube_payload(desc_kwargs, book, words_host):
+    """The .cube bytes around a device-coded payload (numpy words)."""
+    keep = []
+    d = _cube_desc(desc_kwargs, keep)
+    w = np.ascontiguousarray(words_host).view(np.uint32)
+    cap = w.size * 4 + (4 << 20)
+    buf = np.empty(cap, np.uint8)
+    n = check(lib().wv_write_cube_payload(C.byref(d), book.handle, w.ctypes.data_as(P), w.size,
+                                          buf.ctypes.data_as(P), cap))
     return buf[:n].tobytes()
 
 

@@ -1,7 +1,8 @@
 // wv_compress.cpp -- the one-call drop-in for compressFile
 // (test_compress_gpu.cu:150-436) on a device-resident cubelet:
-//   forward DWT (statistics fused) -> radix-select threshold -> [Lloyd] ->
-//   quantize + histogram -> bins and counts to the host -> Huffman + .cube.
+//   forward DWT -> radix-select threshold -> [Lloyd] -> quantize +
+//   histogram -> Huffman coding on the device -> payload to the host ->
+//   .cube container.
 #include <hip/hip_runtime.h>
 #include <chrono>
 #include <cstring>
@@ -16,7 +17,7 @@ namespace {
 inline int64_t align256(int64_t v) { return (v + 255) & ~(int64_t)255; }
 
 struct Layout {
-  int64_t coef, ws, stats, result, counts, table, lloyd, total;
+  int64_t coef, ws, stats, result, counts, table, enc, lloyd, total;
 };
 
 Layout layout_for(const wv_geom &g, const wv_options *opt) {
@@ -31,6 +32,7 @@ Layout layout_for(const wv_geom &g, const wv_options *opt) {
   L.result = off; off += align256(sizeof(wv_thresh_result));
   L.counts = off; off += align256(((int64_t)opt->bin_count + 128) * 4);
   L.table = off; off += align256(((int64_t)opt->bin_count * 3 + 16) * 4);
+  L.enc = off; off += align256(wv_encode_ws_bytes(g.count, opt->bin_count));
   L.lloyd = off;
   if (opt->quant_alg == WV_QUANT_LLOYD) off += align256(wv_lloyd_ws_bytes(g.count, opt->bin_count));
   L.total = off;
@@ -123,6 +125,51 @@ int device_stages(const void *in_d, int dtype, const int orig[3], const wv_optio
 
 }  // namespace
 
+namespace wv {
+
+// Device Huffman coding of the bins device_stages left in the work buffer
+// (over the coefficients); the payload words land in the DWT workspace,
+// which holds at least 4 bytes per bin, and are copied to words_host.
+int encode_work(const wv_geom &g, const wv_options *opt, int zero_bin, void *work_d,
+                std::vector<uint32_t> &words_host, wv_huff **book, hipStream_t s) {
+  const Layout L = layout_for(g, opt);
+  char *w = (char *)work_d;
+  const int64_t cap = (L.stats - L.ws) / 4;
+  int64_t nw = 0;
+  CHK(wv_encode_device((const int32_t *)(w + L.coef), g.count, (const uint32_t *)(w + L.counts),
+                       opt->bin_count, opt->compress_zeros ? zero_bin : -1, w + L.enc,
+                       (uint32_t *)(w + L.ws), cap, &nw, book, s));
+  words_host.resize((size_t)nw);
+  if (nw > 0)
+    HCHK(hipMemcpyAsync(words_host.data(), w + L.ws, nw * 4, hipMemcpyDeviceToHost, s),
+         "D2H payload");
+  HCHK(hipStreamSynchronize(s), "sync");
+  return WV_OK;
+}
+
+void cube_desc_of(const wv_geom &g, const wv_options *opt, int dtype, const wv_thresh_result &res,
+                  const std::vector<float> &bounds, const std::vector<float> &values,
+                  int zero_bin, wv_cube_desc &d) {
+  d = wv_cube_desc{};
+  std::memcpy(d.orig, g.orig, sizeof d.orig);
+  std::memcpy(d.steps, g.steps, sizeof d.steps);
+  std::memcpy(d.out_size, g.out_size, sizeof d.out_size);
+  d.wavelet = opt->wavelet;
+  d.quant_alg = opt->quant_alg;
+  d.bin_count = opt->bin_count;
+  // 16-bit data is stored as an INT32 cubelet (shortImagesToCubelets.cc:9-27)
+  d.orig_dtype = dtype == WV_DTYPE_INT16 ? WV_DTYPE_INT32 : dtype;
+  d.thresh_frac = opt->thresh_frac;
+  d.threshold = res.threshold;
+  d.max_val = res.max_abs;
+  d.boundaries = bounds.empty() ? nullptr : bounds.data();
+  d.values = values.empty() ? nullptr : values.data();
+  d.zero_bin = opt->compress_zeros ? zero_bin : -1;
+  d.with_footer = opt->with_footer;
+}
+
+}  // namespace wv
+
 extern "C" int64_t wv_compress_counts_offset(const int orig[3], const wv_options *opt) {
   wv_geom g;
   if (wv_geom_init(&g, orig, opt->steps, opt->wavelet, WV_DTYPE_FLOAT32) != WV_OK) return -1;
@@ -173,13 +220,8 @@ extern "C" int wv_compress(const void *in_d, int dtype, const int orig[3],
   HCHK(hipStreamSynchronize(s), "sync");
   const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 
-  std::vector<uint32_t> counts(opt->bin_count);
-  std::vector<int32_t> own_bins;
-  int32_t *bins = bins_host;
-  if (!bins) { own_bins.resize(g.count); bins = own_bins.data(); }
-  HCHK(hipMemcpyAsync(counts.data(), counts_d, counts.size() * 4, hipMemcpyDeviceToHost, s), "D2H counts");
-  HCHK(hipMemcpyAsync(bins, coef, g.count * 4, hipMemcpyDeviceToHost, s), "D2H bins");
-  HCHK(hipStreamSynchronize(s), "sync");
+  if (bins_host)
+    HCHK(hipMemcpyAsync(bins_host, coef, g.count * 4, hipMemcpyDeviceToHost, s), "D2H bins");
 
   info->thresh = res;
   std::memcpy(info->out_size, g.out_size, sizeof g.out_size);
@@ -189,27 +231,19 @@ extern "C" int wv_compress(const void *in_d, int dtype, const int orig[3],
   info->ms_device = ms;
 
   if (file_buf) {
-    wv_cube_desc d{};
-    std::memcpy(d.orig, g.orig, sizeof d.orig);
-    std::memcpy(d.steps, g.steps, sizeof d.steps);
-    std::memcpy(d.out_size, g.out_size, sizeof d.out_size);
-    d.wavelet = opt->wavelet;
-    d.quant_alg = opt->quant_alg;
-    d.bin_count = opt->bin_count;
-    // 16-bit data is stored as an INT32 cubelet (shortImagesToCubelets.cc:9-27)
-    d.orig_dtype = dtype == WV_DTYPE_INT16 ? WV_DTYPE_INT32 : dtype;
-    d.thresh_frac = opt->thresh_frac;
-    d.threshold = res.threshold;
-    d.max_val = res.max_abs;
-    d.boundaries = bounds.empty() ? nullptr : bounds.data();
-    d.values = values.empty() ? nullptr : values.data();
-    d.zero_bin = opt->compress_zeros ? zero_bin : -1;
-    d.with_footer = opt->with_footer;
-    int64_t pay = 0;
-    const int64_t nb = wv_write_cube(&d, bins, g.count, counts.data(), file_buf, file_cap, &pay);
+    std::vector<uint32_t> words;
+    wv_huff *book = nullptr;
+    CHK(encode_work(g, opt, zero_bin, work_d, words, &book, s));
+    wv_cube_desc d;
+    cube_desc_of(g, opt, dtype, res, bounds, values, zero_bin, d);
+    const int64_t nb = wv_write_cube_payload(&d, book, words.data(), (int64_t)words.size(),
+                                             file_buf, file_cap);
+    wv_huff_free(book);
     if (nb < 0) { set_error("file buffer too small"); return (int)nb; }
     info->file_bytes = nb;
-    info->payload_bytes = pay;
+    info->payload_bytes = (int64_t)words.size() * 4;
+  } else {
+    HCHK(hipStreamSynchronize(s), "sync");
   }
   info->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return WV_OK;

@@ -630,7 +630,7 @@ static int bpc_override() {
   return v;
 }
 
-static int cu_count() {
+int cu_count() {
   static std::atomic<int> cus[64];
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;

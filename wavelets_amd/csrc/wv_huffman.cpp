@@ -15,7 +15,7 @@
 #include <queue>
 #include <string>
 #include <vector>
-#include "../../include/wv_hip.h"
+#include "wv_internal.h"
 
 namespace wvh {
 
@@ -57,6 +57,25 @@ class Coder {
     }
     build();
   }
+
+  // the same code book from finished frequencies: counts with the
+  // duplicate value's entry replaced by its single codes, plus the dup
+  // key's count (the run statistics the device computes)
+  Coder(const uint32_t *counts, int bin_count, int dup_value, int64_t dup_singles,
+        int64_t dup_codes)
+      : nb_(bin_count), dup_(dup_value >= 0 && dup_value < bin_count ? dup_value : -1) {
+    freq_.assign(counts, counts + bin_count);
+    if (dup_ >= 0) {
+      freq_[dup_] = dup_singles;
+      dupcount_ = dup_codes;
+      freq_.push_back(dupcount_);
+    }
+    build();
+  }
+
+  int dup() const { return dup_; }
+  int leaves() const { return (int)freq_.size(); }
+  int64_t freq(int v) const { return freq_[v]; }
 
   int key() const { return nb_; }
   const std::vector<int32_t> &decode_table() const { return table_; }
@@ -276,20 +295,10 @@ static std::string cubelet_msg(const wv_cube_desc *d, const std::string &params,
   return m.b;
 }
 
-}  // namespace wvh
-
-extern "C" int64_t wv_write_cube(const wv_cube_desc *d, const int32_t *bins,
-                                 int64_t n, const uint32_t *counts,
-                                 uint8_t *file_buf, int64_t cap,
-                                 int64_t *payload_bytes) {
-  using namespace wvh;
-  if (!d || !bins || !counts || n <= 0 || d->bin_count < 1) return WV_ERR_ARG;
-  const int dup = d->zero_bin;
-  Coder coder(counts, d->bin_count, dup, bins, n);
-  std::vector<uint32_t> words;
-  words.reserve((size_t)(n / 8 + 16));
-  coder.encode(bins, n, words);
-  const uint32_t nbytes = (uint32_t)(words.size() * 4);
+// The .cube image around a finished payload (nbytes of LSB-first words).
+static int64_t write_container(const wv_cube_desc *d, const Coder &coder, const void *payload,
+                               uint32_t nbytes, uint8_t *file_buf, int64_t cap) {
+  const int dup = coder.dup();
   const int dup_key = dup >= 0 ? d->bin_count : -1;
   const std::string params = params_msg(d, coder, dup_key, dup);
   const std::string meta = cubelet_msg(d, params, nbytes, 0);
@@ -301,7 +310,7 @@ extern "C" int64_t wv_write_cube(const wv_cube_desc *d, const int32_t *bins,
   out.append((const char *)&mlen, 4);
   out += meta;
   const uint64_t data_offset = out.size();
-  out.append((const char *)words.data(), nbytes);
+  out.append((const char *)payload, nbytes);
   if (d->with_footer) {
     const int32_t eof = -1;
     out.append((const char *)&eof, 4);
@@ -312,8 +321,136 @@ extern "C" int64_t wv_write_cube(const wv_cube_desc *d, const int32_t *bins,
     out.append((const char *)&ilen, 4);
     out += "cube";
   }
-  if (payload_bytes) *payload_bytes = nbytes;
   if ((int64_t)out.size() > cap) return WV_ERR_NOMEM;
   std::memcpy(file_buf, out.data(), out.size());
   return (int64_t)out.size();
 }
+
+}  // namespace wvh
+
+struct wv_huff {
+  wvh::Coder coder;
+  int bin_count;
+  int64_t payload_bits;
+  wv_huff(const uint32_t *counts, int b, int dup, int64_t singles, int64_t dups)
+      : coder(counts, b, dup, singles, dups), bin_count(b), payload_bits(0) {}
+};
+
+extern "C" int64_t wv_write_cube(const wv_cube_desc *d, const int32_t *bins,
+                                 int64_t n, const uint32_t *counts,
+                                 uint8_t *file_buf, int64_t cap,
+                                 int64_t *payload_bytes) {
+  using namespace wvh;
+  if (!d || !bins || !counts || n <= 0 || d->bin_count < 1) return WV_ERR_ARG;
+  Coder coder(counts, d->bin_count, d->zero_bin, bins, n);
+  std::vector<uint32_t> words;
+  words.reserve((size_t)(n / 8 + 16));
+  coder.encode(bins, n, words);
+  const uint32_t nbytes = (uint32_t)(words.size() * 4);
+  if (payload_bytes) *payload_bytes = nbytes;
+  return write_container(d, coder, words.data(), nbytes, file_buf, cap);
+}
+
+extern "C" int64_t wv_encode_ws_bytes(int64_t n, int bin_count) {
+  if (n <= 0 || bin_count < 1 || bin_count > WV_MAX_BINS) return -1;
+  return wv::enc_ws_bytes(n, bin_count);
+}
+
+extern "C" int64_t wv_encode_words_bound(int64_t n, int bin_count) {
+  if (n <= 0 || bin_count < 1 || bin_count > WV_MAX_BINS) return -1;
+  // Huffman codes average at most log2(leaves) + 1 bits per coded symbol;
+  // at most n symbols are coded and each dup code (>= 8 bins) adds 3 bits
+  int lg = 0;
+  while ((1ll << lg) < (int64_t)bin_count + 1) lg++;
+  return (n * (lg + 1) + (n / 8) * 3) / 32 + 2;
+}
+
+extern "C" int wv_encode_device(const int32_t *bins_d, int64_t n, const uint32_t *counts_d,
+                                int bin_count, int dup_value, void *ws_d, uint32_t *words_d,
+                                int64_t words_cap, int64_t *nwords_out, wv_huff **book_out,
+                                void *stream) {
+  using namespace wv;
+  if (book_out) *book_out = nullptr;
+  if (!bins_d || !counts_d || !ws_d || !nwords_out || n <= 0 || bin_count < 1 ||
+      bin_count > WV_MAX_BINS) {
+    set_error("wv_encode_device: bad argument");
+    return WV_ERR_ARG;
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const int dup = dup_value >= 0 && dup_value < bin_count ? dup_value : -1;
+  std::vector<uint32_t> counts((size_t)bin_count + 1);
+  hipError_t e = hipMemcpyAsync(counts.data(), counts_d, counts.size() * 4,
+                                hipMemcpyDeviceToHost, s);
+  unsigned long long sd[2] = {0, 0};
+  if (e == hipSuccess)
+    e = dup >= 0 ? enc_count_runs(bins_d, n, dup, bin_count, ws_d, sd, s)
+                 : hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    set_error("wv_encode_device: %s", hipGetErrorString(e));
+    return WV_ERR_HIP;
+  }
+  // bins outside [0, bin_count) have no code of their own: the host coder
+  // gives the value bin_count the dup key's code and has none for others
+  const uint64_t outside = counts[bin_count];
+  if (outside && dup < 0) {
+    set_error("wv_encode_device: %llu bins outside [0, %d) and no dup key",
+              (unsigned long long)outside, bin_count);
+    return WV_ERR_ARG;
+  }
+  wv_huff *book = new wv_huff(counts.data(), bin_count, dup, (int64_t)sd[0], (int64_t)sd[1]);
+  const wvh::Coder &c = book->coder;
+  const int nb1 = c.leaves();
+  if (c.longest() > 56) {
+    delete book;
+    set_error("wv_encode_device: code of %d bits", c.longest());
+    return WV_ERR_ARG;
+  }
+  std::vector<uint64_t> packed((size_t)nb1);
+  uint64_t bits = 0;
+  for (int v = 0; v < nb1; v++) {
+    const wvh::Code &cd = c.code(v);
+    const uint64_t cb = cd.len ? (cd.words[0] | (cd.len > 32 ? (uint64_t)cd.words[1] << 32 : 0)) : 0;
+    packed[v] = cb | ((uint64_t)cd.len << 56);
+    bits += (uint64_t)c.freq(v) * (uint64_t)cd.len;
+  }
+  if (dup >= 0) bits += 3ull * sd[1] + outside * (uint64_t)c.code(bin_count).len;
+  const int64_t nwords = (int64_t)((bits + 31) / 32);
+  *nwords_out = nwords;
+  if (nwords > words_cap || (nwords > 0 && !words_d)) {
+    delete book;
+    set_error("wv_encode_device: payload needs %lld words, buffer holds %lld",
+              (long long)nwords, (long long)words_cap);
+    return WV_ERR_NOMEM;
+  }
+  e = hipMemcpyAsync(enc_book_ptr(ws_d, n, bin_count), packed.data(), packed.size() * 8,
+                     hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);   // packed is a local
+  if (e == hipSuccess && nwords > 0)
+    e = enc_write_payload(bins_d, n, dup, bin_count, nb1, ws_d, words_d, nwords, s);
+  if (e != hipSuccess) {
+    delete book;
+    set_error("wv_encode_device: %s", hipGetErrorString(e));
+    return WV_ERR_HIP;
+  }
+  book->payload_bits = (int64_t)bits;
+  if (book_out) *book_out = book;
+  else delete book;
+  return WV_OK;
+}
+
+extern "C" int64_t wv_huff_payload_bits(const wv_huff *book) {
+  return book ? book->payload_bits : -1;
+}
+
+extern "C" int64_t wv_write_cube_payload(const wv_cube_desc *d, const wv_huff *book,
+                                         const uint32_t *words, int64_t nwords,
+                                         uint8_t *file_buf, int64_t cap) {
+  if (!d || !book || (nwords > 0 && !words) || nwords < 0 || d->bin_count != book->bin_count ||
+      nwords != (book->payload_bits + 31) / 32)
+    return WV_ERR_ARG;
+  const int want_dup = d->zero_bin >= 0 && d->zero_bin < d->bin_count ? d->zero_bin : -1;
+  if (want_dup != book->coder.dup()) return WV_ERR_ARG;
+  return wvh::write_container(d, book->coder, words, (uint32_t)(nwords * 4), file_buf, cap);
+}
+
+extern "C" void wv_huff_free(wv_huff *book) { delete book; }

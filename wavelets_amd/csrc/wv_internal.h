@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <vector>
 #include "../../include/wv_hip.h"
 
 #define WV_MAX_LINE 16384   // longest axis one LDS tile holds
@@ -145,6 +146,28 @@ hipError_t quantize_hist_batch(const BQuantJob *jobs_d, const int *blk0_d, int n
 hipError_t dequantize_table(const int32_t *bins, int64_t n, const float *table,
                             int bin_count, float *out, hipStream_t s);
 
+// device Huffman coding (wv_encode.hip)
+int64_t enc_ws_bytes(int64_t n, int bin_count);
+uint64_t *enc_book_ptr(void *ws, int64_t n, int bin_count);
+// run statistics of the duplicate value zb: sd_host[0] single codes,
+// sd_host[1] dup codes; synchronises the stream
+hipError_t enc_count_runs(const int32_t *bins, int64_t n, int zb, int bin_count, void *ws,
+                          unsigned long long *sd_host, hipStream_t s);
+// payload words from the code book at enc_book_ptr (nb1 packed codes)
+hipError_t enc_write_payload(const int32_t *bins, int64_t n, int zb, int bin_count, int nb1,
+                             void *ws, uint32_t *words, int64_t nwords, hipStream_t s);
+
+// after wv_compress_device on work_d: device Huffman coding of its bins,
+// payload copied to words_host (synchronises); and the container fields
+int encode_work(const wv_geom &g, const wv_options *opt, int zero_bin, void *work_d,
+                std::vector<uint32_t> &words_host, wv_huff **book, hipStream_t s);
+void cube_desc_of(const wv_geom &g, const wv_options *opt, int dtype, const wv_thresh_result &res,
+                  const std::vector<float> &bounds, const std::vector<float> &values,
+                  int zero_bin, wv_cube_desc &d);
+
 void set_error(const char *fmt, ...);
+
+// compute units of the current device (cached per device)
+int cu_count();
 
 }  // namespace wv

@@ -16,10 +16,11 @@
 //              streams.
 //   device     H2D of the input, then wv_compress_device (transform,
 //              threshold, [Lloyd], quantize + histogram; the cubelet stays
-//              in HBM throughout), then only bins and counts come back.
+//              in HBM throughout) and the device Huffman coder; only the
+//              payload (and bins / counts when asked for) comes back.
 //              The device memory returns to the pool right after.
-//   encode     host threads run the Huffman coder and the .cube writer, so
-//              the device worker moves on to the next request at once.
+//   encode     host threads write the .cube container around the payload,
+//              so the device worker moves on to the next request at once.
 //
 // Device memory comes from a caching pool (power-of-two size classes) so
 // steady-state requests do not call hipMalloc / hipFree (hipFree would
@@ -102,9 +103,9 @@ struct Req {
   wv_serve_request r;
   int64_t in_bytes = 0, work_bytes = 0, footprint = 0;
   double t_submit = 0, t_admit = 0;
-  // device results kept for the encoder
-  std::vector<int32_t> bins;
-  std::vector<uint32_t> counts;
+  // device results kept for the container writer
+  std::vector<uint32_t> words;   // Huffman payload
+  wv_huff *book = nullptr;
   std::vector<float> codebook;   // Lloyd: boundaries then values
   wv_compress_info info{};
   int status = WV_OK;
@@ -226,25 +227,22 @@ struct wv_server {
       rc = wv_geom_init(&g, r.orig, r.opt.steps, r.opt.wavelet, r.dtype);
       const int nb = r.opt.bin_count + 1;
       const int64_t coff = wv_compress_counts_offset(r.orig, &r.opt);
-      const bool need_bins = r.bins_out || (cfg.encode && r.file_out);
-      int32_t *bins = r.bins_out;
-      if (rc == WV_OK && need_bins && !bins) {
-        q->bins.resize(g.count);
-        bins = q->bins.data();
-      }
-      uint32_t *counts = r.counts_out;
-      if (!counts) {
-        q->counts.resize(nb);
-        counts = q->counts.data();
-      }
-      if (rc == WV_OK &&
-          hipMemcpyAsync(counts, (char *)work + coff, nb * 4, hipMemcpyDeviceToHost, s) !=
+      if (rc == WV_OK && r.counts_out &&
+          hipMemcpyAsync(r.counts_out, (char *)work + coff, nb * 4, hipMemcpyDeviceToHost, s) !=
               hipSuccess)
         rc = WV_ERR_HIP;
-      if (rc == WV_OK && need_bins &&
-          hipMemcpyAsync(bins, work, g.count * 4, hipMemcpyDeviceToHost, s) != hipSuccess)
+      if (rc == WV_OK && r.bins_out &&
+          hipMemcpyAsync(r.bins_out, work, g.count * 4, hipMemcpyDeviceToHost, s) != hipSuccess)
         rc = WV_ERR_HIP;
-      if (rc == WV_OK && hipStreamSynchronize(s) != hipSuccess) rc = WV_ERR_HIP;
+      // encode_work synchronises the stream after the payload copy
+      if (rc == WV_OK && cfg.encode && r.file_out)
+        rc = encode_work(g, &r.opt, q->info.zero_bin, work, q->words, &q->book, s);
+      else if (rc == WV_OK && hipStreamSynchronize(s) != hipSuccess)
+        rc = WV_ERR_HIP;
+    }
+    if (rc != WV_OK && q->book) {
+      wv_huff_free(q->book);
+      q->book = nullptr;
     }
     q->status = rc;
     pool.put(din, (size_t)q->in_bytes);
@@ -265,8 +263,6 @@ struct wv_server {
       wv_geom g;
       int rc = wv_geom_init(&g, r.orig, r.opt.steps, r.opt.wavelet, r.dtype);
       if (rc == WV_OK) {
-        const int32_t *bins = r.bins_out ? r.bins_out : q->bins.data();
-        const uint32_t *counts = r.counts_out ? r.counts_out : q->counts.data();
         wv_cube_desc d{};
         std::memcpy(d.orig, g.orig, sizeof d.orig);
         std::memcpy(d.steps, g.steps, sizeof d.steps);
@@ -284,13 +280,15 @@ struct wv_server {
           d.boundaries = q->codebook.data();
           d.values = q->codebook.data() + (r.opt.bin_count - 1);
         }
-        {
-          int64_t pay = 0;
-          const int64_t nb = wv_write_cube(&d, bins, g.count, counts, r.file_out, r.file_cap, &pay);
-          if (nb < 0) rc = (int)nb;
-          else q->info.file_bytes = nb;
-        }
+        const int64_t nb = wv_write_cube_payload(&d, q->book, q->words.data(),
+                                                 (int64_t)q->words.size(), r.file_out,
+                                                 r.file_cap);
+        if (nb < 0) rc = (int)nb;
+        else q->info.file_bytes = nb;
       }
+      wv_huff_free(q->book);
+      q->book = nullptr;
+      std::vector<uint32_t>().swap(q->words);
       q->status = rc;
       finish(q);
     }
