@@ -136,6 +136,21 @@ def test_large_code_books(wv, nb):
     _check(wv, x, nb, zb)
 
 
+def test_long_run_codes(wv):
+    """Rare zero bin and rare dup key: their codes are long, the codes of a
+    run of 1023 pass 120 bits, and the writer codes runs code by code."""
+    rng = np.random.default_rng(21)
+    n = 400_000
+    zb = 511
+    x = rng.integers(0, 1024, n).astype(np.int32)
+    x[x == zb] = 3
+    pos = 100
+    for ln in [1023, 9, 1015, 17, 1031, 2047, 8, 7, 4096 + 1023] * 3:
+        x[pos:pos + ln] = zb
+        pos += ln + 5000
+    _check(wv, x, 1024, zb)
+
+
 def test_single_symbol_trees(wv):
     # one leaf: its code has no bits
     _check(wv, np.full(77_777, 5, np.int32), 16, 7)

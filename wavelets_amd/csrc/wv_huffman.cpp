@@ -382,9 +382,9 @@ extern "C" int wv_encode_device(const int32_t *bins_d, int64_t n, const uint32_t
   hipError_t e = hipMemcpyAsync(counts.data(), counts_d, counts.size() * 4,
                                 hipMemcpyDeviceToHost, s);
   unsigned long long sd[2] = {0, 0};
-  if (e == hipSuccess)
-    e = dup >= 0 ? enc_count_runs(bins_d, n, dup, bin_count, ws_d, sd, s)
-                 : hipStreamSynchronize(s);
+  // zero-run statistics (and the per-range histograms the payload
+  // kernels size their output from); synchronises
+  if (e == hipSuccess) e = enc_count_runs(bins_d, n, dup, bin_count, ws_d, sd, s);
   if (e != hipSuccess) {
     set_error("wv_encode_device: %s", hipGetErrorString(e));
     return WV_ERR_HIP;
@@ -425,8 +425,20 @@ extern "C" int wv_encode_device(const int32_t *bins_d, int64_t n, const uint32_t
   e = hipMemcpyAsync(enc_book_ptr(ws_d, n, bin_count), packed.data(), packed.size() * 8,
                      hipMemcpyHostToDevice, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);   // packed is a local
+  // longest code string of a run shorter than 1024 (its dup codes for the
+  // set bits above bit 2, then up to 8 single codes)
+  bool short_runs = true;
+  if (dup >= 0) {
+    const int u = c.code(bin_count).len + 3, zl = c.code(dup).len;
+    for (int r = 9; r < 1024 && short_runs; r++) {
+      int hi = r >> 3, singles = r & 7;
+      if ((hi & 1) && singles == 0) { hi &= ~1; singles = 8; }
+      short_runs = __builtin_popcount(hi) * u + singles * zl <= 120;
+    }
+    short_runs = short_runs && 8 * zl <= 120;
+  }
   if (e == hipSuccess && nwords > 0)
-    e = enc_write_payload(bins_d, n, dup, bin_count, nb1, ws_d, words_d, nwords, s);
+    e = enc_write_payload(bins_d, n, dup, bin_count, nb1, short_runs, ws_d, words_d, nwords, s);
   if (e != hipSuccess) {
     delete book;
     set_error("wv_encode_device: %s", hipGetErrorString(e));

@@ -149,13 +149,16 @@ hipError_t dequantize_table(const int32_t *bins, int64_t n, const float *table,
 // device Huffman coding (wv_encode.hip)
 int64_t enc_ws_bytes(int64_t n, int bin_count);
 uint64_t *enc_book_ptr(void *ws, int64_t n, int bin_count);
-// run statistics of the duplicate value zb: sd_host[0] single codes,
-// sd_host[1] dup codes; synchronises the stream
+// run statistics of the duplicate value zb (-1: none): sd_host[0] single
+// codes, sd_host[1] dup codes, plus per-range tables in ws that
+// enc_write_payload uses; synchronises the stream
 hipError_t enc_count_runs(const int32_t *bins, int64_t n, int zb, int bin_count, void *ws,
                           unsigned long long *sd_host, hipStream_t s);
 // payload words from the code book at enc_book_ptr (nb1 packed codes)
+// short_runs: the codes of every run shorter than 1024 fit 120 bits
 hipError_t enc_write_payload(const int32_t *bins, int64_t n, int zb, int bin_count, int nb1,
-                             void *ws, uint32_t *words, int64_t nwords, hipStream_t s);
+                             bool short_runs, void *ws, uint32_t *words, int64_t nwords,
+                             hipStream_t s);
 
 // after wv_compress_device on work_d: device Huffman coding of its bins,
 // payload copied to words_host (synchronises); and the container fields
