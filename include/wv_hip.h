@@ -283,6 +283,12 @@ int64_t wv_huff_payload_bits(const wv_huff *book);
 int64_t wv_write_cube_payload(const wv_cube_desc *d, const wv_huff *book,
                               const uint32_t *words_host, int64_t nwords,
                               uint8_t *file_buf, int64_t cap);
+/* the same, with the payload still on the device (words_d, as
+   wv_encode_device left it): it is copied straight into file_buf.
+   Synchronises the stream. */
+int64_t wv_write_cube_device(const wv_cube_desc *d, const wv_huff *book,
+                             const uint32_t *words_d, int64_t nwords,
+                             uint8_t *file_buf, int64_t cap, void *stream);
 void wv_huff_free(wv_huff *book);
 
 /* ------------------------------------------------------------------ *

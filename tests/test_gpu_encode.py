@@ -57,6 +57,15 @@ def _check(wv, bins_np, nb, dup, offset=0, container=True):
         meta = _meta(nb, dup, noz=dup < 0)
         got = wv.write_cube_payload(dict(meta), book, words)
         assert got == wv.write_cube(dict(meta), bins_np, counts[:nb])
+        # the container around the payload still on the device
+        keep = []
+        d = wv._cube_desc(dict(meta), keep)
+        wd = torch.from_numpy(words.view(np.int32).copy()).cuda()
+        buf = np.empty(len(got) + 64, np.uint8)
+        nbytes = wv.check(wv.lib().wv_write_cube_device(
+            C.byref(d), book.handle, C.c_void_p(wd.data_ptr()) if words.size else None,
+            words.size, buf.ctypes.data_as(C.c_void_p), buf.size, wv._stream()))
+        assert buf[:nbytes].tobytes() == got
     book.close()
 
 

@@ -151,6 +151,8 @@ def _declare(L):
         L.wv_huff_payload_bits.argtypes = [P]
         L.wv_write_cube_payload.restype = I64
         L.wv_write_cube_payload.argtypes = [C.POINTER(CubeDesc), P, P, I64, P, I64]
+        L.wv_write_cube_device.restype = I64
+        L.wv_write_cube_device.argtypes = [C.POINTER(CubeDesc), P, P, I64, P, I64, P]
         L.wv_huff_free.restype = None
         L.wv_huff_free.argtypes = [P]
     if hasattr(L, "wv_compress"):

@@ -102,9 +102,22 @@ int device_stages(const void *in_d, int dtype, const int orig[3], const wv_optio
   CHK(wv_dwt3d_forward(&g, in_d, coef, ws, nullptr, s));
   // the threshold select needs no precomputed statistics
   CHK(wv_threshold(coef, g.count, opt->thresh_frac, nullptr, ws, res_d, s));
+  const wv_thresh_result &res = st.res;
+  if (opt->quant_alg == WV_QUANT_UNIFORM) {
+    // parameters from the threshold record on the device (the float
+    // expressions of QuantUniform::init): the quantize follows the select
+    // with no host round trip; the record comes back after it
+    CHK(wv_quantize_hist_uniform_dev(coef, g.count, opt->bin_count, res_d, (int32_t *)coef,
+                                     counts_d, s));
+    HCHK(hipMemcpyAsync(&st.res, res_d, sizeof st.res, hipMemcpyDeviceToHost, s), "D2H threshold");
+    HCHK(hipStreamSynchronize(s), "sync");
+    CHK(wv_quant_params_init(&st.qp, WV_QUANT_UNIFORM, opt->bin_count, res.threshold,
+                             res.max_abs, nullptr, nullptr));
+    st.have_qp = true;
+    return WV_OK;
+  }
   HCHK(hipMemcpyAsync(&st.res, res_d, sizeof st.res, hipMemcpyDeviceToHost, s), "D2H threshold");
   HCHK(hipStreamSynchronize(s), "sync");
-  const wv_thresh_result &res = st.res;
   if (opt->quant_alg == WV_QUANT_LLOYD) {
     st.bounds.resize(opt->bin_count - 1);
     st.values.resize(opt->bin_count);
@@ -129,20 +142,29 @@ namespace wv {
 
 // Device Huffman coding of the bins device_stages left in the work buffer
 // (over the coefficients); the payload words land in the DWT workspace,
-// which holds at least 4 bytes per bin, and are copied to words_host.
-int encode_work(const wv_geom &g, const wv_options *opt, int zero_bin, void *work_d,
-                std::vector<uint32_t> &words_host, wv_huff **book, hipStream_t s) {
+// which holds at least 4 bytes per bin (*words_d, *nwords).
+int encode_work_device(const wv_geom &g, const wv_options *opt, int zero_bin, void *work_d,
+                       const uint32_t **words_d, int64_t *nwords, wv_huff **book,
+                       hipStream_t s) {
   const Layout L = layout_for(g, opt);
   char *w = (char *)work_d;
   const int64_t cap = (L.stats - L.ws) / 4;
-  int64_t nw = 0;
   CHK(wv_encode_device((const int32_t *)(w + L.coef), g.count, (const uint32_t *)(w + L.counts),
                        opt->bin_count, opt->compress_zeros ? zero_bin : -1, w + L.enc,
-                       (uint32_t *)(w + L.ws), cap, &nw, book, s));
+                       (uint32_t *)(w + L.ws), cap, nwords, book, s));
+  *words_d = (const uint32_t *)(w + L.ws);
+  return WV_OK;
+}
+
+// ... and the payload copied to words_host
+int encode_work(const wv_geom &g, const wv_options *opt, int zero_bin, void *work_d,
+                std::vector<uint32_t> &words_host, wv_huff **book, hipStream_t s) {
+  const uint32_t *wd = nullptr;
+  int64_t nw = 0;
+  CHK(encode_work_device(g, opt, zero_bin, work_d, &wd, &nw, book, s));
   words_host.resize((size_t)nw);
   if (nw > 0)
-    HCHK(hipMemcpyAsync(words_host.data(), w + L.ws, nw * 4, hipMemcpyDeviceToHost, s),
-         "D2H payload");
+    HCHK(hipMemcpyAsync(words_host.data(), wd, nw * 4, hipMemcpyDeviceToHost, s), "D2H payload");
   HCHK(hipStreamSynchronize(s), "sync");
   return WV_OK;
 }
@@ -231,17 +253,21 @@ extern "C" int wv_compress(const void *in_d, int dtype, const int orig[3],
   info->ms_device = ms;
 
   if (file_buf) {
-    std::vector<uint32_t> words;
+    // the payload goes from the device straight into the file image
+    const uint32_t *words_d = nullptr;
+    int64_t nw = 0;
     wv_huff *book = nullptr;
-    CHK(encode_work(g, opt, zero_bin, work_d, words, &book, s));
+    CHK(encode_work_device(g, opt, zero_bin, work_d, &words_d, &nw, &book, s));
     wv_cube_desc d;
     cube_desc_of(g, opt, dtype, res, bounds, values, zero_bin, d);
-    const int64_t nb = wv_write_cube_payload(&d, book, words.data(), (int64_t)words.size(),
-                                             file_buf, file_cap);
+    const int64_t nb = wv_write_cube_device(&d, book, words_d, nw, file_buf, file_cap, s);
     wv_huff_free(book);
-    if (nb < 0) { set_error("file buffer too small"); return (int)nb; }
+    if (nb < 0) {
+      if (nb == WV_ERR_NOMEM) set_error("file buffer too small");
+      return (int)nb;
+    }
     info->file_bytes = nb;
-    info->payload_bytes = (int64_t)words.size() * 4;
+    info->payload_bytes = nw * 4;
   } else {
     HCHK(hipStreamSynchronize(s), "sync");
   }

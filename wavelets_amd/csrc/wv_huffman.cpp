@@ -295,35 +295,45 @@ static std::string cubelet_msg(const wv_cube_desc *d, const std::string &params,
   return m.b;
 }
 
-// The .cube image around a finished payload (nbytes of LSB-first words).
+// The .cube image around a payload of nbytes (LSB-first words): the file
+// header and the record's metadata (head), then the payload, then the
+// GPU driver's footer (tail).
+struct Container {
+  std::string head, tail;
+  Container(const wv_cube_desc *d, const Coder &coder, uint32_t nbytes) {
+    const int dup = coder.dup();
+    const int dup_key = dup >= 0 ? d->bin_count : -1;
+    const std::string params = params_msg(d, coder, dup_key, dup);
+    const std::string meta = cubelet_msg(d, params, nbytes, 0);
+    char header[64] = {0};
+    std::memcpy(header, "SCU cubelets 1.0\n", 17);
+    head.append(header, 64);
+    const uint32_t mlen = (uint32_t)meta.size();
+    head.append((const char *)&mlen, 4);
+    head += meta;
+    if (d->with_footer) {
+      const int32_t eof = -1;
+      tail.append((const char *)&eof, 4);
+      Proto index;
+      index.bytes(5, cubelet_msg(d, params, nbytes, head.size()));
+      tail += index.b;
+      const int32_t ilen = (int32_t)index.b.size();
+      tail.append((const char *)&ilen, 4);
+      tail += "cube";
+    }
+  }
+  int64_t size(uint32_t nbytes) const { return (int64_t)(head.size() + nbytes + tail.size()); }
+};
+
 static int64_t write_container(const wv_cube_desc *d, const Coder &coder, const void *payload,
                                uint32_t nbytes, uint8_t *file_buf, int64_t cap) {
-  const int dup = coder.dup();
-  const int dup_key = dup >= 0 ? d->bin_count : -1;
-  const std::string params = params_msg(d, coder, dup_key, dup);
-  const std::string meta = cubelet_msg(d, params, nbytes, 0);
-  std::string out;
-  char header[64] = {0};
-  std::memcpy(header, "SCU cubelets 1.0\n", 17);
-  out.append(header, 64);
-  const uint32_t mlen = (uint32_t)meta.size();
-  out.append((const char *)&mlen, 4);
-  out += meta;
-  const uint64_t data_offset = out.size();
-  out.append((const char *)payload, nbytes);
-  if (d->with_footer) {
-    const int32_t eof = -1;
-    out.append((const char *)&eof, 4);
-    Proto index;
-    index.bytes(5, cubelet_msg(d, params, nbytes, data_offset));
-    out += index.b;
-    const int32_t ilen = (int32_t)index.b.size();
-    out.append((const char *)&ilen, 4);
-    out += "cube";
-  }
-  if ((int64_t)out.size() > cap) return WV_ERR_NOMEM;
-  std::memcpy(file_buf, out.data(), out.size());
-  return (int64_t)out.size();
+  const Container ct(d, coder, nbytes);
+  const int64_t total = ct.size(nbytes);
+  if (total > cap) return WV_ERR_NOMEM;
+  std::memcpy(file_buf, ct.head.data(), ct.head.size());
+  if (nbytes) std::memcpy(file_buf + ct.head.size(), payload, nbytes);
+  std::memcpy(file_buf + ct.head.size() + nbytes, ct.tail.data(), ct.tail.size());
+  return total;
 }
 
 }  // namespace wvh
@@ -463,6 +473,33 @@ extern "C" int64_t wv_write_cube_payload(const wv_cube_desc *d, const wv_huff *b
   const int want_dup = d->zero_bin >= 0 && d->zero_bin < d->bin_count ? d->zero_bin : -1;
   if (want_dup != book->coder.dup()) return WV_ERR_ARG;
   return wvh::write_container(d, book->coder, words, (uint32_t)(nwords * 4), file_buf, cap);
+}
+
+extern "C" int64_t wv_write_cube_device(const wv_cube_desc *d, const wv_huff *book,
+                                        const uint32_t *words_d, int64_t nwords,
+                                        uint8_t *file_buf, int64_t cap, void *stream) {
+  if (!d || !book || (nwords > 0 && !words_d) || nwords < 0 || d->bin_count != book->bin_count ||
+      nwords != (book->payload_bits + 31) / 32)
+    return WV_ERR_ARG;
+  const int want_dup = d->zero_bin >= 0 && d->zero_bin < d->bin_count ? d->zero_bin : -1;
+  if (want_dup != book->coder.dup()) return WV_ERR_ARG;
+  const uint32_t nbytes = (uint32_t)(nwords * 4);
+  const wvh::Container ct(d, book->coder, nbytes);
+  const int64_t total = ct.size(nbytes);
+  if (total > cap) return WV_ERR_NOMEM;
+  std::memcpy(file_buf, ct.head.data(), ct.head.size());
+  hipStream_t s = (hipStream_t)stream;
+  if (nbytes) {
+    hipError_t e = hipMemcpyAsync(file_buf + ct.head.size(), words_d, nbytes,
+                                  hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      wv::set_error("wv_write_cube_device: %s", hipGetErrorString(e));
+      return WV_ERR_HIP;
+    }
+  }
+  std::memcpy(file_buf + ct.head.size() + nbytes, ct.tail.data(), ct.tail.size());
+  return total;
 }
 
 extern "C" void wv_huff_free(wv_huff *book) { delete book; }
