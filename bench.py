@@ -140,9 +140,14 @@ def single_cubelet(args, wv, torch, dev, steps, warmup):
             C.c_void_p(coef.data_ptr()), g.count, 1024, C.c_void_p(res_d.data_ptr()),
             C.c_void_p(bins.data_ptr()), C.c_void_p(counts.data_ptr()), sp))
 
-    for _ in range(warmup):
+    # W warmup steps, continued until 2 s of work have run, so that the timed
+    # steps see settled clocks (the roofline figure comes from them)
+    t_w = time.time()
+    k = 0
+    while k < warmup or time.time() - t_w < 2.0:
         step([wv.Event() for _ in range(npass + 1)])
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        k += 1
     evsets = [[wv.Event() for _ in range(npass + 1)] for _ in range(steps)]
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
@@ -393,7 +398,7 @@ def main():
     torch.cuda.set_device(dev)
 
     single, roof, vol = single_cubelet(args, wv, torch, dev,
-                                       args.steps if args.workload == "single" else 10,
+                                       args.steps if args.workload == "single" else 20,
                                        args.warmup)
     batched = None
     if args.workload == "batched":

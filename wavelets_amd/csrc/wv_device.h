@@ -25,6 +25,17 @@
 #define WV_H2_SHIFT 9
 #define WV_H3_BINS 512      // digit 3: bits 8..0
 
+// The DWT pass marks its full-tile global accesses non-temporal (each
+// sample is read and written once per pass).  WV_NT_LD / WV_NT_ST = 0 turn
+// the hints off.  The quantize pass measured slower with them, the select
+// unchanged, so those kernels use plain accesses.
+#ifndef WV_NT_LD
+#define WV_NT_LD 1
+#endif
+#ifndef WV_NT_ST
+#define WV_NT_ST 1
+#endif
+
 // Fused statistics block (device).  Layout is part of the ABI size
 // WV_STATS_BYTES.
 struct WvStats {
