@@ -292,6 +292,58 @@ int64_t wv_write_cube_device(const wv_cube_desc *d, const wv_huff *book,
 void wv_huff_free(wv_huff *book);
 
 /* ------------------------------------------------------------------ *
+ * Decompression.  Replaces decompressFile (test_compress_cpu.cc:187-236;
+ * the reference GPU decompress is a stub, test_compress_gpu.cu:438-501):
+ * readQuantData (test_compress_common.cc:850-894) -> dequantize (:567-584)
+ * -> inverse waveletTransform -> translateCubeDataToOriginal (:699-747)
+ * -> an uncompressed cubelet file (cubelet_file.cc:259-327).
+ * ------------------------------------------------------------------ */
+typedef struct {
+  int size[3];             /* stored (rotated, padded) width, height, depth */
+  int orig[3];             /* original width, height, depth */
+  int orig_dtype;          /* WV_DTYPE_UINT8 / INT32 / FLOAT32; 0: not recorded */
+  int steps[3];
+  int wavelet;             /* WV_WAVELET_* */
+  int quant_alg;           /* WV_QUANT_*; -1: unknown */
+  int bin_count;
+  float thresh_frac, threshold, max_val;
+  int dup_key, dup_value;  /* -1 when absent */
+  int maximum_value;       /* the cubelet's maximum_value, 0 if unset */
+  int n_decode, n_boundaries, n_values;   /* table / codebook lengths */
+  int64_t payload_offset, payload_bytes;  /* Huffman payload in the file */
+} wv_cube_info;
+
+/* Finds the first wavelet-compressed INT32 cubelet of a .cube file image
+   (readQuantData's rule) and fills *info; decode_table / boundaries /
+   values (may be NULL) receive the arrays when their capacities suffice. */
+int wv_cube_parse(const uint8_t *file, int64_t len, wv_cube_info *info,
+                  int32_t *decode_table, int decode_cap, float *boundaries,
+                  float *values, int codebook_cap);
+/* HuffmanDecoder::decodeFromStream (huffman.h:96-131) on the host: decodes
+   up to count values; returns the number decoded (count on success) or a
+   negative status for a malformed table. */
+int64_t wv_huff_decode_host(const int32_t *table, int n_table, int dup_key,
+                            int dup_value, const uint32_t *words, int64_t nwords,
+                            int32_t *out, int64_t count);
+/* the device half: dequantize the bins (stored layout), invert the
+   transform and restore the original type over the original size into
+   out_d (x fastest; FLOAT32 when the original type was not recorded).
+   boundaries / values: the Lloyd codebook (host), else NULL. */
+int64_t wv_decompress_work_bytes(const wv_cube_info *info);
+int wv_decompress_device(const wv_cube_info *info, const int32_t *bins_d,
+                         const float *boundaries, const float *values,
+                         void *work_d, int64_t work_bytes, void *out_d, void *stream);
+/* one call on a file image: parse, host decode, device half, D2H into
+   out_host.  Returns the output bytes; *ms_decode (optional) receives the
+   host decode time.  Synchronises. */
+int64_t wv_decompress(const uint8_t *file, int64_t len, void *work_d,
+                      int64_t work_bytes, uint8_t *out_host, int64_t out_cap,
+                      wv_cube_info *info_out, double *ms_decode, void *stream);
+/* an uncompressed cubelet file (addCubelet + close with the footer) */
+int64_t wv_write_cube_raw(const int size[3], int dtype, const void *data,
+                          uint8_t *file_buf, int64_t cap);
+
+/* ------------------------------------------------------------------ *
  * One-call drop-in: the whole of compressFile (test_compress_gpu.cu:
  * 150-436) on a device-resident input.  Synchronises.  bins_host (may be
  * NULL) receives the bins; file_buf the .cube image.

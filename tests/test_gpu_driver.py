@@ -94,3 +94,25 @@ def test_driver_file_matches_oracle(tmp_path, shape, dtype, wave, qalg, bins, st
     m = re.search(r"PSNR ([0-9.]+) dB", r.stdout)
     assert m, r.stdout
     assert abs(float(m.group(1)) - psnr) <= 0.01, (m.group(1), psnr)
+
+
+@pytest.mark.parametrize("shape,dtype,wave,qalg,bins,steps,frac,maxv", CASES[:3])
+def test_driver_decompress_matches_oracle(tmp_path, shape, dtype, wave, qalg, bins, steps, frac,
+                                          maxv):
+    """-d on the oracle's compressed file: an uncompressed cubelet holding the
+    oracle's reconstruction in the original type (decompressFile,
+    test_compress_cpu.cc:187-236)."""
+    import wavelets_amd as wv
+    from test_gpu_decompress import oracle_decompress
+    vol = _vol(shape, dtype, 7)
+    st = [int(s) for s in steps.split(",")]
+    st = st * 3 if len(st) == 1 else (st + [0] if len(st) == 2 else st)
+    wv_id = {"haar": O.HAAR, "cdf97": O.CDF97}[wave]
+    q_id = {"uniform": O.UNIFORM, "log": O.LOG, "lloyd": O.LLOYD}[qalg]
+    comp = O.compress(vol, wv_id, q_id, bins, st, frac, lloyd_max_iters=30)["file"]
+    src, dst = str(tmp_path / "c.cube"), str(tmp_path / "d.cube")
+    open(src, "wb").write(comp)
+    r = subprocess.run([DRIVER, "-d", src, dst], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    want = wv.write_cube_raw(oracle_decompress(comp))
+    assert open(dst, "rb").read() == want

@@ -109,6 +109,16 @@ class ServeResult(C.Structure):
                 ("thresh", ThreshResult), ("zero_bin", C.c_int), ("lloyd_iters", C.c_int)]
 
 
+class CubeInfo(C.Structure):
+    _fields_ = [("size", C.c_int * 3), ("orig", C.c_int * 3), ("orig_dtype", C.c_int),
+                ("steps", C.c_int * 3), ("wavelet", C.c_int), ("quant_alg", C.c_int),
+                ("bin_count", C.c_int), ("thresh_frac", C.c_float), ("threshold", C.c_float),
+                ("max_val", C.c_float), ("dup_key", C.c_int), ("dup_value", C.c_int),
+                ("maximum_value", C.c_int), ("n_decode", C.c_int), ("n_boundaries", C.c_int),
+                ("n_values", C.c_int), ("payload_offset", C.c_int64),
+                ("payload_bytes", C.c_int64)]
+
+
 def _declare(L):
     L.wv_last_error.restype = C.c_char_p
     L.wv_version.restype = C.c_char_p
@@ -140,6 +150,19 @@ def _declare(L):
     L.wv_dequantize.argtypes = [P, I64, C.POINTER(QuantParams), P, P]
     L.wv_write_cube.restype = I64
     L.wv_write_cube.argtypes = [C.POINTER(CubeDesc), P, I64, P, P, I64, P]
+    if hasattr(L, "wv_cube_parse"):
+        L.wv_cube_parse.argtypes = [P, I64, C.POINTER(CubeInfo), P, C.c_int, P, P, C.c_int]
+        L.wv_huff_decode_host.restype = I64
+        L.wv_huff_decode_host.argtypes = [P, C.c_int, C.c_int, C.c_int, P, I64, P, I64]
+        L.wv_write_cube_raw.restype = I64
+        L.wv_write_cube_raw.argtypes = [P, C.c_int, P, P, I64]
+    if hasattr(L, "wv_decompress"):
+        L.wv_decompress_work_bytes.restype = I64
+        L.wv_decompress_work_bytes.argtypes = [C.POINTER(CubeInfo)]
+        L.wv_decompress_device.argtypes = [C.POINTER(CubeInfo), P, P, P, P, I64, P, P]
+        L.wv_decompress.restype = I64
+        L.wv_decompress.argtypes = [P, I64, P, I64, P, I64, C.POINTER(CubeInfo),
+                                    C.POINTER(C.c_double), P]
     if hasattr(L, "wv_encode_device"):
         L.wv_encode_ws_bytes.restype = I64
         L.wv_encode_ws_bytes.argtypes = [I64, C.c_int]
@@ -463,6 +486,73 @@ def write_cube_payload(desc_kwargs, book, words_host):
     buf = np.empty(cap, np.uint8)
     n = check(lib().wv_write_cube_payload(C.byref(d), book.handle, w.ctypes.data_as(P), w.size,
                                           buf.ctypes.data_as(P), cap))
+    return buf[:n].tobytes()
+
+
+def cube_parse(file_bytes):
+    """The first compressed cubelet of a .cube image: (CubeInfo, decode table,
+    boundaries, values) -- the arrays as numpy."""
+    buf = np.frombuffer(file_bytes, np.uint8)
+    info = CubeInfo()
+    check(lib().wv_cube_parse(buf.ctypes.data_as(P), buf.size, C.byref(info), None, 0, None,
+                              None, 0))
+    table = np.empty(max(info.n_decode, 1), np.int32)
+    cap = max(info.n_boundaries, info.n_values, 1)
+    bnd = np.empty(cap, np.float32)
+    val = np.empty(cap, np.float32)
+    check(lib().wv_cube_parse(buf.ctypes.data_as(P), buf.size, C.byref(info),
+                              table.ctypes.data_as(P), table.size, bnd.ctypes.data_as(P),
+                              val.ctypes.data_as(P), cap))
+    return info, table[:info.n_decode], bnd[:info.n_boundaries], val[:info.n_values]
+
+
+def huff_decode_host(table, dup_key, dup_value, words, count):
+    t = np.ascontiguousarray(table, np.int32)
+    w = np.ascontiguousarray(words).view(np.uint32)
+    out = np.zeros(count, np.int32)
+    n = lib().wv_huff_decode_host(t.ctypes.data_as(P), t.size, dup_key, dup_value,
+                                  w.ctypes.data_as(P), w.size, out.ctypes.data_as(P), count)
+    check(n if n < 0 else 0)
+    return out[:n]
+
+
+OUT_NP = {UINT8: np.uint8, INT32: np.int32, FLOAT32: np.float32}
+
+
+def decompress(file_bytes, stream=None, with_info=False):
+    """GPU decompression of a .cube image: the reconstructed volume, indexed
+    [z, y, x], in the original type."""
+    import torch
+    buf = np.frombuffer(file_bytes, np.uint8)
+    info, _, _, _ = cube_parse(file_bytes)
+    wb = check_size(lib().wv_decompress_work_bytes(C.byref(info)))
+    work = torch.empty(wb, dtype=torch.uint8, device="cuda")
+    dt = info.orig_dtype if info.orig_dtype in (UINT8, INT32) else FLOAT32
+    out = np.empty((info.orig[2], info.orig[1], info.orig[0]), OUT_NP[dt])
+    ms = C.c_double()
+    n = lib().wv_decompress(buf.ctypes.data_as(P), buf.size, _ptr(work), wb,
+                            out.ctypes.data_as(P), out.nbytes, C.byref(info), C.byref(ms),
+                            _stream(stream))
+    check(n if n < 0 else 0)
+    return (out, info, ms.value) if with_info else out
+
+
+def check_size(n):
+    if n < 0:
+        check(int(n))
+    return n
+
+
+def write_cube_raw(vol):
+    """An uncompressed cubelet file holding vol ([z, y, x]; uint8 / int32 / float32)."""
+    v = np.ascontiguousarray(vol)
+    dt = {np.dtype(np.uint8): UINT8, np.dtype(np.int32): INT32,
+          np.dtype(np.float32): FLOAT32}[v.dtype]
+    size = (C.c_int * 3)(v.shape[2], v.shape[1], v.shape[0])
+    cap = v.nbytes + (1 << 16)
+    buf = np.empty(cap, np.uint8)
+    n = lib().wv_write_cube_raw(size, dt, v.ctypes.data_as(P), buf.ctypes.data_as(P), cap)
+    check(n if n < 0 else 0)
     return buf[:n].tobytes()
 
 

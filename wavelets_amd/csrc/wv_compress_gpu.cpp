@@ -13,6 +13,11 @@
 //                               transform) and print MSE / PSNR
 //     -lloyd-iters <n>          Lloyd iteration cap (default: none)
 //     -q                        quiet
+//     -d                        decompress (test_compress_common.cc:80):
+//                               the first compressed cubelet of the input
+//                               back to an uncompressed cubelet of its
+//                               original type and size (decompressFile,
+//                               test_compress_cpu.cc:187-236)
 //
 // Reads the first uncompressed cubelet of the input .cube file (uint8,
 // int32 or float32), compresses it on the GPU and writes one compressed
@@ -128,11 +133,72 @@ bool read_first_uncompressed(const char *path, InputCube &c) {
   return false;
 }
 
+double now_ms();
+
+// decompressFile: read, decode on the host, dequantize + inverse transform +
+// original type on the GPU, write an uncompressed cubelet with the footer
+int decompress_file(const char *in_path, const char *out_path, bool quiet) {
+  const double t0 = now_ms();
+  FILE *f = fopen(in_path, "rb");
+  if (!f) { fprintf(stderr, "cannot open %s\n", in_path); return 1; }
+  std::vector<uint8_t> file;
+  {
+    uint8_t buf[1 << 16];
+    size_t n;
+    while ((n = fread(buf, 1, sizeof buf, f)) > 0) file.insert(file.end(), buf, buf + n);
+    fclose(f);
+  }
+  wv_cube_info info;
+  if (wv_cube_parse(file.data(), (int64_t)file.size(), &info, nullptr, 0, nullptr, nullptr, 0) !=
+      WV_OK) {
+    fprintf(stderr, "%s\n", wv_last_error());
+    return 1;
+  }
+  if (!quiet)
+    printf("Read %dx%dx%d data file: %.2f ms\n", info.size[0], info.size[1], info.size[2],
+           now_ms() - t0);
+  const int64_t work_bytes = wv_decompress_work_bytes(&info);
+  if (work_bytes < 0) { fprintf(stderr, "%s\n", wv_last_error()); return 1; }
+  void *work_d = nullptr;
+  if (hipMalloc(&work_d, work_bytes) != hipSuccess) { fprintf(stderr, "hipMalloc failed\n"); return 1; }
+  const int dt = info.orig_dtype == WV_DTYPE_UINT8 || info.orig_dtype == WV_DTYPE_INT32
+                     ? info.orig_dtype : WV_DTYPE_FLOAT32;
+  const int64_t out_bytes =
+      (int64_t)info.orig[0] * info.orig[1] * info.orig[2] * (dt == WV_DTYPE_UINT8 ? 1 : 4);
+  std::vector<uint8_t> out((size_t)out_bytes);
+  const double t1 = now_ms();
+  double ms_decode = 0;
+  const int64_t n = wv_decompress(file.data(), (int64_t)file.size(), work_d, work_bytes,
+                                  out.data(), out_bytes, &info, &ms_decode, nullptr);
+  (void)hipFree(work_d);
+  if (n < 0) { fprintf(stderr, "decompress: %s\n", wv_last_error()); return 1; }
+  const double t2 = now_ms();
+  std::vector<uint8_t> cube((size_t)out_bytes + (1 << 16));
+  const int64_t nb = wv_write_cube_raw(info.orig, dt, out.data(), cube.data(), (int64_t)cube.size());
+  if (nb < 0) { fprintf(stderr, "write: %s\n", wv_last_error()); return 1; }
+  FILE *g = fopen(out_path, "wb");
+  if (!g || fwrite(cube.data(), 1, (size_t)nb, g) != (size_t)nb) {
+    fprintf(stderr, "cannot write %s\n", out_path);
+    if (g) fclose(g);
+    return 1;
+  }
+  fclose(g);
+  if (!quiet) {
+    printf("Huffman decode (host): %.2f ms\n", ms_decode);
+    printf("Decompress (decode, H2D, dequantize + inverse transform + type on the GPU, D2H): "
+           "%.2f ms\n", t2 - t1);
+    printf("Write file: %.2f ms\n", now_ms() - t2);
+    printf("Total: %.2f ms\n", now_ms() - t0);
+  }
+  return 0;
+}
+
 void usage() {
   fprintf(stderr,
           "\n  wv_compress_gpu [opts] <input.cube> <output.cube>\n"
           "    -steps <n|x,y|x,y,z> -thresh <f> -qcount <n> -wave <haar|cdf97>\n"
-          "    -qalg <uniform|log|lloyd> -noz -err -lloyd-iters <n> -q\n\n");
+          "    -qalg <uniform|log|lloyd> -noz -err -lloyd-iters <n> -q\n"
+          "  wv_compress_gpu -d [-q] <compressed.cube> <output.cube>\n\n");
   exit(1);
 }
 
@@ -171,7 +237,7 @@ int main(int argc, char **argv) {
   opt.compress_zeros = 1;
   opt.lloyd_max_iters = -1;
   opt.with_footer = 1;                     // the GPU driver writes the index
-  bool quiet = false, do_err = false;
+  bool quiet = false, do_err = false, do_decompress = false;
   int a = 1;
   for (; a < argc && argv[a][0] == '-'; a++) {
     const char *s = argv[a];
@@ -216,6 +282,8 @@ int main(int argc, char **argv) {
       opt.lloyd_max_iters = atoi(argv[a]);
     } else if (!strcmp(s, "-q")) {
       quiet = true;
+    } else if (!strcmp(s, "-d")) {
+      do_decompress = true;
     } else {
       fprintf(stderr, "Unrecognized option: \"%s\"\n", s);
       return 1;
@@ -223,6 +291,7 @@ int main(int argc, char **argv) {
   }
   if (argc - a != 2) usage();
   const char *in_path = argv[a], *out_path = argv[a + 1];
+  if (do_decompress) return decompress_file(in_path, out_path, quiet);
 
   const double t0 = now_ms();
   InputCube cube;
