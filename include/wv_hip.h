@@ -325,6 +325,17 @@ int wv_cube_parse(const uint8_t *file, int64_t len, wv_cube_info *info,
 int64_t wv_huff_decode_host(const int32_t *table, int n_table, int dup_key,
                             int dup_value, const uint32_t *words, int64_t nwords,
                             int32_t *out, int64_t count);
+/* The same decoding on the device (words_d: the payload words).  The
+   stream is cut into 512-bit pieces decoded in parallel; pieces whose
+   assumed start is not a code boundary decode again from where their
+   predecessor ended, until every start agrees (*rounds_out: how many
+   rounds that took).  out_d receives count values.  Synchronises.
+   Returns WV_OK, or an error when the stream holds fewer values. */
+int64_t wv_huff_decode_ws_bytes(int64_t nwords, int n_table);
+int wv_huff_decode_device(const int32_t *table, int n_table, int dup_key,
+                          int dup_value, const uint32_t *words_d, int64_t nwords,
+                          int32_t *out_d, int64_t count, void *ws_d, int *rounds_out,
+                          void *stream);
 /* the device half: dequantize the bins (stored layout), invert the
    transform and restore the original type over the original size into
    out_d (x fastest; FLOAT32 when the original type was not recorded).
@@ -333,9 +344,9 @@ int64_t wv_decompress_work_bytes(const wv_cube_info *info);
 int wv_decompress_device(const wv_cube_info *info, const int32_t *bins_d,
                          const float *boundaries, const float *values,
                          void *work_d, int64_t work_bytes, void *out_d, void *stream);
-/* one call on a file image: parse, host decode, device half, D2H into
-   out_host.  Returns the output bytes; *ms_decode (optional) receives the
-   host decode time.  Synchronises. */
+/* one call on a file image: parse, payload H2D, device decode, device
+   half, D2H into out_host.  Returns the output bytes; *ms_decode
+   (optional) receives the time of the Huffman decode.  Synchronises. */
 int64_t wv_decompress(const uint8_t *file, int64_t len, void *work_d,
                       int64_t work_bytes, uint8_t *out_host, int64_t out_cap,
                       wv_cube_info *info_out, double *ms_decode, void *stream);

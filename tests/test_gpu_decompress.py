@@ -1,5 +1,5 @@
 """GPU tests of decompression (decompressFile, test_compress_cpu.cc:187-236):
-a .cube written by the oracle's compress is read, Huffman-decoded on the host,
+a .cube written by the oracle's compress is read, Huffman-decoded on the device,
 dequantized and inverted on the GPU and restored to the original type
 (translateCubeDataToOriginal, test_compress_common.cc:699-747).  The result
 equals the oracle's own reconstruction of the same file bit for bit: oracle
@@ -121,3 +121,58 @@ def test_gpu_round_trip(wv):
 def test_decompress_rejects_non_cube(wv):
     with pytest.raises(wv.WvError):
         wv.decompress(b"not a cube file" + b"\0" * 80)
+
+
+def _stream_bins(n, nb, seed, p_zero=0.85, max_run=3000, wide=False):
+    rng = np.random.default_rng(seed)
+    zb = (nb - 1) // 2
+    x = np.clip(rng.geometric(0.2, n) * rng.choice([-1, 1], n) + zb, 0, nb - 1).astype(np.int32)
+    pos = 0
+    while pos < n:
+        run = int(rng.integers(1, max_run)) if rng.random() < p_zero else 0
+        x[pos:pos + run] = zb
+        pos += run + int(rng.integers(1, 8))
+    if wide:
+        sel = rng.random(n) < 0.1
+        x[sel] = rng.integers(0, nb, int(sel.sum()))
+    return x, zb
+
+
+@pytest.mark.parametrize("n,nb,dup,wide,seed", [
+    (50_000, 256, True, False, 0), (50_000, 256, False, False, 1),
+    (1_234_567, 1024, True, False, 2), (300_000, 65_536, True, True, 4),
+    (9000, 2, True, False, 5), (37, 16, False, True, 6), (2_000_000, 1024, True, True, 7)])
+def test_device_decoder_matches_oracle(wv, n, nb, dup, wide, seed):
+    """Parallel decoding with self-synchronising subsequences gives the
+    oracle decoder's values (runs of the dup value over piece boundaries,
+    codes longer than the 12-bit lookup, no dup key)."""
+    x, zb = _stream_bins(n, nb, seed, wide=wide)
+    d = zb if dup else -1
+    h = O.Huffman(x, nb, d)
+    words, _ = h.encode()
+    table = h.decode_table()
+    key = nb if dup else -1
+    want = O.huff_decode(table, key, d, words, n)
+    assert np.array_equal(want, x)
+    got, rounds = wv.huff_decode_device(table, key, d, words, n)
+    assert np.array_equal(got, want)
+    assert rounds <= 16, rounds   # synchronisation takes a few rounds
+
+
+def test_device_decoder_one_symbol(wv):
+    """One symbol only: its code has no bits, the stream is empty, and like
+    the reference decoder (and the oracle) no value can be read."""
+    x = np.full(3, 7, np.int32)
+    h = O.Huffman(x, 16, -1)
+    words, _ = h.encode()
+    assert O.huff_decode(h.decode_table(), -1, -1, words, 3).size == 0
+    with pytest.raises(wv.WvError):
+        wv.huff_decode_device(h.decode_table(), -1, -1, words, 3)
+
+
+def test_device_decoder_short_stream_is_an_error(wv):
+    x, zb = _stream_bins(20_000, 256, 9)
+    h = O.Huffman(x, 256, zb)
+    words, _ = h.encode()
+    with pytest.raises(wv.WvError):
+        wv.huff_decode_device(h.decode_table(), 256, zb, words[: len(words) // 2], x.size)

@@ -156,6 +156,11 @@ def _declare(L):
         L.wv_huff_decode_host.argtypes = [P, C.c_int, C.c_int, C.c_int, P, I64, P, I64]
         L.wv_write_cube_raw.restype = I64
         L.wv_write_cube_raw.argtypes = [P, C.c_int, P, P, I64]
+    if hasattr(L, "wv_huff_decode_device"):
+        L.wv_huff_decode_ws_bytes.restype = I64
+        L.wv_huff_decode_ws_bytes.argtypes = [I64, C.c_int]
+        L.wv_huff_decode_device.argtypes = [P, C.c_int, C.c_int, C.c_int, P, I64, P, I64, P,
+                                            C.POINTER(C.c_int), P]
     if hasattr(L, "wv_decompress"):
         L.wv_decompress_work_bytes.restype = I64
         L.wv_decompress_work_bytes.argtypes = [C.POINTER(CubeInfo)]
@@ -514,6 +519,22 @@ def huff_decode_host(table, dup_key, dup_value, words, count):
                                   w.ctypes.data_as(P), w.size, out.ctypes.data_as(P), count)
     check(n if n < 0 else 0)
     return out[:n]
+
+
+def huff_decode_device(table, dup_key, dup_value, words, count, stream=None):
+    """Device decoding of a payload (numpy words); returns (values, rounds)."""
+    import torch
+    t = np.ascontiguousarray(table, np.int32)
+    w = np.ascontiguousarray(words).view(np.uint32)
+    wd = torch.from_numpy(w.view(np.int32).copy()).cuda() if w.size else None
+    out = torch.empty(max(count, 1), dtype=torch.int32, device="cuda")
+    ws = torch.empty(check_size(lib().wv_huff_decode_ws_bytes(w.size, t.size)), dtype=torch.uint8,
+                     device="cuda")
+    rounds = C.c_int()
+    check(lib().wv_huff_decode_device(t.ctypes.data_as(P), t.size, dup_key, dup_value,
+                                      _ptr(wd), w.size, _ptr(out), count, _ptr(ws),
+                                      C.byref(rounds), _stream(stream)))
+    return out[:count].cpu().numpy(), rounds.value
 
 
 OUT_NP = {UINT8: np.uint8, INT32: np.int32, FLOAT32: np.float32}

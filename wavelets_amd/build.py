@@ -26,7 +26,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 DEVICE = ["--offload-arch=" + ARCH, "-munsafe-fp-atomics"]
 
-SOURCES = ["wv_dwt.hip", "wv_select.hip", "wv_quant.hip", "wv_lloyd.hip", "wv_encode.hip", "wv_restore.hip",
+SOURCES = ["wv_dwt.hip", "wv_select.hip", "wv_quant.hip", "wv_lloyd.hip", "wv_encode.hip", "wv_decode.hip", "wv_restore.hip",
            "wv_api.cpp", "wv_compress.cpp", "wv_params.cpp", "wv_huffman.cpp",
            "wv_batch.cpp", "wv_serve.cpp", "wv_cube.cpp"]
 

@@ -47,7 +47,7 @@ namespace {
 inline int64_t al256(int64_t v) { return (v + 255) & ~(int64_t)255; }
 
 struct DLayout {
-  int64_t table, coef, rec, ws, bins, out, total;
+  int64_t table, coef, rec, ws, bins, out, words, dws, total;
 };
 
 int out_dtype_of(const wv_cube_info *I) {
@@ -77,6 +77,8 @@ DLayout dlayout(const wv_geom &g, const wv_cube_info *I) {
   L.bins = o; o += al256(g.count * 4);
   const int64_t on = (int64_t)I->orig[0] * I->orig[1] * I->orig[2];
   L.out = o; o += al256(on * (out_dtype_of(I) == WV_DTYPE_UINT8 ? 1 : 4));
+  L.words = o; o += al256(I->payload_bytes);
+  L.dws = o; o += al256(wv_huff_decode_ws_bytes(I->payload_bytes / 4, I->n_decode));
   L.total = o;
   return L;
 }
@@ -191,24 +193,19 @@ extern "C" int64_t wv_decompress(const uint8_t *file, int64_t len, void *work_d,
     set_error("wv_decompress: output buffer %lld < %lld", (long long)out_cap, (long long)out_bytes);
     return WV_ERR_NOMEM;
   }
-  // readQuantData: the payload words, decoded on the host
-  const auto t0 = std::chrono::steady_clock::now();
-  const int64_t nwords = I.payload_bytes / 4;
-  std::vector<uint32_t> words((size_t)nwords);
-  if (nwords) std::memcpy(words.data(), file + I.payload_offset, (size_t)nwords * 4);
-  std::vector<int32_t> bins((size_t)g.count);
-  const int64_t got = wv_huff_decode_host(table.data(), I.n_decode, I.dup_key, I.dup_value,
-                                          words.data(), nwords, bins.data(), g.count);
-  if (got < 0) return (int)got;
-  if (got != g.count) {
-    set_error("read only %lld of %lld values", (long long)got, (long long)g.count);
-    return WV_ERR_ARG;
-  }
-  if (ms_decode)
-    *ms_decode = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  // readQuantData: the payload words, decoded on the device
   hipStream_t s = (hipStream_t)stream;
   char *w = (char *)work_d;
-  DHIP(hipMemcpyAsync(w + L.bins, bins.data(), g.count * 4, hipMemcpyHostToDevice, s), "H2D bins");
+  const int64_t nwords = I.payload_bytes / 4;
+  if (nwords)
+    DHIP(hipMemcpyAsync(w + L.words, file + I.payload_offset, nwords * 4, hipMemcpyHostToDevice, s),
+         "H2D payload");
+  const auto t0 = std::chrono::steady_clock::now();
+  DCHK(wv_huff_decode_device(table.data(), I.n_decode, I.dup_key, I.dup_value,
+                             (const uint32_t *)(w + L.words), nwords, (int32_t *)(w + L.bins),
+                             g.count, w + L.dws, nullptr, stream));
+  if (ms_decode)
+    *ms_decode = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   DCHK(wv_decompress_device(&I, (const int32_t *)(w + L.bins),
                             bounds.empty() ? nullptr : bounds.data(),
                             values.empty() ? nullptr : values.data(), work_d, work_bytes,
