@@ -225,7 +225,66 @@ def single_cubelet(args, wv, torch, dev, steps, warmup):
             "payload_bytes": int(host_words.numel() * 4), "payload_bits": int(nbits),
             "note": "wv_encode_device on the 512^3 bins + payload D2H (median of 5)"}
         del work
+        out["decompress"] = decompress_leg(wv, torch, dev, fbuf[:info.file_bytes].tobytes(),
+                                           vol.nbytes)
     return out, roof, vol
+
+
+def decompress_leg(wv, torch, dev, file_bytes, out_bytes):
+    """Decompression of the 512^3 file: device-resident (payload on the device:
+    parallel Huffman decode, dequantize, inverse transform, float output on
+    the device) and the one call from a host file image to a host volume."""
+    L = wv.lib()
+    info, table, bnd, val = wv.cube_parse(file_bytes)
+    nwords = info.payload_bytes // 4
+    words = np.frombuffer(file_bytes, np.uint32, nwords, info.payload_offset)
+    words_d = torch.from_numpy(words.view(np.int32).copy()).to(dev)
+    count = info.size[0] * info.size[1] * info.size[2]
+    bins_d = torch.empty(count, dtype=torch.int32, device=dev)
+    dws = torch.empty(L.wv_huff_decode_ws_bytes(nwords, table.size), dtype=torch.uint8, device=dev)
+    wb = L.wv_decompress_work_bytes(C.byref(info))
+    work = torch.empty(wb, dtype=torch.uint8, device=dev)
+    out_d = torch.empty(out_bytes // 4, dtype=torch.float32, device=dev)
+    sp = wv._stream()
+    rounds = C.c_int()
+    t_dec, t_all = [], []
+    for _ in range(6):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        wv.check(L.wv_huff_decode_device(table.ctypes.data_as(C.c_void_p), table.size,
+                                         info.dup_key, info.dup_value,
+                                         C.c_void_p(words_d.data_ptr()), nwords,
+                                         C.c_void_p(bins_d.data_ptr()), count,
+                                         C.c_void_p(dws.data_ptr()), C.byref(rounds), sp))
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        wv.check(L.wv_decompress_device(C.byref(info), C.c_void_p(bins_d.data_ptr()), None, None,
+                                        C.c_void_p(work.data_ptr()), wb,
+                                        C.c_void_p(out_d.data_ptr()), sp))
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        t_dec.append((t1 - t0) * 1e3)
+        t_all.append((t2 - t0) * 1e3)
+    ms_dec = float(np.median(t_dec[1:]))
+    ms_all = float(np.median(t_all[1:]))
+    host_out = np.empty(out_bytes, np.uint8)
+    buf = np.frombuffer(file_bytes, np.uint8)
+    t_one = []
+    for _ in range(3):
+        t0 = time.perf_counter()
+        n = L.wv_decompress(buf.ctypes.data_as(C.c_void_p), buf.size, C.c_void_p(work.data_ptr()),
+                            wb, host_out.ctypes.data_as(C.c_void_p), out_bytes, None, None, sp)
+        wv.check(n if n < 0 else 0)
+        t_one.append((time.perf_counter() - t0) * 1e3)
+    ms_one = float(np.median(t_one[1:]))
+    return {
+        "device_resident_ms": round(ms_all, 3),
+        "gbps_output": round(out_bytes / (ms_all * 1e-3) / 1e9, 3),
+        "huffman_decode_ms": round(ms_dec, 3), "sync_rounds": rounds.value,
+        "one_call_ms": round(ms_one, 3),
+        "note": "512^3 float32 file (7.6 MB): device decode (incl. its host sync rounds) + "
+                "dequantize + inverse CDF97 + restore, payload and output on the device; "
+                "one_call = wv_decompress from a host file image to a host volume (512 MB D2H)"}
 
 
 def mixed_stream(args, wv, torch, dev, steps, warmup):

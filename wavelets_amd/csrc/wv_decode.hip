@@ -42,8 +42,9 @@ struct DecArgs {
   const uint32_t *words;
   int64_t nwords, nbits;
   int64_t nsub;
-  const int2 *lut;          // (value or node, length | leaf << 8)
+  const uint32_t *lut;      // leaf << 31 | (value or node) << 4 | length
   const int32_t *table;     // decode table, pairs
+  int n_table;
   int dup_key, dup_value;
   int64_t *start, *end_pos, *nstart;
   uint64_t *cnt, *off;
@@ -54,37 +55,85 @@ struct DecArgs {
   int fill;                 // out was filled with dup_value
 };
 
-__device__ inline uint32_t bits32(const DecArgs &A, int64_t p) {
-  const int64_t w = p >> 5;
-  const int sh = (int)(p & 31);
-  const uint32_t a = w < A.nwords ? A.words[w] : 0u;
-  const uint32_t b = w + 1 < A.nwords ? A.words[w + 1] : 0u;
-  return sh ? (a >> sh) | (b << (32 - sh)) : a;
+// The payload words of a block's subsequences (plus a few past them),
+// staged in LDS: decoding reads its bits there, with no global-memory
+// latency in the chain of dependent codes.  Reads past the staged words
+// (long codes at the end of the last subsequence) go to global memory.
+#define DEC_WPS (DEC_S / 32)
+#define DEC_BW (DEC_NT * DEC_WPS + 4)
+// word k sits at k + k / 16: the lanes of a wave read words 16 apart,
+// which the spare word per 16 spreads over the LDS banks
+#define DEC_SLOT(k) ((k) + ((k) >> 4))
+#define DEC_BWP (DEC_SLOT(DEC_BW) + 1)
+
+struct Bits {
+  const uint32_t *words;
+  int64_t nwords;
+  const uint32_t *sw;   // staged words
+  int64_t w0;           // index of sw[0]
+  __device__ inline uint32_t word(int64_t w) const {
+    const int64_t j = w - w0;
+    if (j >= 0 && j < DEC_BW) return sw[DEC_SLOT(j)];
+    return w < nwords ? words[w] : 0u;
+  }
+  __device__ inline uint32_t get32(int64_t p) const {
+    const int64_t w = p >> 5;
+    const int sh = (int)(p & 31);
+    const uint32_t a = word(w), b = word(w + 1);
+    return sh ? (a >> sh) | (b << (32 - sh)) : a;
+  }
+};
+
+__device__ inline Bits stage_words(const DecArgs &A, uint32_t *sw) {
+  const int64_t w0 = (int64_t)blockIdx.x * DEC_NT * DEC_WPS;
+  for (int k = threadIdx.x; k < DEC_BW; k += DEC_NT) {
+    const int64_t w = w0 + k;
+    sw[DEC_SLOT(k)] = w < A.nwords ? A.words[w] : 0u;
+  }
+  __syncthreads();
+  Bits b;
+  b.words = A.words;
+  b.nwords = A.nwords;
+  b.sw = sw;
+  b.w0 = w0;
+  return b;
 }
 
 // Decodes the code at bit p: returns the value, advances p; a dup key also
 // consumes its 3 length bits and sets *run to 2^(e+3) (else 1).
-__device__ inline int decode_one(const DecArgs &A, const int2 *lut, int64_t &p, int64_t &run) {
-  const uint32_t w = bits32(A, p);
-  const int2 e = lut[w & ((1u << DEC_K) - 1)];
+__device__ inline int decode_one(const DecArgs &A, const Bits &B, const uint32_t *lut,
+                                 const int32_t *table, int64_t &p, int64_t &run) {
+  const uint32_t w = B.get32(p);
+  const uint32_t e = lut[w & ((1u << DEC_K) - 1)];
+  const int ex = (int)((e >> 4) & 0x7ffffffu);
   int v;
-  if (e.y & 256) {
-    v = e.x;
-    p += e.y & 255;
+  if (e >> 31) {
+    v = ex;
+    p += e & 15u;
   } else {
-    int pos = e.x;
+    // past DEC_K bits: the rest of the code one bit at a time, from the
+    // bits already fetched while they last
+    int pos = ex;
+    uint32_t rest = w >> DEC_K;
+    int left = 32 - DEC_K;
     p += DEC_K;
     for (;;) {
-      const unsigned bit = (bits32(A, p) & 1u);
+      if (left == 0) {
+        rest = B.get32(p);
+        left = 32;
+      }
+      const unsigned bit = rest & 1u;
+      rest >>= 1;
+      left--;
       p++;
-      pos = A.table[pos + bit] * 2;
-      if (A.table[pos] == 0) break;
+      pos = table[pos + bit] * 2;
+      if (table[pos] == 0) break;
     }
-    v = A.table[pos + 1];
+    v = table[pos + 1];
   }
   run = 1;
   if (v == A.dup_key) {
-    const unsigned ebits = bits32(A, p) & 7u;
+    const unsigned ebits = B.get32(p) & 7u;
     p += 3;
     run = (int64_t)1 << (ebits + 3);
   }
@@ -96,19 +145,26 @@ __device__ inline int64_t sub_limit(const DecArgs &A, int64_t i) {
   return i + 1 < A.nsub ? (i + 1) * DEC_S : A.nbits;
 }
 
-__device__ inline void load_lut(const DecArgs &A, int2 *lut) {
+// the lookup table, and the decode table when it fits, into LDS
+#define DEC_TAB_LDS 8192
+__device__ inline const int32_t *load_tables(const DecArgs &A, uint32_t *lut, int32_t *tab) {
   for (int i = threadIdx.x; i < (1 << DEC_K); i += DEC_NT) lut[i] = A.lut[i];
+  const bool in_lds = A.n_table <= DEC_TAB_LDS;
+  if (in_lds)
+    for (int i = threadIdx.x; i < A.n_table; i += DEC_NT) tab[i] = A.table[i];
   __syncthreads();
+  return in_lds ? tab : A.table;
 }
 
 // decodes subsequence i from s: its end position and value count
-__device__ inline void decode_sub(const DecArgs &A, const int2 *lut, int64_t i, int64_t s) {
+__device__ inline void decode_sub(const DecArgs &A, const Bits &B, const uint32_t *lut,
+                                  const int32_t *tab, int64_t i, int64_t s) {
   const int64_t lim = sub_limit(A, i);
   int64_t p = s;
   uint64_t n = 0;
   while (p < lim) {
     int64_t run;
-    decode_one(A, lut, p, run);
+    decode_one(A, B, lut, tab, p, run);
     n += (uint64_t)run;
   }
   A.end_pos[i] = p;
@@ -116,13 +172,16 @@ __device__ inline void decode_sub(const DecArgs &A, const int2 *lut, int64_t i, 
 }
 
 __global__ void __launch_bounds__(DEC_NT) dec_first(DecArgs A) {
-  __shared__ int2 lut[1 << DEC_K];
-  load_lut(A, lut);
+  __shared__ uint32_t lut[1 << DEC_K];
+  __shared__ int32_t tab[DEC_TAB_LDS];
+  __shared__ uint32_t sw[DEC_BWP];
+  const int32_t *t = load_tables(A, lut, tab);
+  const Bits B = stage_words(A, sw);
   const int64_t i = (int64_t)blockIdx.x * DEC_NT + threadIdx.x;
   if (i >= A.nsub) return;
   const int64_t s = i * DEC_S;
   A.start[i] = s;
-  decode_sub(A, lut, i, s);
+  decode_sub(A, B, lut, t, i, s);
 }
 
 // a sync round, part 1: the start each subsequence must have
@@ -136,44 +195,62 @@ __global__ void dec_starts(DecArgs A) {
 
 // part 2: subsequences whose start changed decode again
 __global__ void __launch_bounds__(DEC_NT) dec_redo(DecArgs A) {
-  __shared__ int2 lut[1 << DEC_K];
-  __shared__ int any;
-  if (threadIdx.x == 0) any = 0;
-  load_lut(A, lut);
+  __shared__ uint32_t lut[1 << DEC_K];
+  __shared__ int32_t tab[DEC_TAB_LDS];
+  __shared__ uint32_t sw[DEC_BWP];
   const int64_t i = (int64_t)blockIdx.x * DEC_NT + threadIdx.x;
-  if (i < A.nsub && A.changed[i]) {
+  const bool mine = i < A.nsub && A.changed[i];
+  if (!__syncthreads_or(mine)) return;   // uniform: nothing to redo here
+  const int32_t *t = load_tables(A, lut, tab);
+  const Bits B = stage_words(A, sw);
+  if (mine) {
     const int64_t s = A.nstart[i];
     A.start[i] = s;
-    decode_sub(A, lut, i, s);
-    any = 1;
+    decode_sub(A, B, lut, t, i, s);
   }
-  __syncthreads();
-  if (threadIdx.x == 0 && any) atomicAdd(A.flag, 1ull);
+  if (threadIdx.x == 0) atomicAdd(A.flag, 1ull);
 }
 
-// exclusive prefix sum of the value counts (one block of 1024)
-__global__ void __launch_bounds__(1024) dec_scan(DecArgs A) {
+// values per block of DEC_NT subsequences
+__global__ void __launch_bounds__(DEC_NT) dec_bsum(DecArgs A) {
+  __shared__ uint64_t red[DEC_NT / 64];
+  const int64_t i = (int64_t)blockIdx.x * DEC_NT + threadIdx.x;
+  uint64_t v = i < A.nsub ? A.cnt[i] : 0;
+  for (int d = 32; d >= 1; d >>= 1) {
+    const uint32_t lo = __shfl_xor((uint32_t)v, d, 64), hi = __shfl_xor((uint32_t)(v >> 32), d, 64);
+    v += ((uint64_t)hi << 32) | lo;
+  }
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t t = 0;
+    for (int k = 0; k < DEC_NT / 64; k++) t += red[k];
+    A.off[blockIdx.x] = t;
+  }
+}
+
+// exclusive prefix over the block sums in place (one block of 1024, in
+// coalesced chunks); the total goes to flag[1]
+__global__ void __launch_bounds__(1024) dec_scan(DecArgs A, int64_t nblk) {
   __shared__ uint64_t s[1024];
   const int t = threadIdx.x;
-  const int64_t per = (A.nsub + 1023) / 1024;
-  const int64_t b = (int64_t)t * per;
-  const int64_t e = b + per < A.nsub ? b + per : A.nsub;
-  uint64_t acc = 0;
-  for (int64_t i = b; i < e; i++) acc += A.cnt[i];
-  s[t] = acc;
-  __syncthreads();
-  for (int d = 1; d < 1024; d <<= 1) {
-    const uint64_t y = t >= d ? s[t - d] : 0;
+  uint64_t carry = 0;
+  for (int64_t c0 = 0; c0 < nblk; c0 += 1024) {
+    const int64_t i = c0 + t;
+    const uint64_t v = i < nblk ? A.off[i] : 0;
+    s[t] = v;
     __syncthreads();
-    s[t] += y;
+    for (int d = 1; d < 1024; d <<= 1) {
+      const uint64_t y = t >= d ? s[t - d] : 0;
+      __syncthreads();
+      s[t] += y;
+      __syncthreads();
+    }
+    if (i < nblk) A.off[i] = carry + s[t] - v;
+    carry += s[1023];
     __syncthreads();
   }
-  uint64_t run = t > 0 ? s[t - 1] : 0;
-  for (int64_t i = b; i < e; i++) {
-    A.off[i] = run;
-    run += A.cnt[i];
-  }
-  if (t == 1023) A.flag[1] = run;   // values in the whole stream
+  if (t == 0) A.flag[1] = carry;   // values in the whole stream
 }
 
 __global__ void dec_fill(int32_t *out, int64_t n, int v) {
@@ -182,17 +259,32 @@ __global__ void dec_fill(int32_t *out, int64_t n, int v) {
 }
 
 __global__ void __launch_bounds__(DEC_NT) dec_write(DecArgs A) {
-  __shared__ int2 lut[1 << DEC_K];
-  load_lut(A, lut);
+  __shared__ uint32_t lut[1 << DEC_K];
+  __shared__ int32_t tab[DEC_TAB_LDS];
+  __shared__ uint32_t sw[DEC_BWP];
+  __shared__ uint64_t sc[DEC_NT];
+  const int32_t *t = load_tables(A, lut, tab);
+  const Bits B = stage_words(A, sw);
   const int64_t i = (int64_t)blockIdx.x * DEC_NT + threadIdx.x;
+  // the thread's output offset: the block's (scanned block sums) plus the
+  // counts of the block's subsequences before it
+  const uint64_t c = i < A.nsub ? A.cnt[i] : 0;
+  sc[threadIdx.x] = c;
+  __syncthreads();
+  for (int d = 1; d < DEC_NT; d <<= 1) {
+    const uint64_t y = (int)threadIdx.x >= d ? sc[threadIdx.x - d] : 0;
+    __syncthreads();
+    sc[threadIdx.x] += y;
+    __syncthreads();
+  }
   if (i >= A.nsub) return;
-  int64_t o = (int64_t)A.off[i];
+  int64_t o = (int64_t)(A.off[blockIdx.x] + sc[threadIdx.x] - c);
   if (o >= A.count) return;
   const int64_t lim = sub_limit(A, i);
   int64_t p = A.start[i];
   while (p < lim && o < A.count) {
     int64_t run;
-    const int v = decode_one(A, lut, p, run);
+    const int v = decode_one(A, B, lut, t, p, run);
     if (run > 1 || v == A.dup_key) {
       if (!A.fill) {
         const int64_t m = run < A.count - o ? run : A.count - o;
@@ -222,7 +314,7 @@ DecLayout dec_layout(int64_t nwords, int n_table) {
   const int64_t nsub = nbits > 0 ? (nbits + DEC_S - 1) / DEC_S : 1;
   DecLayout L;
   int64_t o = 0;
-  L.lut = o; o += al256(((int64_t)1 << DEC_K) * 8);
+  L.lut = o; o += al256(((int64_t)1 << DEC_K) * 4);
   L.table = o; o += al256((int64_t)n_table * 4);
   L.start = o; o += al256(nsub * 8);
   L.end = o; o += al256(nsub * 8);
@@ -279,18 +371,20 @@ extern "C" int wv_huff_decode_device(const int32_t *table, int n_table, int dup_
   char *w = (char *)ws_d;
   // lookup table (the host decoder's rule): leaf within DEC_K bits, or the
   // node reached after DEC_K bits
-  std::vector<int2> lut((size_t)1 << DEC_K);
+  std::vector<uint32_t> lut((size_t)1 << DEC_K);
   for (uint32_t p = 0; p < (1u << DEC_K); p++) {
     int pos = 0;
-    int2 e = make_int2(0, DEC_K);
+    uint32_t e = 0;
+    bool leaf = false;
     for (int b = 0; b < DEC_K; b++) {
       pos = table[pos + ((p >> b) & 1u)] * 2;
       if (table[pos] == 0) {
-        e = make_int2(table[pos + 1], (b + 1) | 256);
+        e = (1u << 31) | ((uint32_t)table[pos + 1] << 4) | (uint32_t)(b + 1);
+        leaf = true;
         break;
       }
     }
-    if (!(e.y & 256)) e.x = pos;
+    if (!leaf) e = ((uint32_t)pos << 4) | DEC_K;
     lut[p] = e;
   }
   // the walk past DEC_K bits reads the table on the device
@@ -301,8 +395,9 @@ extern "C" int wv_huff_decode_device(const int32_t *table, int n_table, int dup_
   A.nwords = nwords;
   A.nbits = nwords * 32;
   A.nsub = A.nbits > 0 ? (A.nbits + DEC_S - 1) / DEC_S : 1;
-  A.lut = (const int2 *)(w + L.lut);
+  A.lut = (const uint32_t *)(w + L.lut);
   A.table = (const int32_t *)tab_d;
+  A.n_table = n_table;
   A.dup_key = dup_key;
   A.dup_value = dup_value;
   A.start = (int64_t *)(w + L.start);
@@ -319,7 +414,7 @@ extern "C" int wv_huff_decode_device(const int32_t *table, int n_table, int dup_
   int rounds = 0;
   unsigned long long hflag[2] = {0, 0};
   const unsigned gsub = (unsigned)((A.nsub + DEC_NT - 1) / DEC_NT);
-  hipError_t e = hipMemcpyAsync(w + L.lut, lut.data(), lut.size() * sizeof(int2),
+  hipError_t e = hipMemcpyAsync(w + L.lut, lut.data(), lut.size() * sizeof(uint32_t),
                                 hipMemcpyHostToDevice, s);
   if (e == hipSuccess)
     e = hipMemcpyAsync(tab_d, table, tab_bytes, hipMemcpyHostToDevice, s);
@@ -341,7 +436,8 @@ extern "C" int wv_huff_decode_device(const int32_t *table, int n_table, int dup_
     if (e != hipSuccess || hflag[0] == 0) break;
   }
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(dec_scan, dim3(1), dim3(1024), 0, s, A);
+    hipLaunchKernelGGL(dec_bsum, dim3(gsub), dim3(DEC_NT), 0, s, A);
+    hipLaunchKernelGGL(dec_scan, dim3(1), dim3(1024), 0, s, A, (int64_t)gsub);
     e = hipMemcpyAsync(hflag, A.flag, 16, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
   }

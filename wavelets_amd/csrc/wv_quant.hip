@@ -316,20 +316,46 @@ hipError_t quantize_hist_uniform_dev(const float *coef, int64_t n,
 }
 
 // dequantization through a host-built table of the bin values
-__global__ void dequant_kernel(const int32_t *bins, int64_t n,
-                               const float *__restrict__ table, int nb,
-                               float *__restrict__ out) {
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int b = bins[i];
-    out[i] = (b >= 0 && b < nb) ? table[b] : 0.f;
+// Quantizer::dequantizeRow: out[i] = table[bins[i]] (0 for bins outside
+// the table); 16-byte accesses, the table in LDS when it fits
+#define DQ_LDS 8192
+template <bool LDS>
+__global__ void __launch_bounds__(WV_BLOCK)
+dequant_kernel(const int32_t *__restrict__ bins, int64_t n, const float *__restrict__ table,
+               int nb, float *__restrict__ out) {
+  __shared__ float t[LDS ? DQ_LDS : 1];
+  const float *tab = table;
+  if (LDS) {
+    for (int i = threadIdx.x; i < nb; i += WV_BLOCK) t[i] = table[i];
+    __syncthreads();
+    tab = t;
   }
+  auto dq = [&](int b) { return (b >= 0 && b < nb) ? tab[b] : 0.f; };
+  const int64_t stride = (int64_t)gridDim.x * WV_BLOCK;
+  const bool aligned = (((uintptr_t)bins | (uintptr_t)out) & 15) == 0;
+  const int64_t nvec = aligned ? n >> 2 : 0;
+  const int4 *b4 = (const int4 *)bins;
+  float4 *o4 = (float4 *)out;
+  for (int64_t i = (int64_t)blockIdx.x * WV_BLOCK + threadIdx.x; i < nvec; i += stride) {
+    const int4 b = b4[i];
+    o4[i] = make_float4(dq(b.x), dq(b.y), dq(b.z), dq(b.w));
+  }
+  for (int64_t i = nvec * 4 + (int64_t)blockIdx.x * WV_BLOCK + threadIdx.x; i < n; i += stride)
+    out[i] = dq(bins[i]);
 }
 
 hipError_t dequantize_table(const int32_t *bins, int64_t n, const float *table,
                             int bin_count, float *out, hipStream_t s) {
-  hipLaunchKernelGGL(dequant_kernel, dim3(grid_for(n * 4)), dim3(WV_BLOCK), 0, s,
-                     bins, n, table, bin_count, out);
+  int64_t g = (n / 4 + WV_BLOCK - 1) / WV_BLOCK;
+  const int64_t cap = 8 * (int64_t)cu_count() * (2048 / WV_BLOCK);
+  if (g > cap) g = cap;
+  if (g < 1) g = 1;
+  if (bin_count <= DQ_LDS)
+    hipLaunchKernelGGL(dequant_kernel<true>, dim3((unsigned)g), dim3(WV_BLOCK), 0, s, bins, n,
+                       table, bin_count, out);
+  else
+    hipLaunchKernelGGL(dequant_kernel<false>, dim3((unsigned)g), dim3(WV_BLOCK), 0, s, bins, n,
+                       table, bin_count, out);
   return hipGetLastError();
 }
 

@@ -14,18 +14,16 @@
 
 namespace wv {
 
-// padded x,y,z floats -> original x,y,z in the original type
+// padded x,y,z floats -> original x,y,z in the original type; block
+// (bx, y, z) covers x in [bx * WV_BLOCK, ...) of row (y, z)
 template <int DT>
 __global__ void __launch_bounds__(WV_BLOCK)
 restore_kernel(const float *__restrict__ src, int px, int py, int ox, int oy, int oz,
                void *__restrict__ dst) {
-  const int64_t n = (int64_t)ox * oy * oz;
-  const int64_t stride = (int64_t)gridDim.x * WV_BLOCK;
-  for (int64_t i = (int64_t)blockIdx.x * WV_BLOCK + threadIdx.x; i < n; i += stride) {
-    const int x = (int)(i % ox);
-    const int64_t r = i / ox;
-    const int y = (int)(r % oy);
-    const int z = (int)(r / oy);
+  const int x = blockIdx.x * WV_BLOCK + threadIdx.x;
+  const int y = blockIdx.y, z = blockIdx.z;
+  if (x < ox) {
+    const int64_t i = ((int64_t)z * oy + y) * ox + x;
     const float f = src[((int64_t)z * py + y) * px + x];
     if (DT == WV_DTYPE_UINT8) {
       // floatToByte: <= 0 -> 0, >= 255 -> 255, else truncation
@@ -146,24 +144,26 @@ extern "C" int wv_decompress_device(const wv_cube_info *info, const int32_t *bin
   if (rc == WV_OK && hipStreamSynchronize(s) != hipSuccess) rc = WV_ERR_HIP;
   wv_quant_params_free(&p);
   DCHK(rc);
-  DCHK(wv_dwt3d_inverse(&g, (const float *)(w + L.coef), (float *)(w + L.rec),
-                        (float *)(w + L.ws), s));
   const int dt = out_dtype_of(info);
-  const int64_t on = (int64_t)info->orig[0] * info->orig[1] * info->orig[2];
-  int64_t blocks = (on + WV_BLOCK * 4 - 1) / (WV_BLOCK * 4);
-  if (blocks > 8192) blocks = 8192;
-  if (blocks < 1) blocks = 1;
+  // float32 with no padding: the inverse transform writes the output itself
+  const bool direct = dt == WV_DTYPE_FLOAT32 && g.padded[0] == info->orig[0] &&
+                      g.padded[1] == info->orig[1] && g.padded[2] == info->orig[2];
+  DCHK(wv_dwt3d_inverse(&g, (const float *)(w + L.coef), direct ? (float *)out_d : (float *)(w + L.rec),
+                        (float *)(w + L.ws), s));
+  if (direct) return WV_OK;
+  const dim3 blocks((unsigned)((info->orig[0] + WV_BLOCK - 1) / WV_BLOCK), (unsigned)info->orig[1],
+                    (unsigned)info->orig[2]);
   const float *rec = (const float *)(w + L.rec);
   if (dt == WV_DTYPE_UINT8)
-    hipLaunchKernelGGL(restore_kernel<WV_DTYPE_UINT8>, dim3((unsigned)blocks), dim3(WV_BLOCK), 0,
+    hipLaunchKernelGGL(restore_kernel<WV_DTYPE_UINT8>, blocks, dim3(WV_BLOCK), 0,
                        s, rec, g.padded[0], g.padded[1], info->orig[0], info->orig[1],
                        info->orig[2], out_d);
   else if (dt == WV_DTYPE_INT32)
-    hipLaunchKernelGGL(restore_kernel<WV_DTYPE_INT32>, dim3((unsigned)blocks), dim3(WV_BLOCK), 0,
+    hipLaunchKernelGGL(restore_kernel<WV_DTYPE_INT32>, blocks, dim3(WV_BLOCK), 0,
                        s, rec, g.padded[0], g.padded[1], info->orig[0], info->orig[1],
                        info->orig[2], out_d);
   else
-    hipLaunchKernelGGL(restore_kernel<WV_DTYPE_FLOAT32>, dim3((unsigned)blocks), dim3(WV_BLOCK),
+    hipLaunchKernelGGL(restore_kernel<WV_DTYPE_FLOAT32>, blocks, dim3(WV_BLOCK),
                        0, s, rec, g.padded[0], g.padded[1], info->orig[0], info->orig[1],
                        info->orig[2], out_d);
   DHIP(hipGetLastError(), "restore");
