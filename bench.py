@@ -361,6 +361,9 @@ def mixed_stream(args, wv, torch, dev, steps, warmup):
         pb = work[:n * 4].view(torch.int32)
         if not torch.equal(pb, b.bins[i]):
             same = False
+    # select statistics: candidates examined, exact fallbacks (top bit)
+    nc = [b.result(i)[0].n_candidates for i in range(len(cubes))]
+    fallbacks = sum(1 for v in nc if v < 0)
     out = {
         "workload": "mixed stream: %d cubelets {32^3, 64^3, 128^3, 256x256x64, 512x512x16}, "
                     "u8/i16/f32, CDF97|Haar, min(3,log2 dim) steps, log 256 bins, "
@@ -374,6 +377,8 @@ def mixed_stream(args, wv, torch, dev, steps, warmup):
         "batched_over_plain": round(batched_gbps / plain_gbps, 3),
         "plain_bins_equal_batched": same,
         "groups": len(b.groups()),
+        "select_fallbacks": fallbacks,
+        "select_max_candidates": max(v & 0x7fffffff for v in nc),
     }
     b.close()
     del cubes, work

@@ -169,3 +169,26 @@ def test_batch_select_fallback_on_overflowing_window(wv):
     r, _ = b.result(0)
     assert r.n_candidates < 0          # the exact path ran
     b.close()
+
+
+def test_batch_select_zero_ties(wv):
+    """Blocks of constant value: most Haar detail coefficients are exactly
+    zero, the rank lies inside the zeros or just above them, and the zeros
+    are counted rather than listed -- no fallback, the oracle's result."""
+    shape = (32, 64, 64)
+    rng = np.random.default_rng(3)
+    tiles = rng.integers(0, 255, (4, 8, 8)).astype(np.uint8)
+    v = np.kron(tiles, np.ones((8, 8, 8), np.uint8))
+    v[::16, ::16, ::16] = rng.integers(0, 255, v[::16, ::16, ::16].shape).astype(np.uint8)
+    # about 98.3% of the coefficients are exact zeros
+    cubes = [dict(vol=torch.from_numpy(v).cuda(), wavelet=1, steps=[3, 3, 3], alg=a,
+                  bins=256, frac=f)
+             for a, f in ((2, 0.5), (2, 0.9), (1, 0.97), (2, 0.985), (1, 0.995))]
+    b = wv.Batch(cubes)
+    b.submit()
+    b.wait()
+    for i, c in enumerate(cubes):
+        _check_item(b, i, v, c)
+        r, _ = b.result(i)
+        assert r.n_candidates >= 0, i
+    b.close()

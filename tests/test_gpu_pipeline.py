@@ -220,16 +220,49 @@ def test_int16_input_equals_widened_int32(wv, wavelet):
     assert f16 == f32 and np.array_equal(b16, b32)
 
 
+@pytest.mark.parametrize("zero_frac", [0.6, 0.95])
+def test_threshold_zero_ties_need_no_fallback(wv, zero_frac):
+    """Most keys exactly zero (and some -0.0): the zeros are counted, not
+    listed, so a rank inside them or just above them needs no exact
+    fallback, and the result is the oracle's."""
+    n = 1_000_003
+    rng = np.random.default_rng(21)
+    x = (rng.standard_normal(n) * 30).astype(np.float32)
+    z = rng.random(n) < zero_frac
+    x[z] = 0.0
+    x[z & (rng.random(n) < 0.3)] = -0.0
+    xd = torch.from_numpy(x).cuda()
+    for frac in (0.3, zero_frac - 0.01, zero_frac + 0.005, 0.99):
+        _, r = wv.threshold(xd, frac)
+        t = O.thresh(x, frac)
+        assert r.threshold == t["threshold"] and r.thresh_idx == t["idx"], frac
+        assert r.nonzero_count == t["nonzero"], frac
+        assert r.n_candidates >= 0, frac      # the sampled window held the rank
+
+
+def _sample_positions(n, m=16384):
+    """The select's sample: one key per stratum [i n/m, (i+1) n/m), at the
+    offset a 32-bit mixer (lowbias32 constants) gives for i."""
+    M = np.uint64(0xffffffff)
+    i = np.arange(m, dtype=np.uint64)
+    h = i.copy()
+    h ^= h >> np.uint64(16)
+    h = (h * np.uint64(0x7feb352d)) & M
+    h ^= h >> np.uint64(15)
+    h = (h * np.uint64(0x846ca68b)) & M
+    h ^= h >> np.uint64(16)
+    lo = i * np.uint64(n) // np.uint64(m)
+    hi = (i + np.uint64(1)) * np.uint64(n) // np.uint64(m)
+    return (lo + h % (hi - lo)).astype(np.int64)
+
+
 def test_threshold_exact_fallback_when_sample_misleads(wv):
     """Values at the sampled positions lie far below the rest, so the
     sampled window misses the rank and the exact full-histogram path runs."""
     n = 1 << 20
     rng = np.random.default_rng(12)
     x = (rng.random(n).astype(np.float32) * 1000 + 100).astype(np.float32)
-    nch, chunk = 1024, 16
-    starts = np.arange(nch, dtype=np.int64) * (n - chunk) // (nch - 1)
-    for s0 in starts:
-        x[s0:s0 + chunk] = rng.random(chunk).astype(np.float32) * 0.01
+    x[_sample_positions(n)] = rng.random(16384).astype(np.float32) * 0.01
     xd = torch.from_numpy(x).cuda()
     for frac in (0.3, 0.9):
         _, r = wv.threshold(xd, frac)

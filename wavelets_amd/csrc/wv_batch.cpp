@@ -17,10 +17,15 @@
 //              one flattened fused quantize + histogram.
 //
 // The transform + select sequence and the quantize sequence are captured as
-// two hipGraphs on the first submit and replayed afterwards (the kernels
-// read their work from device tables, so replays need no re-capture).
-// Between them the host computes the log quantizer's bin-edge tables with
-// its own logf, exactly as for a single cubelet, on a pool of threads.
+// hipGraphs on the first submit and replayed afterwards (the kernels read
+// their work from device tables, so replays need no re-capture).  Between
+// them the host computes the log quantizer's bin-edge tables with its own
+// logf, exactly as for a single cubelet, on a pool of threads.  To hide that
+// host stage, a batch with log jobs is cut into two parts of about equal
+// work (whole groups): the device runs transform + select of part 0, then of
+// part 1, then the quantize of part 0 and of part 1, while the host builds
+// part 0's tables during part 1's transform and part 1's during part 0's
+// quantize.
 //
 // An autotuner (WV_BATCH_AUTOTUNE) times tile-width caps, elements per
 // thread of x-line passes, and resident blocks per CU for every group on a
@@ -34,6 +39,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <condition_variable>
+#include <functional>
 #include <map>
 #include <mutex>
 #include <thread>
@@ -69,6 +76,67 @@ static GroupKey key_of(const wv_geom &g, int dtype) {
                   g.steps[0], g.steps[1], g.steps[2], dtype, g.wavelet, 0);
 }
 
+// A fixed set of host threads for the edge tables (threads are started
+// once per batch, not per submit).  run(n, f) calls f(0 .. n-1) on them and
+// returns when all calls have returned.
+class HostPool {
+ public:
+  explicit HostPool(unsigned n) {
+    for (unsigned t = 0; t < n; t++) th_.emplace_back([this] { loop(); });
+  }
+  ~HostPool() {
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    for (auto &t : th_) t.join();
+  }
+  void run(int n, const std::function<void(int)> &f) {
+    if (th_.empty()) {
+      for (int i = 0; i < n; i++) f(i);
+      return;
+    }
+    std::unique_lock<std::mutex> lk(mu_);
+    fn_ = &f;
+    next_ = 0;
+    total_ = n;
+    busy_ = 0;
+    gen_++;
+    cv_.notify_all();
+    done_.wait(lk, [&] { return next_ >= total_ && busy_ == 0; });
+    fn_ = nullptr;
+  }
+
+ private:
+  void loop() {
+    unsigned seen = 0;
+    std::unique_lock<std::mutex> lk(mu_);
+    for (;;) {
+      cv_.wait(lk, [&] { return stop_ || (gen_ != seen && next_ < total_); });
+      if (stop_) return;
+      seen = gen_;
+      while (next_ < total_) {
+        const int i = next_++;
+        busy_++;
+        const std::function<void(int)> *f = fn_;
+        lk.unlock();
+        (*f)(i);
+        lk.lock();
+        busy_--;
+      }
+      if (busy_ == 0) done_.notify_all();
+    }
+  }
+  std::vector<std::thread> th_;
+  std::mutex mu_;
+  std::condition_variable cv_, done_;
+  const std::function<void(int)> *fn_ = nullptr;
+  int next_ = 0, total_ = 0, busy_ = 0;
+  unsigned gen_ = 0;
+  bool stop_ = false;
+};
+
 static std::mutex g_tune_mu;
 static std::map<GroupKey, Tuning> g_tune_cache;
 
@@ -99,12 +167,20 @@ struct wv_batch {
   wv_thresh_result *res_h = nullptr;
   QuantDev *qdev_h = nullptr;
   float *qtab_h = nullptr;
+  wv_thresh_result *res_h2 = nullptr;   // after the last transform part: final
   bool host_params = false;    // any log job
+  // parts: groups -> part, items -> slots of the job arrays (part 0 first)
+  int nparts = 1;
+  std::vector<int> group_part, slot, slot_item;
+  int part_begin[3] = {0, 0, 0};
+  int sel_total_p[2] = {0, 0}, q_total_p[2] = {0, 0};
+  HostPool *pool = nullptr;
   hipStream_t side[BATCH_STREAMS] = {};
   hipStream_t cap = nullptr;   // graphs are captured here (the caller's
                                // stream may be the null stream)
   hipEvent_t fork = nullptr, join[BATCH_STREAMS] = {};
-  hipGraphExec_t gA = nullptr, gB = nullptr;
+  hipEvent_t part_done[2] = {};
+  hipGraphExec_t gA[2] = {}, gB[2] = {};
   hipStream_t last_stream = nullptr;
 };
 
@@ -137,8 +213,9 @@ static int enqueue_group(wv_batch *b, Group &G, hipStream_t s, int ncub_limit) {
   return WV_OK;
 }
 
-// transform of every group (spread over the side streams) + select
-static int enqueue_A(wv_batch *b, hipStream_t s) {
+// transform of every group of a part (spread over the side streams) +
+// select of the part's cubelets
+static int enqueue_A(wv_batch *b, hipStream_t s, int part) {
   hipError_t e = hipEventRecord(b->fork, s);
   if (e != hipSuccess) { set_error("batch: %s", hipGetErrorString(e)); return WV_ERR_HIP; }
   for (int k = 0; k < BATCH_STREAMS; k++) {
@@ -146,8 +223,9 @@ static int enqueue_A(wv_batch *b, hipStream_t s) {
     if (e != hipSuccess) { set_error("batch: %s", hipGetErrorString(e)); return WV_ERR_HIP; }
   }
   // largest groups first, round-robin over the streams
-  std::vector<int> order(b->groups.size());
-  for (size_t i = 0; i < order.size(); i++) order[i] = (int)i;
+  std::vector<int> order;
+  for (size_t i = 0; i < b->groups.size(); i++)
+    if (b->group_part[i] == part) order.push_back((int)i);
   std::sort(order.begin(), order.end(), [&](int x, int y) {
     const Group &A = b->groups[x], &B = b->groups[y];
     const int64_t ca = (int64_t)A.padded[0] * A.padded[1] * A.padded[2] * A.items.size();
@@ -163,20 +241,22 @@ static int enqueue_A(wv_batch *b, hipStream_t s) {
     if (e == hipSuccess) e = hipStreamWaitEvent(s, b->join[k], 0);
     if (e != hipSuccess) { set_error("batch: %s", hipGetErrorString(e)); return WV_ERR_HIP; }
   }
-  e = threshold_select_batch(b->sel_d, b->selblk_d, (int)b->items.size(), b->sel_total, s);
+  const int pb = b->part_begin[part], pe = b->part_begin[part + 1];
+  e = threshold_select_batch(b->sel_d + pb, b->selblk_d + pb, pe - pb, b->sel_total_p[part], s);
   if (e != hipSuccess) { set_error("batch select: %s", hipGetErrorString(e)); return WV_ERR_HIP; }
   return WV_OK;
 }
 
-static int enqueue_B(wv_batch *b, hipStream_t s) {
-  hipError_t e = quantize_hist_batch(b->q_d, b->qblk_d, (int)b->items.size(), b->q_total, s);
+static int enqueue_B(wv_batch *b, hipStream_t s, int part) {
+  const int pb = b->part_begin[part], pe = b->part_begin[part + 1];
+  hipError_t e = quantize_hist_batch(b->q_d + pb, b->qblk_d + pb, pe - pb, b->q_total_p[part], s);
   if (e != hipSuccess) { set_error("batch quantize: %s", hipGetErrorString(e)); return WV_ERR_HIP; }
   return WV_OK;
 }
 
-typedef int (*EnqueueFn)(wv_batch *, hipStream_t);
+typedef std::function<int(wv_batch *, hipStream_t)> EnqueueFn;
 
-static int capture(wv_batch *b, EnqueueFn fn, hipGraphExec_t *out) {
+static int capture(wv_batch *b, const EnqueueFn &fn, hipGraphExec_t *out) {
   hipStream_t s = b->cap;
   hipError_t e = hipStreamBeginCapture(s, hipStreamCaptureModeRelaxed);
   if (e != hipSuccess) { set_error("capture: %s", hipGetErrorString(e)); return WV_ERR_HIP; }
@@ -191,43 +271,30 @@ static int capture(wv_batch *b, EnqueueFn fn, hipGraphExec_t *out) {
   return WV_OK;
 }
 
-// Host parameters of the log jobs from the select results (res_h): edge
-// tables into qtab_h, parameter blocks into qdev_h.
-static int host_params(wv_batch *b) {
-  const int n = (int)b->items.size();
+// Host parameters of a part's log jobs from the select results res (by
+// item): edge tables into qtab_h, parameter blocks into qdev_h (by slot).
+static int host_params(wv_batch *b, int part, const wv_thresh_result *res) {
   std::vector<int> jobs;
-  for (int i = 0; i < n; i++)
-    if (b->items[i].quant_alg == WV_QUANT_LOG) jobs.push_back(i);
+  for (int k = b->part_begin[part]; k < b->part_begin[part + 1]; k++)
+    if (b->items[b->slot_item[k]].quant_alg == WV_QUANT_LOG) jobs.push_back(k);
   if (jobs.empty()) return WV_OK;
-  unsigned nt = std::thread::hardware_concurrency();
-  if (nt < 1) nt = 1;
-  if (nt > 16) nt = 16;
-  if (nt > jobs.size()) nt = (unsigned)jobs.size();
-  std::vector<int> rcs(nt, WV_OK);
-  auto work = [&](unsigned t) {
-    for (size_t k = t; k < jobs.size(); k += nt) {
-      const int i = jobs[k];
-      const wv_thresh_result &r = b->res_h[i];
-      wv_quant_params p;
-      // QuantLog with maxValue = max |x| (test_compress_cpu.cc:262-265)
-      int rc = wv_quant_params_init(&p, WV_QUANT_LOG, b->items[i].bin_count, r.threshold,
-                                    r.max_abs, nullptr, nullptr);
-      if (rc != WV_OK) { rcs[t] = rc; continue; }
-      float *tab = b->qtab_h + (size_t)i * b->qtab_stride;
-      std::memcpy(tab, p.pos_edges, sizeof(float) * p.n_pos_edges);
-      std::memcpy(tab + p.n_pos_edges, p.neg_edges, sizeof(float) * p.n_neg_edges);
-      p.table_d = b->qtab_d + (size_t)i * b->qtab_stride;
-      b->qdev_h[i] = quant_dev_of(&p);
-      wv_quant_params_free(&p);
-    }
-  };
-  if (nt == 1) {
-    work(0);
-  } else {
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < nt; t++) th.emplace_back(work, t);
-    for (auto &x : th) x.join();
-  }
+  std::vector<int> rcs(jobs.size(), WV_OK);
+  b->pool->run((int)jobs.size(), [&](int j) {
+    const int k = jobs[j];
+    const int i = b->slot_item[k];
+    const wv_thresh_result &r = res[i];
+    wv_quant_params p;
+    // QuantLog with maxValue = max |x| (test_compress_cpu.cc:262-265)
+    int rc = wv_quant_params_init(&p, WV_QUANT_LOG, b->items[i].bin_count, r.threshold,
+                                  r.max_abs, nullptr, nullptr);
+    if (rc != WV_OK) { rcs[j] = rc; return; }
+    float *tab = b->qtab_h + (size_t)k * b->qtab_stride;
+    std::memcpy(tab, p.pos_edges, sizeof(float) * p.n_pos_edges);
+    std::memcpy(tab + p.n_pos_edges, p.neg_edges, sizeof(float) * p.n_neg_edges);
+    p.table_d = b->qtab_d + (size_t)k * b->qtab_stride;
+    b->qdev_h[k] = quant_dev_of(&p);
+    wv_quant_params_free(&p);
+  });
   for (int rc : rcs)
     if (rc != WV_OK) return rc;
   return WV_OK;
@@ -286,8 +353,12 @@ static int autotune_group(wv_batch *b, Group &G, hipStream_t s) {
 
 static void free_batch(wv_batch *b) {
   if (!b) return;
-  if (b->gA) (void)hipGraphExecDestroy(b->gA);
-  if (b->gB) (void)hipGraphExecDestroy(b->gB);
+  for (int p = 0; p < 2; p++) {
+    if (b->gA[p]) (void)hipGraphExecDestroy(b->gA[p]);
+    if (b->gB[p]) (void)hipGraphExecDestroy(b->gB[p]);
+    if (b->part_done[p]) (void)hipEventDestroy(b->part_done[p]);
+  }
+  delete b->pool;
   for (int k = 0; k < BATCH_STREAMS; k++) {
     if (b->side[k]) (void)hipStreamDestroy(b->side[k]);
     if (b->join[k]) (void)hipEventDestroy(b->join[k]);
@@ -296,6 +367,7 @@ static void free_batch(wv_batch *b) {
   if (b->cap) (void)hipStreamDestroy(b->cap);
   if (b->arena) (void)hipFree(b->arena);
   if (b->res_h) (void)hipHostFree(b->res_h);
+  if (b->res_h2) (void)hipHostFree(b->res_h2);
   if (b->qdev_h) (void)hipHostFree(b->qdev_h);
   if (b->qtab_h) (void)hipHostFree(b->qtab_h);
   delete b;
@@ -364,6 +436,40 @@ int wv_batch_plan(const wv_batch_item *items, int n, int flags, wv_batch **out) 
     b->groups[f->second].items.push_back(i);
   }
 
+  // parts: with log jobs, two parts of about equal work (largest groups
+  // first, each to the lighter part); the job arrays list part 0's items
+  // first (slots)
+  const int ng = (int)b->groups.size();
+  b->nparts = b->host_params && ng >= 2 ? 2 : 1;
+  b->group_part.assign(ng, 0);
+  if (b->nparts == 2) {
+    std::vector<int> ord(ng);
+    for (int g = 0; g < ng; g++) ord[g] = g;
+    auto work = [&](int g) {
+      const Group &G = b->groups[g];
+      return (int64_t)G.padded[0] * G.padded[1] * G.padded[2] * (int64_t)G.items.size();
+    };
+    std::sort(ord.begin(), ord.end(), [&](int x, int y) { return work(x) > work(y); });
+    int64_t load[2] = {0, 0};
+    for (int g : ord) {
+      const int pt = load[0] <= load[1] ? 0 : 1;
+      b->group_part[g] = pt;
+      load[pt] += work(g);
+    }
+  }
+  b->slot.assign(n, 0);
+  b->slot_item.clear();
+  for (int pt = 0; pt < b->nparts; pt++) {
+    b->part_begin[pt] = (int)b->slot_item.size();
+    for (int g = 0; g < ng; g++)
+      if (b->group_part[g] == pt)
+        for (int i : b->groups[g].items) {
+          b->slot[i] = (int)b->slot_item.size();
+          b->slot_item.push_back(i);
+        }
+  }
+  b->part_begin[b->nparts] = n;
+
   // arena layout
   const size_t A = 256;
   auto up = [&](size_t x) { return (x + A - 1) & ~(A - 1); };
@@ -403,6 +509,7 @@ int wv_batch_plan(const wv_batch_item *items, int n, int flags, wv_batch **out) 
   b->qtab_d = (float *)(ar + o_qtab);
   b->res_d = (wv_thresh_result *)(ar + o_res);
   BCHK(hipHostMalloc((void **)&b->res_h, sizeof(wv_thresh_result) * n, 0), "pinned");
+  BCHK(hipHostMalloc((void **)&b->res_h2, sizeof(wv_thresh_result) * n, 0), "pinned");
   BCHK(hipHostMalloc((void **)&b->qdev_h, sizeof(QuantDev) * n, 0), "pinned");
   BCHK(hipHostMalloc((void **)&b->qtab_h, sizeof(float) * (size_t)b->qtab_stride * n, 0),
        "pinned");
@@ -415,9 +522,12 @@ int wv_batch_plan(const wv_batch_item *items, int n, int flags, wv_batch **out) 
   std::vector<int> qblk(n);
   int stot = 0, qtot = 0;
   const int64_t chunk = sel_collect_chunk();
-  for (int i = 0; i < n; i++) {
+  for (int k = 0; k < n; k++) {
+    const int i = b->slot_item[k];
+    for (int pt = 0; pt < b->nparts; pt++)
+      if (k == b->part_begin[pt]) stot = qtot = 0;   // block ranges restart per part
     const int64_t cnt = b->geoms[i].count;
-    BSelJob &s = sj[i];
+    BSelJob &s = sj[k];
     s.coef = b->items[i].coef_d;
     s.n = cnt;
     s.frac = b->items[i].thresh_frac;
@@ -429,24 +539,29 @@ int wv_batch_plan(const wv_batch_item *items, int n, int flags, wv_batch **out) 
     const int nch = ceil_div(cnt, chunk);
     s.nblk = std::max(1, std::min(sel_max_collect_blocks(), ceil_div(nch, 8)));
     s.blk0 = stot;
-    sblk[i] = stot;
+    sblk[k] = stot;
     stot += s.nblk;
-    BQuantJob &q = qj[i];
+    BQuantJob &q = qj[k];
     q.coef = b->items[i].coef_d;
     q.n = cnt;
     q.bins = b->items[i].bins_d;
     q.counts = b->items[i].counts_d;
-    q.q = b->qdev_d + i;
+    q.q = b->qdev_d + k;
     q.res = b->res_d + i;
     q.alg = b->items[i].quant_alg;
     q.bin_count = b->items[i].bin_count;
     q.nblk = quant_blocks_for(cnt);
     q.blk0 = qtot;
-    qblk[i] = qtot;
+    qblk[k] = qtot;
     qtot += q.nblk;
+    for (int pt = 0; pt < b->nparts; pt++)
+      if (k + 1 == b->part_begin[pt + 1]) {
+        b->sel_total_p[pt] = stot;
+        b->q_total_p[pt] = qtot;
+      }
   }
-  b->sel_total = stot;
-  b->q_total = qtot;
+  b->sel_total = b->sel_total_p[0] + (b->nparts > 1 ? b->sel_total_p[1] : 0);
+  b->q_total = b->q_total_p[0] + (b->nparts > 1 ? b->q_total_p[1] : 0);
   BCHK(hipMemcpy(b->sel_d, sj.data(), sizeof(BSelJob) * n, hipMemcpyHostToDevice), "H2D");
   BCHK(hipMemcpy(b->selblk_d, sblk.data(), sizeof(int) * n, hipMemcpyHostToDevice), "H2D");
   BCHK(hipMemcpy(b->q_d, qj.data(), sizeof(BQuantJob) * n, hipMemcpyHostToDevice), "H2D");
@@ -481,6 +596,14 @@ int wv_batch_plan(const wv_batch_item *items, int n, int flags, wv_batch **out) 
     BCHK(hipEventCreateWithFlags(&b->join[k], hipEventDisableTiming), "event");
   }
   BCHK(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming), "event");
+  for (int pt = 0; pt < 2; pt++)
+    BCHK(hipEventCreateWithFlags(&b->part_done[pt], hipEventDisableTiming), "event");
+  if (b->host_params) {
+    unsigned nt = std::thread::hardware_concurrency();
+    if (nt < 1) nt = 1;
+    if (nt > 16) nt = 16;
+    b->pool = new HostPool(nt);
+  }
   BCHK(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking), "stream");
   if (flags & WV_BATCH_AUTOTUNE) {
     for (auto &G : b->groups) {
@@ -497,51 +620,61 @@ int wv_batch_submit(wv_batch *b, void *stream) {
   hipStream_t s = (hipStream_t)stream;
   b->last_stream = s;
   const bool graphs = !(b->flags & WV_BATCH_NO_GRAPH);
-  int rc;
-  if (graphs) {
-    if (!b->gA) {
+  const int n = (int)b->items.size();
+  auto run = [&](hipGraphExec_t *g, const EnqueueFn &fn) -> int {
+    if (!graphs) return fn(b, s);
+    if (!*g) {
       // one eager run first: kernel attributes are set outside any capture
-      rc = enqueue_A(b, s);
+      int rc = fn(b, s);
       if (rc != WV_OK) return rc;
       if (hipStreamSynchronize(s) != hipSuccess) { set_error("batch sync"); return WV_ERR_HIP; }
-      rc = capture(b, enqueue_A, &b->gA);
+      rc = capture(b, fn, g);
       if (rc != WV_OK) return rc;
     }
-    if (hipGraphLaunch(b->gA, s) != hipSuccess) { set_error("graph launch"); return WV_ERR_HIP; }
-  } else {
-    rc = enqueue_A(b, s);
+    if (hipGraphLaunch(*g, s) != hipSuccess) { set_error("graph launch"); return WV_ERR_HIP; }
+    return WV_OK;
+  };
+  int rc;
+  // transform + select of every part; with log jobs, each part's results
+  // come back (the last copy holds every cubelet's final result)
+  for (int pt = 0; pt < b->nparts; pt++) {
+    rc = run(&b->gA[pt], [pt](wv_batch *bb, hipStream_t ss) { return enqueue_A(bb, ss, pt); });
     if (rc != WV_OK) return rc;
-  }
-  const int n = (int)b->items.size();
-  if (b->host_params) {
-    if (hipMemcpyAsync(b->res_h, b->res_d, sizeof(wv_thresh_result) * n,
-                       hipMemcpyDeviceToHost, s) != hipSuccess ||
-        hipStreamSynchronize(s) != hipSuccess) {
-      set_error("batch: threshold results D2H");
-      return WV_ERR_HIP;
-    }
-    rc = host_params(b);
-    if (rc != WV_OK) return rc;
-    if (hipMemcpyAsync(b->qtab_d, b->qtab_h, sizeof(float) * (size_t)b->qtab_stride * n,
-                       hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(b->qdev_d, b->qdev_h, sizeof(QuantDev) * n, hipMemcpyHostToDevice,
-                       s) != hipSuccess) {
-      set_error("batch: parameter upload");
-      return WV_ERR_HIP;
+    if (b->host_params) {
+      wv_thresh_result *dst = pt == b->nparts - 1 ? b->res_h2 : b->res_h;
+      if (hipMemcpyAsync(dst, b->res_d, sizeof(wv_thresh_result) * n, hipMemcpyDeviceToHost,
+                         s) != hipSuccess ||
+          hipEventRecord(b->part_done[pt], s) != hipSuccess) {
+        set_error("batch: threshold results D2H");
+        return WV_ERR_HIP;
+      }
     }
   }
-  if (graphs) {
-    if (!b->gB) {
-      rc = capture(b, enqueue_B, &b->gB);
+  // quantize of every part, each after the host built its log tables
+  for (int pt = 0; pt < b->nparts; pt++) {
+    if (b->host_params) {
+      if (hipEventSynchronize(b->part_done[pt]) != hipSuccess) {
+        set_error("batch: threshold results");
+        return WV_ERR_HIP;
+      }
+      rc = host_params(b, pt, pt == b->nparts - 1 ? b->res_h2 : b->res_h);
       if (rc != WV_OK) return rc;
+      const int pb = b->part_begin[pt], pe = b->part_begin[pt + 1];
+      if (hipMemcpyAsync(b->qtab_d + (size_t)pb * b->qtab_stride,
+                         b->qtab_h + (size_t)pb * b->qtab_stride,
+                         sizeof(float) * (size_t)b->qtab_stride * (pe - pb), hipMemcpyHostToDevice,
+                         s) != hipSuccess ||
+          hipMemcpyAsync(b->qdev_d + pb, b->qdev_h + pb, sizeof(QuantDev) * (pe - pb),
+                         hipMemcpyHostToDevice, s) != hipSuccess) {
+        set_error("batch: parameter upload");
+        return WV_ERR_HIP;
+      }
     }
-    if (hipGraphLaunch(b->gB, s) != hipSuccess) { set_error("graph launch"); return WV_ERR_HIP; }
-  } else {
-    rc = enqueue_B(b, s);
+    rc = run(&b->gB[pt], [pt](wv_batch *bb, hipStream_t ss) { return enqueue_B(bb, ss, pt); });
     if (rc != WV_OK) return rc;
   }
   if (!b->host_params)
-    (void)hipMemcpyAsync(b->res_h, b->res_d, sizeof(wv_thresh_result) * n,
+    (void)hipMemcpyAsync(b->res_h2, b->res_d, sizeof(wv_thresh_result) * n,
                          hipMemcpyDeviceToHost, s);
   return WV_OK;
 }
@@ -582,7 +715,7 @@ int wv_batch_group_tuning(const wv_batch *b, int g, int *tile_cap, int *ept0, in
 
 int wv_batch_result(const wv_batch *b, int i, wv_thresh_result *res, int *zero_bin) {
   if (!b || i < 0 || i >= (int)b->items.size() || !res) { set_error("bad arguments"); return WV_ERR_ARG; }
-  *res = b->res_h[i];
+  *res = b->res_h2[i];
   if (zero_bin) *zero_bin = (b->items[i].bin_count - 1) >> 1;   // quant(0) for uniform / log
   return WV_OK;
 }

@@ -413,7 +413,13 @@ bquant_hist(const BQuantJob *J, const int *blk0, int njobs) {
     quant_hist_body<WV_QUANT_LLOYD>(j.coef, j.n, q, j.bins, j.counts, bidx, j.nblk, tab, h);
 }
 
-int quant_blocks_for(int64_t n) { return (int)grid_for(n); }
+// blocks of a batch job: 64 K samples each (a block's fixed costs -- table
+// load, histogram clear and flush -- over more samples; one launch covers
+// every job, so the grid stays large)
+int quant_blocks_for(int64_t n) {
+  const int64_t b = (n + 65535) / 65536;
+  return (int)(b < 1 ? 1 : (b > 2048 ? 2048 : b));
+}
 
 hipError_t quantize_hist_batch(const BQuantJob *jobs_d, const int *blk0_d, int njobs,
                                int total_blocks, hipStream_t s) {
