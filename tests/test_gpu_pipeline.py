@@ -140,6 +140,38 @@ def test_quantize_hist_matches_oracle(wv, alg, bins):
     assert np.array_equal(dq.view(np.uint32), O.dequantize(oq, want).view(np.uint32))
 
 
+@pytest.mark.parametrize("bins", [7, 256, 1023, 4000])
+def test_log_quantize_at_bin_edges(wv, bins):
+    """Values on and one to three floats either side of every log bin edge
+    (and of the threshold and max): the slot-table count (up to ~4000 bins)
+    and the logarithm-guess count (4000 bins leave no LDS room for the slot
+    tables) both give the oracle's QuantLog bins."""
+    thr, mx = np.float32(0.37), np.float32(2345.6)
+    oq = O.quantizer(2, bins, thr, mx)
+    xs = []
+    for scale, off in ((oq.pos_scale, oq.pos_offset), (oq.neg_scale, oq.neg_offset)):
+        k = np.arange(-2, bins // 2 + 3, dtype=np.float64)
+        e = np.exp((k - off) / scale).astype(np.float32)
+        xs.append(e)
+    e = np.concatenate(xs + [np.array([thr, mx, mx * 2, 1e-30, 3e38], np.float32)])
+    e = e[np.isfinite(e)]
+    near = [e]
+    up, dn = e.copy(), e.copy()
+    for _ in range(3):
+        up = np.nextafter(up, np.float32(np.inf))
+        dn = np.nextafter(dn, np.float32(0))
+        near += [up, dn]
+    pos = np.concatenate(near).astype(np.float32)
+    rng = np.random.default_rng(bins)
+    x = np.concatenate([pos, -pos, np.float32([0.0, -0.0]),
+                        (rng.standard_normal(20000) * 300).astype(np.float32)])
+    q = wv.Quantizer(2, bins, thr, mx)
+    b, counts = q.quantize(torch.from_numpy(x).cuda())
+    want = O.quantize(oq, x)
+    assert np.array_equal(b.cpu().numpy(), want)
+    assert np.array_equal(counts.cpu().numpy()[:bins], np.bincount(want, minlength=bins)[:bins])
+
+
 def test_quantize_uniform_device_params(wv):
     vol = _vol((16, 32, 32), np.float32, 6)
     coef, g, stats = wv.dwt3d_forward(torch.from_numpy(vol).cuda(), [3, 3, 3], 1)

@@ -10,11 +10,17 @@
 //            contraction off, so bins equal the host's;
 //  log     : the bin is the number of host-tabulated bin edges (the
 //            smallest float of every bin under the host logf) at or below
-//            |x|, found in LDS from a device-logarithm guess; this reproduces
-//            QuantLog::quant (quant.h:374-406) exactly, whatever the device
-//            logarithm's rounding;
+//            |x|; this reproduces QuantLog::quant (quant.h:374-406) exactly,
+//            whatever the device logarithm's rounding.  The count starts
+//            from a per-block LDS table indexed by the top bits of |x|'s
+//            float pattern (the edge count at the start of each slot of
+//            keys, up to 1024 slots per sign between the threshold and the
+//            last edge) and steps up over the edges inside the slot -- no
+//            logarithm, usually one or two LDS reads; when the edges leave
+//            no room for the slot table, from a device-logarithm guess;
 //  codebook: lower_bound over the boundaries plus the lastBoundary rule of
 //            the host QuantCodebook::quant (quant.h:515-526).
+#include <type_traits>
 #include "wv_device.h"
 #include "wv_internal.h"
 
@@ -45,6 +51,74 @@ __device__ inline int lower_bound_f(const float *e, int m, float x) {
   return lo;
 }
 
+// The slot table of one sign of the log quantizer: slot s covers the float
+// keys [(base + s) << sh, (base + s + 1) << sh) and holds the number of
+// edges <= the slot's first key; keys >= k1 (the last edge's) count all m.
+// The fields are block-uniform (scalar registers).
+struct LogSide {
+  uint32_t k1, base, sh;
+};
+#define LOG_LUT_MAX 1024
+
+__device__ inline int log_count(const float *e, int m, const int *lut, const LogSide &L,
+                                float a) {
+  const uint32_t key = __builtin_bit_cast(uint32_t, a);
+  // NaN counts 0 edges, like the logarithm-guess path
+  if (key >= L.k1) return key > 0x7f800000u ? 0 : m;
+  int c = lut[(key >> L.sh) - L.base];
+  while (c < m && e[c] <= a) c++;
+  return c;
+}
+
+// log quantizer through the slot tables lut (positive side) and lut + per:
+// one count for either sign (the sign selects the tables), so a wave with
+// both signs runs the count once, not once per side
+__device__ inline int quant_log_lut(const QuantDev &q, const float *tab, const int *lut,
+                                   int per, const LogSide &P, const LogSide &N, float x) {
+  const int zb = q.zero_bin;
+  const bool neg = x < 0;
+  const float a = fabsf(x);
+  if (a <= q.threshold) return zb;
+  if (!neg && x >= q.pos_clamp) return q.bin_count - 1;
+  LogSide L;
+  L.k1 = neg ? N.k1 : P.k1;
+  L.base = neg ? N.base : P.base;
+  L.sh = neg ? N.sh : P.sh;
+  const int c = log_count(neg ? tab + q.n_pos : tab, neg ? q.n_neg : q.n_pos,
+                          neg ? lut + per : lut, L, a);
+  return neg ? zb - c : zb + c;
+}
+
+// Builds the slot table of one side (edges e[0..m), per slots at most) into
+// l; every thread of the block takes part and gets the same parameters.
+__device__ inline LogSide build_log_side(float threshold, const float *e, int m, int *l,
+                                         int per) {
+  const uint32_t k0 = __builtin_bit_cast(uint32_t, threshold) & 0x7fffffffu;
+  const uint32_t k1 = (uint32_t)__builtin_amdgcn_readfirstlane(
+      m ? (int)__builtin_bit_cast(uint32_t, e[m - 1]) : 0);
+  uint32_t sh = 0, nslot = 0;
+  if (k1 > k0) {
+    while (((k1 >> sh) - (k0 >> sh) + 1u) > (uint32_t)per) sh++;
+    nslot = (k1 >> sh) - (k0 >> sh) + 1u;
+  }
+  const uint32_t base = k0 >> sh;
+  for (uint32_t sl = threadIdx.x; sl < nslot; sl += WV_BLOCK) {
+    const float sf = __builtin_bit_cast(float, (base + sl) << sh);
+    int lo = 0, len = m;   // edges <= sf (upper bound)
+    while (len > 0) {
+      const int half = len >> 1;
+      if (e[lo + half] <= sf) { lo += half + 1; len -= half + 1; }
+      else len = half;
+    }
+    l[sl] = lo;
+  }
+  LogSide L;
+  L.k1 = k1;
+  L.base = base;
+  L.sh = sh;
+  return L;
+}
+
 template <int ALG>
 __device__ inline int quant_one(const QuantDev &q, const float *tab, float x) {
   const int zb = q.zero_bin;
@@ -64,18 +138,18 @@ __device__ inline int quant_one(const QuantDev &q, const float *tab, float x) {
     // The bin is the number of host-tabulated edges <= |x| (each edge is the
     // smallest float of its bin under the host logf); the device logarithm
     // only supplies the starting guess, scale*ln|x| + offset, of that count.
+    // One count for either sign (the sign selects table and guess), so a
+    // wave holding both signs runs the search once.
     constexpr float LN2 = 0.69314718055994531f;
-    if (x < 0) {
-      const float a = -x;
-      if (a <= q.threshold) return zb;
-      const int g = (int)(q.neg_scale * (__log2f(a) * LN2) + q.neg_offset);
-      return zb - count_le_from(tab + q.n_pos, q.n_neg, a, g);
-    } else {
-      if (x <= q.threshold) return zb;
-      if (x >= q.pos_clamp) return q.bin_count - 1;
-      const int g = (int)(q.pos_scale * (__log2f(x) * LN2) + q.pos_offset + 0.5f);
-      return zb + count_le_from(tab, q.n_pos, x, g);
-    }
+    const bool neg = x < 0;
+    const float a = fabsf(x);
+    if (a <= q.threshold) return zb;
+    if (!neg && x >= q.pos_clamp) return q.bin_count - 1;
+    const float sc = neg ? q.neg_scale : q.pos_scale;
+    const float off = neg ? q.neg_offset : q.pos_offset + 0.5f;
+    const int g = (int)(sc * (__log2f(a) * LN2) + off);
+    const int c = count_le_from(neg ? tab + q.n_pos : tab, neg ? q.n_neg : q.n_pos, a, g);
+    return neg ? zb - c : zb + c;
   } else {
     const int m = q.n_pos;   // boundaries
     if (x >= tab[m - 1]) return m;
@@ -89,7 +163,7 @@ __device__ inline int quant_one(const QuantDev &q, const float *tab, float x) {
 // (declared once per kernel, so instantiations for several quantizers in
 // one kernel share them).
 template <int ALG>
-__device__ void quant_hist_body(const float *coef, int64_t n, const QuantDev &q,
+__device__ __forceinline__ void quant_hist_body(const float *coef, int64_t n, const QuantDev &q,
                                 int32_t *bins, uint32_t *__restrict__ counts,
                                 int bidx, int nblk, float *tab, uint32_t *h) {
   const int ntab = q.n_pos + (ALG == WV_QUANT_LOG ? q.n_neg : 0);
@@ -101,7 +175,16 @@ __device__ void quant_hist_body(const float *coef, int64_t n, const QuantDev &q,
     for (int i = threadIdx.x; i < q.bin_count; i += WV_BLOCK) h[i] = 0;
   __syncthreads();
   const float *t = tab_in_lds ? tab : q.table;
-#define QUANT_ONE(x) quant_one<ALG>(q, t, (x))
+  // log: slot tables after the edges when they leave room
+  LogSide LP{}, LN{};
+  const int per = tab_in_lds ? min((WV_LDS_TABLE - ntab) / 2, LOG_LUT_MAX) : 0;
+  const bool use_lut = ALG == WV_QUANT_LOG && per >= 64;
+  int *lut = (int *)(tab + ntab);
+  if (use_lut) {
+    LP = build_log_side(q.threshold, tab, q.n_pos, lut, per);
+    LN = build_log_side(q.threshold, tab + q.n_pos, q.n_neg, lut + per, per);
+    __syncthreads();
+  }
   unsigned zcount = 0;
   const int zb = q.zero_bin;
 
@@ -114,45 +197,62 @@ __device__ void quant_hist_body(const float *coef, int64_t n, const QuantDev &q,
     else atomicAdd(&counts[b], 1u);
   };
 
-  const bool aligned = (((uintptr_t)coef | (uintptr_t)bins) & 15) == 0;
-  const int64_t nvec = aligned ? (n >> 2) : 0;
-  const float4 *c4 = (const float4 *)coef;
-  int4 *b4 = (int4 *)bins;
-  // four independent 16-byte loads per thread in flight per iteration
-  const int64_t stride = (int64_t)nblk * WV_BLOCK;
-  int64_t i0 = (int64_t)bidx * WV_BLOCK + threadIdx.x;
-  for (; i0 + 3 * stride < nvec; i0 += 4 * stride) {
-    float4 vv[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) vv[u] = c4[i0 + u * stride];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const float4 v = vv[u];
+  // the pass, with the quantizer qf (one loop body per quantizer form).
+  // ROLL keeps four copies of qf in the code, not sixteen (the loop over
+  // the four loaded vectors is rolled, registers shifted through, no
+  // indexed array): the log quantizers have loops of their own.
+  auto pass = [&](auto qf, auto roll) {
+    constexpr bool ROLL = decltype(roll)::value;
+    const bool aligned = (((uintptr_t)coef | (uintptr_t)bins) & 15) == 0;
+    const int64_t nvec = aligned ? (n >> 2) : 0;
+    const float4 *c4 = (const float4 *)coef;
+    int4 *b4 = (int4 *)bins;
+    auto one4 = [&](float4 v, int64_t i) {
       int4 r;
-      r.x = QUANT_ONE(v.x);
-      r.y = QUANT_ONE(v.y);
-      r.z = QUANT_ONE(v.z);
-      r.w = QUANT_ONE(v.w);
-      b4[i0 + u * stride] = r;
+      r.x = qf(v.x);
+      r.y = qf(v.y);
+      r.z = qf(v.z);
+      r.w = qf(v.w);
       count(r.x); count(r.y); count(r.z); count(r.w);
+      b4[i] = r;
+    };
+    // four independent 16-byte loads per thread in flight per iteration
+    const int64_t stride = (int64_t)nblk * WV_BLOCK;
+    int64_t i0 = (int64_t)bidx * WV_BLOCK + threadIdx.x;
+    for (; i0 + 3 * stride < nvec; i0 += 4 * stride) {
+      float4 v0 = c4[i0], v1 = c4[i0 + stride], v2 = c4[i0 + 2 * stride],
+             v3 = c4[i0 + 3 * stride];
+      if constexpr (ROLL) {
+#pragma unroll 1
+        for (int u = 0; u < 4; u++) {
+          one4(v0, i0 + u * stride);
+          v0 = v1; v1 = v2; v2 = v3;
+        }
+      } else {
+        one4(v0, i0);
+        one4(v1, i0 + stride);
+        one4(v2, i0 + 2 * stride);
+        one4(v3, i0 + 3 * stride);
+      }
     }
-  }
-  for (int64_t i = i0; i < nvec; i += stride) {
-    const float4 v = c4[i];
-    int4 r;
-    r.x = quant_one<ALG>(q, t, v.x);
-    r.y = quant_one<ALG>(q, t, v.y);
-    r.z = quant_one<ALG>(q, t, v.z);
-    r.w = quant_one<ALG>(q, t, v.w);
-    b4[i] = r;
-    count(r.x); count(r.y); count(r.z); count(r.w);
-  }
-  // tail
-  for (int64_t i = nvec * 4 + (int64_t)bidx * WV_BLOCK + threadIdx.x; i < n;
-       i += (int64_t)nblk * WV_BLOCK) {
-    const int r = quant_one<ALG>(q, t, coef[i]);
-    bins[i] = r;
-    count(r);
+    for (int64_t i = i0; i < nvec; i += stride) one4(c4[i], i);
+    // tail
+    for (int64_t i = nvec * 4 + (int64_t)bidx * WV_BLOCK + threadIdx.x; i < n;
+         i += (int64_t)nblk * WV_BLOCK) {
+      const int r = qf(coef[i]);
+      bins[i] = r;
+      count(r);
+    }
+  };
+  using Unrolled = std::integral_constant<bool, false>;
+  using Rolled = std::integral_constant<bool, true>;
+  if constexpr (ALG == WV_QUANT_LOG) {
+    if (use_lut)
+      pass([&](float x) { return quant_log_lut(q, tab, lut, per, LP, LN, x); }, Rolled());
+    else
+      pass([&](float x) { return quant_one<ALG>(q, t, x); }, Rolled());
+  } else {
+    pass([&](float x) { return quant_one<ALG>(q, t, x); }, Unrolled());
   }
   for (int off = 32; off > 0; off >>= 1)
     zcount += (unsigned)__shfl_xor((int)zcount, off);
@@ -231,7 +331,6 @@ quant_hist_uniform_dev_kernel(const float *coef, int64_t n,
                               uint32_t *__restrict__ counts) {
   __shared__ uint32_t h[WV_LDS_HIST];
   const QuantDev q = *qp;
-#undef QUANT_ONE
 #define QUANT_ONE(x) quant_one<WV_QUANT_UNIFORM>(q, nullptr, (x))
   const bool hist_in_lds = q.bin_count <= WV_LDS_HIST;
   if (hist_in_lds)
