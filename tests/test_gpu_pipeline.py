@@ -56,6 +56,10 @@ def _oracle_coef(vol, wavelet, steps):
 
 CASES = [
     ((16, 16, 16), np.float32, [3, 3, 3]),
+    ((16, 64, 512), np.uint8, [3, 3, 3]),     # thin: merged rows in the z pass
+    ((32, 32, 32), np.float32, [3, 3, 3]),    # short lines, 8 elements per thread
+    ((64, 64, 64), np.float32, [3, 3, 3]),
+    ((12, 48, 80), np.int32, [3, 3, 2]),      # padded, thin
     ((8, 32, 64), np.uint8, [-1, -1, -1]),
     ((5, 24, 40), np.int32, [3, 2, 1]),
     ((1, 96, 128), np.uint8, [-1, -1, 0]),
@@ -83,7 +87,7 @@ def test_dwt_forward_bit_exact(wv, wavelet, shape, dtype, steps):
 
 
 @pytest.mark.parametrize("wavelet", [1, 2])
-@pytest.mark.parametrize("shape,dtype,steps", CASES[:7])
+@pytest.mark.parametrize("shape,dtype,steps", CASES[:11])
 def test_dwt_inverse_bit_exact(wv, wavelet, shape, dtype, steps):
     vol = _vol(shape, dtype, 2)
     want_c, size, st = _oracle_coef(vol, wavelet, steps)

@@ -18,6 +18,10 @@
 #ifndef WV_DWAVES16
 #define WV_DWAVES16 6
 #endif
+// 8 elements per thread (short lines, few lines): 8 waves per EU, 64 VGPRs
+#ifndef WV_DWAVES8
+#define WV_DWAVES8 8
+#endif
 #define WV_H1_BITS 11       // threshold select digit 1: bits 30..20 of |x|
 #define WV_H1_BINS 2048
 #define WV_H1_SHIFT 20

@@ -592,7 +592,8 @@ pipe_step(const DwtPass &P, float *tile, const float *tout, const Tile &t, const
 // previous tile's stores behind the next tile's loads); waits on the loads
 // then never cover those stores.
 template <int WAVE, bool INV, typename SRC, bool CLAMP, int TT, int EPT, int NT>
-__global__ void __launch_bounds__(NT, EPT == 16 ? WV_DWAVES16 : WV_DWAVES)
+__global__ void __launch_bounds__(NT, EPT == 8 ? WV_DWAVES8
+                                              : (EPT == 16 && TT > 0 ? WV_DWAVES16 : WV_DWAVES))
 dwt_pass_kernel(DwtPass P, int ntiles) {
   constexpr int GAP = (WAVE == WV_WAVELET_CDF97 && !INV) ? WV_HALO : 0;
   extern __shared__ float smem[];
@@ -713,7 +714,18 @@ static hipError_t launch_tile(const DwtPass &P, int ntiles, hipStream_t s) {
       default: break;
     }
   }
-  // generic mapping (holds any tile with L*T <= 512*32)
+  if (P.L % 64 == 0 && P.nt == 512 && P.ept == 8) {
+    switch (P.T) {
+      case 64: return launch_one<WAVE, INV, SRC, CLAMP, 64, 8, 512>(P, ntiles, s);
+      case 32: return launch_one<WAVE, INV, SRC, CLAMP, 32, 8, 512>(P, ntiles, s);
+      case 16: return launch_one<WAVE, INV, SRC, CLAMP, 16, 8, 512>(P, ntiles, s);
+      case 8: return launch_one<WAVE, INV, SRC, CLAMP, 8, 8, 512>(P, ntiles, s);
+      default: break;
+    }
+  }
+  // generic mapping (holds any tile with L*T <= 512*EPT)
+  if (P.ept == 8) return launch_one<WAVE, INV, SRC, CLAMP, 0, 8, 512>(P, ntiles, s);
+  if (P.ept == 16) return launch_one<WAVE, INV, SRC, CLAMP, 0, 16, 512>(P, ntiles, s);
   return launch_one<WAVE, INV, SRC, CLAMP, 0, 32, 512>(P, ntiles, s);
 }
 
@@ -841,8 +853,10 @@ int plan_pass(DwtPass &P, const PassSpec &s, bool clamp) {
   P.ld = {si[u], si[v], si[a], cap[u] - 1, cap[v] - 1, cap[a] - 1};
   P.st = {so[u], so[v], so[a], dims[u] - 1, dims[v] - 1, dims[a] - 1};
   const bool unpadded = !clamp || (P.ox == P.X && P.oy == P.Y && P.oz == P.Z);
-  if (P.ld_line && P.st_line && unpadded && P.ld.su * P.Nu == P.ld.sv &&
-      P.st.su * P.Nu == P.st.sv) {
+  // rows that continue each other on both sides (line (u, v) at
+  // (u + v * Nu) * su) form one run of lines -- for line-contiguous sides
+  // and for position-contiguous ones (su == 1, whole rows of the fast axis)
+  if (unpadded && P.ld.su * P.Nu == P.ld.sv && P.st.su * P.Nu == P.st.sv) {
     P.Nu *= P.Nv;
     P.Nv = 1;
     P.ld.ucap = P.st.ucap = P.Nu - 1;
@@ -858,6 +872,10 @@ int plan_pass(DwtPass &P, const PassSpec &s, bool clamp) {
   while (T & (T - 1)) T &= T - 1;
   while (T > 1 && (long)P.L * T > (long)P.nt * P.ept) T >>= 1;
   while (T > 1 && T / 2 >= P.Nu) T >>= 1;
+  // the fewest elements per thread (32, 16 or 8) that hold the tile: short
+  // lines or few lines leave most of a 32-element tile empty, and smaller
+  // tiles keep more blocks resident
+  while (P.ept > 8 && (long)P.L * T * 2 <= (long)P.nt * P.ept) P.ept >>= 1;
   P.T = T;
   P.log2T = ilog2(T);
   P.log2dblock_T = ilog2(P.nt / T);
