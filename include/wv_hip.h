@@ -419,6 +419,10 @@ typedef struct {
   float *coef_d;           /* out: padded count coefficients (z,x,y layout) */
   int32_t *bins_d;         /* out: padded count bins; also the intermediate */
   uint32_t *counts_d;      /* out: bin_count + 1 counts */
+  /* samples between rows (y) and slices (z) of in_d; {0, 0}: dense
+     (orig[0], orig[0] * orig[1]).  A box inside a larger volume is read in
+     place by the first transform pass (no extraction copy). */
+  int64_t in_stride[2];
 } wv_batch_item;
 
 #define WV_BATCH_AUTOTUNE 1   /* time tile / occupancy candidates per group */
@@ -486,6 +490,67 @@ int wv_serve_poll(wv_server *s, wv_serve_result *out, int max);
 int wv_serve_drain(wv_server *s);   /* until every submitted request completed */
 int wv_serve_stats(wv_server *s, int64_t *peak_bytes, int *peak_inflight, double *now_s);
 int wv_serve_destroy(wv_server *s);  /* drains, then stops the threads */
+
+/* ------------------------------------------------------------------ *
+ * Cube split with overlap (cube split, cube_split_typed
+ * cubelet_convert.cc:368-553): the cubelets of one large volume of
+ * size[0] x size[1] x size[2] at cubelet size cub with `overlap` samples
+ * shared wherever neighbours meet.  x and y: ceil-style counts
+ * (size - overlap - 1) / (cub - overlap) + 1 at offsets k * (cub - overlap),
+ * trimmed at the volume edge; z: the reference's frame loop -- a chunk of
+ * cub[2] frames, then back up `overlap` frames, the frames left at the end
+ * one shorter chunk.  Boxes come in the reference's write order (z chunk,
+ * then y, then x).  Returns the number of boxes (those past cap are not
+ * written; boxes may be NULL to count), or a negative WV_ERR_*:
+ * 0 <= overlap < cub[a] on every axis.
+ * ------------------------------------------------------------------ */
+typedef struct {
+  int offset[3];   /* x, y, z of the box's first sample in the volume */
+  int size[3];
+} wv_split_box;
+int wv_split_plan(const int size[3], const int cub[3], int overlap, wv_split_box *boxes,
+                  int cap);
+
+/* ------------------------------------------------------------------ *
+ * Many cubelets in one .cube stream (CubeletStreamWriter::addCubelet per
+ * cubelet, close with the index footer: cubelet_file.cc:259-327).  The
+ * stream is kept in host memory until finished.
+ * ------------------------------------------------------------------ */
+typedef struct wv_cube_stream wv_cube_stream;
+wv_cube_stream *wv_cube_stream_open(void);
+/* a compressed cubelet: d and book as for wv_write_cube_payload, the
+   payload's nwords words on the host, the box offset in the volume */
+int wv_cube_stream_add(wv_cube_stream *w, const wv_cube_desc *d, const int offset[3],
+                       const wv_huff *book, const uint32_t *words, int64_t nwords);
+/* an uncompressed cubelet of size[0]*size[1]*size[2] samples (host data,
+   x fastest; WV_DTYPE_UINT8 / INT32 / FLOAT32) */
+int wv_cube_stream_add_raw(wv_cube_stream *w, const int size[3], int dtype,
+                           const void *data, const int offset[3]);
+/* bytes of the finished stream, and the stream written to file_buf */
+int64_t wv_cube_stream_bytes(const wv_cube_stream *w, int with_footer);
+int64_t wv_cube_stream_finish(wv_cube_stream *w, int with_footer, uint8_t *file_buf,
+                              int64_t cap);
+void wv_cube_stream_free(wv_cube_stream *w);
+
+/* ------------------------------------------------------------------ *
+ * Split + batched compression of one device-resident volume: the boxes of
+ * wv_split_plan are compressed by the batched path straight from the
+ * volume (no extraction copies), each coded on the device, and written as
+ * one .cube stream of compressed cubelets with their offsets.  The
+ * reference splits with `cube split` and then compresses one cubelet per
+ * run of test_compress_gpu.  Uniform or log quantizer (opt->quant_alg).
+ * Allocates its device buffers; synchronises.  Returns the file bytes
+ * (file_buf NULL or too small: WV_ERR_NOMEM with *need set).
+ * ------------------------------------------------------------------ */
+typedef struct {
+  int cubelets;
+  int64_t need;            /* bytes of the whole .cube stream */
+  double ms_plan, ms_batch, ms_encode, ms_total;
+} wv_split_info;
+int64_t wv_split_compress(const void *vol_d, int dtype, const int size[3], const int cub[3],
+                          int overlap, const wv_options *opt, int batch_flags,
+                          uint8_t *file_buf, int64_t cap, wv_split_info *info,
+                          void *stream);
 
 #ifdef __cplusplus
 }

@@ -213,4 +213,10 @@ double or_now(void);
 #ifdef __cplusplus
 }
 #endif
+/* cube split of one large cubelet (cube_split_typed cubelet_convert.cc:424-553):
+   boxes as 6 ints (offset x,y,z, size x,y,z) in write order; optional
+   packed copies of the boxes' samples */
+int or_split(const int size[3], const int cub[3], int overlap, int *boxes, int cap,
+             const void *vol, int esize, void *out);
+
 #endif

@@ -84,6 +84,7 @@ def _declare(L):
     L.or_count_leading_zeros.argtypes = [C.c_uint]
     L.or_padded_length.argtypes = [I, I]
     L.or_set_steps.argtypes = [P, P]
+    L.or_split.argtypes = [P, P, I, P, I, P, I, P]
     L.or_pad_convert.argtypes = [P, I, P, P, P]
     for f in ("or_cdf97_fwd", "or_cdf97_inv", "or_haar_fwd", "or_haar_inv"):
         getattr(L, f).argtypes = [I, P, I, P]
@@ -281,6 +282,31 @@ def huff_decode(table, dup_key, dup_value, words, count):
     n = lib().or_huff_decode(_p(t), len(t), dup_key, dup_value, _p(w), len(w),
                              _p(out), count)
     return out[:n]
+
+
+def split(size, cub, overlap, vol=None):
+    """cube split of one large cubelet: list of (offset xyz, size xyz) in
+    write order and, with vol ([z, y, x] numpy), the boxes' samples."""
+    sz = (I * 3)(*size)
+    cb = (I * 3)(*cub)
+    n = lib().or_split(sz, cb, overlap, None, 0, None, 0, None)
+    boxes = np.zeros(6 * max(n, 1), np.int32)
+    lib().or_split(sz, cb, overlap, _p(boxes), n, None, 0, None)
+    if vol is not None:
+        v = np.ascontiguousarray(vol)
+        total = int(sum(int(np.prod(boxes[6 * k + 3:6 * k + 6])) for k in range(n)))
+        packed = np.empty(max(total, 1), v.dtype)
+        lib().or_split(sz, cb, overlap, _p(boxes), n, _p(v), v.itemsize, _p(packed))
+    b = [(tuple(int(x) for x in boxes[6 * k:6 * k + 3]), tuple(int(x) for x in boxes[6 * k + 3:6 * k + 6]))
+         for k in range(n)]
+    if vol is None:
+        return b
+    out, pos = [], 0
+    for off, s3 in b:
+        m = s3[0] * s3[1] * s3[2]
+        out.append(packed[pos:pos + m].reshape(s3[2], s3[1], s3[0]))
+        pos += m
+    return b, out
 
 
 def compress(vol, wavelet, quant_alg, bins, steps, frac, compress_zeros=True,

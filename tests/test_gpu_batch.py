@@ -192,3 +192,22 @@ def test_batch_select_zero_ties(wv):
         r, _ = b.result(i)
         assert r.n_candidates >= 0, i
     b.close()
+
+
+def test_batch_wait_on_null_stream_before_any_other_sync(wv):
+    """Submitted on the null stream (handle 0), wait() must still wait:
+    the threshold records read right after it, before any other
+    synchronisation, are the final ones."""
+    assert torch.cuda.current_stream().cuda_stream == 0
+    cubes, vols = _cubes(MIX[:6], seed0=300)
+    want = [O.compress(v.astype(np.int32) if v.dtype == np.int16 else v, c["wavelet"],
+                       c["alg"], c["bins"], c["steps"], c["frac"], want_file=False)["res"]
+            for v, c in zip(vols, cubes)]
+    b = wv.Batch(cubes)
+    for _ in range(2):
+        b.submit()
+        b.wait()
+        got = [b.result(i)[0] for i in range(len(cubes))]
+        for i, (r, w) in enumerate(zip(got, want)):
+            assert r.threshold == w.threshold and r.max_abs == w.max_abs, i
+    b.close()

@@ -82,10 +82,20 @@ class BatchItem(C.Structure):
     _fields_ = [("in_d", C.c_void_p), ("dtype", C.c_int), ("orig", C.c_int * 3),
                 ("wavelet", C.c_int), ("steps", C.c_int * 3), ("quant_alg", C.c_int),
                 ("bin_count", C.c_int), ("thresh_frac", C.c_float),
-                ("coef_d", C.c_void_p), ("bins_d", C.c_void_p), ("counts_d", C.c_void_p)]
+                ("coef_d", C.c_void_p), ("bins_d", C.c_void_p), ("counts_d", C.c_void_p),
+                ("in_stride", C.c_int64 * 2)]
 
 
 BATCH_AUTOTUNE, BATCH_NO_GRAPH = 1, 2
+
+
+class SplitBox(C.Structure):
+    _fields_ = [("offset", C.c_int * 3), ("size", C.c_int * 3)]
+
+
+class SplitInfo(C.Structure):
+    _fields_ = [("cubelets", C.c_int), ("need", C.c_int64), ("ms_plan", C.c_double),
+                ("ms_batch", C.c_double), ("ms_encode", C.c_double), ("ms_total", C.c_double)]
 
 
 class ServeConfig(C.Structure):
@@ -156,6 +166,18 @@ def _declare(L):
         L.wv_huff_decode_host.argtypes = [P, C.c_int, C.c_int, C.c_int, P, I64, P, I64]
         L.wv_write_cube_raw.restype = I64
         L.wv_write_cube_raw.argtypes = [P, C.c_int, P, P, I64]
+    if hasattr(L, "wv_split_plan"):
+        L.wv_split_plan.argtypes = [P, P, C.c_int, C.POINTER(SplitBox), C.c_int]
+        L.wv_cube_stream_open.restype = P
+        L.wv_cube_stream_add_raw.argtypes = [P, P, C.c_int, P, P]
+        L.wv_cube_stream_bytes.restype = I64
+        L.wv_cube_stream_bytes.argtypes = [P, C.c_int]
+        L.wv_cube_stream_finish.restype = I64
+        L.wv_cube_stream_finish.argtypes = [P, C.c_int, P, I64]
+        L.wv_cube_stream_free.argtypes = [P]
+        L.wv_split_compress.restype = I64
+        L.wv_split_compress.argtypes = [P, C.c_int, P, P, C.c_int, C.POINTER(Options), C.c_int,
+                                        P, I64, C.POINTER(SplitInfo), P]
     if hasattr(L, "wv_huff_decode_device"):
         L.wv_huff_decode_ws_bytes.restype = I64
         L.wv_huff_decode_ws_bytes.argtypes = [I64, C.c_int]
@@ -577,6 +599,61 @@ def write_cube_raw(vol):
     return buf[:n].tobytes()
 
 
+def split_plan(size, cub, overlap):
+    """Boxes of `cube split` (wv_split_plan): list of (offset xyz, size xyz)."""
+    n = lib().wv_split_plan(_i3(size), _i3(cub), overlap, None, 0)
+    check(n if n < 0 else 0)
+    boxes = (SplitBox * max(1, n))()
+    check(min(0, lib().wv_split_plan(_i3(size), _i3(cub), overlap, boxes, n)))
+    return [(tuple(b.offset), tuple(b.size)) for b in boxes[:n]]
+
+
+def split_raw(vol, cub, overlap, with_footer=True):
+    """`cube split` of one large volume (numpy [z, y, x]; uint8 / int32 /
+    float32) into an uncompressed cubelet stream with the boxes' offsets."""
+    v = np.ascontiguousarray(vol)
+    dt = {np.dtype(np.uint8): UINT8, np.dtype(np.int32): INT32,
+          np.dtype(np.float32): FLOAT32}[v.dtype]
+    L = lib()
+    w = L.wv_cube_stream_open()
+    try:
+        for off, sz in split_plan([v.shape[2], v.shape[1], v.shape[0]], cub, overlap):
+            box = np.ascontiguousarray(v[off[2]:off[2] + sz[2], off[1]:off[1] + sz[1],
+                                         off[0]:off[0] + sz[0]])
+            check(L.wv_cube_stream_add_raw(w, _i3(sz), dt, box.ctypes.data_as(P), _i3(off)))
+        n = check_size(L.wv_cube_stream_bytes(w, int(with_footer)))
+        buf = np.empty(n, np.uint8)
+        check_size(L.wv_cube_stream_finish(w, int(with_footer), buf.ctypes.data_as(P), n))
+        return buf.tobytes()
+    finally:
+        L.wv_cube_stream_free(w)
+
+
+def split_compress(vol, cub, overlap, wavelet, steps, alg, bins, frac, flags=0,
+                   compress_zeros=True, with_footer=True, stream=None):
+    """Split a device volume (cuda tensor [z, y, x]) into boxes and compress
+    them all through the batched path into one .cube stream
+    (wv_split_compress).  Returns (file bytes, SplitInfo)."""
+    if not vol.is_contiguous():
+        raise ValueError("split_compress: the volume must be contiguous")
+    opt = Options(wavelet, alg, bins, _i3(steps), C.c_float(frac), int(compress_zeros), 0,
+                  int(with_footer))
+    size = _i3([vol.shape[2], vol.shape[1], vol.shape[0]])
+    info = SplitInfo()
+    L = lib()
+    cap = vol.numel() * vol.element_size() * 2 + (1 << 20)
+    for _ in range(2):   # a second run only when the first buffer was short
+        buf = np.empty(cap, np.uint8)
+        n = L.wv_split_compress(_ptr(vol), dtype_id(vol), size, _i3(cub), overlap,
+                                C.byref(opt), flags, buf.ctypes.data_as(P), cap,
+                                C.byref(info), _stream(stream))
+        if n != -4 or info.need <= cap:   # WV_ERR_NOMEM with a known size
+            break
+        cap = info.need
+    check_size(n)
+    return buf[:n].tobytes(), info
+
+
 class Batch:
     """Many cubelets compressed as one batch (wv_batch_*).
 
@@ -592,8 +669,9 @@ class Batch:
         items = (BatchItem * len(cubes))()
         for i, c in enumerate(cubes):
             vol = c["vol"]
-            if not vol.is_contiguous():
-                raise ValueError("batch input %d is not contiguous" % i)
+            # a view of a box inside a larger volume is read in place
+            if vol.dim() != 3 or vol.stride(2) != 1:
+                raise ValueError("batch input %d: needs [z, y, x] with x contiguous" % i)
             orig = [vol.shape[2], vol.shape[1], vol.shape[0]]
             g = geom(orig, c["steps"], c["wavelet"], dtype_id(vol))
             self.geoms.append(g)
@@ -613,6 +691,9 @@ class Batch:
             it.coef_d = self.coef[-1].data_ptr()
             it.bins_d = self.bins[-1].data_ptr()
             it.counts_d = self.counts[-1].data_ptr()
+            if not vol.is_contiguous():
+                it.in_stride[0] = vol.stride(1)
+                it.in_stride[1] = vol.stride(0)
         self.h = C.c_void_p()
         check(lib().wv_batch_plan(items, len(cubes), flags, C.byref(self.h)))
 

@@ -60,6 +60,7 @@ struct Tuning {
 struct Group {
   int padded[3], orig[3], steps[3];
   int dtype, wavelet;
+  int64_t src_sy, src_sz;      // first pass: source strides (0: dense)
   std::vector<int> items;
   PassSpec spec[3];
   int npass;
@@ -69,11 +70,12 @@ struct Group {
   Tuning tune;
 };
 
-typedef std::tuple<int, int, int, int, int, int, int, int, int, int, int, int> GroupKey;
+typedef std::tuple<int, int, int, int, int, int, int, int, int, int, int, int64_t, int64_t>
+    GroupKey;
 
-static GroupKey key_of(const wv_geom &g, int dtype) {
+static GroupKey key_of(const wv_geom &g, int dtype, int64_t sy, int64_t sz) {
   return GroupKey(g.padded[0], g.padded[1], g.padded[2], g.orig[0], g.orig[1], g.orig[2],
-                  g.steps[0], g.steps[1], g.steps[2], dtype, g.wavelet, 0);
+                  g.steps[0], g.steps[1], g.steps[2], dtype, g.wavelet, sy, sz);
 }
 
 // A fixed set of host threads for the edge tables (threads are started
@@ -181,7 +183,8 @@ struct wv_batch {
   hipEvent_t fork = nullptr, join[BATCH_STREAMS] = {};
   hipEvent_t part_done[2] = {};
   hipGraphExec_t gA[2] = {}, gB[2] = {};
-  hipStream_t last_stream = nullptr;
+  hipStream_t last_stream = nullptr;   // may be the null stream
+  bool submitted = false;
 };
 
 namespace wv {
@@ -197,6 +200,10 @@ static int enqueue_group(wv_batch *b, Group &G, hipStream_t s, int ncub_limit) {
     P.src_tab = G.src_tab_d[i];
     P.dst_tab = G.dst_tab_d[i];
     P.ncub = ncub;
+    if (i == 0) {
+      P.src_sy = G.src_sy;
+      P.src_sz = G.src_sz;
+    }
     P.tcap = G.tune.tcap;
     P.ept0 = G.tune.ept0;
     P.bpc = G.tune.bpc;
@@ -303,7 +310,7 @@ static int host_params(wv_batch *b, int part, const wv_thresh_result *res) {
 // Autotuning of one group: time its passes on a sample of its cubelets for
 // each candidate and keep the fastest.
 static int autotune_group(wv_batch *b, Group &G, hipStream_t s) {
-  const GroupKey key = key_of(b->geoms[G.items[0]], G.dtype);
+  const GroupKey key = key_of(b->geoms[G.items[0]], G.dtype, G.src_sy, G.src_sz);
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune_cache.find(key);
@@ -417,7 +424,19 @@ int wv_batch_plan(const wv_batch_item *items, int n, int flags, wv_batch **out) 
     if (rc != WV_OK) { free_batch(b); return rc; }
     max_bins = std::max(max_bins, it.bin_count);
     if (it.quant_alg == WV_QUANT_LOG) b->host_params = true;
-    const GroupKey k = key_of(b->geoms[i], it.dtype);
+    // source strides: a box inside a larger volume (dense strides count
+    // as dense, so such boxes group with dense cubelets)
+    int64_t sy = it.in_stride[0], sz = it.in_stride[1];
+    if (sy || sz) {
+      if (sy < it.orig[0] || sz < sy * it.orig[1]) {
+        set_error("batch item %d: strides %lld, %lld for %dx%dx%d", i, (long long)sy,
+                  (long long)sz, it.orig[0], it.orig[1], it.orig[2]);
+        free_batch(b);
+        return WV_ERR_ARG;
+      }
+      if (sy == it.orig[0] && sz == (int64_t)it.orig[0] * it.orig[1]) sy = sz = 0;
+    }
+    const GroupKey k = key_of(b->geoms[i], it.dtype, sy, sz);
     auto f = gidx.find(k);
     if (f == gidx.end()) {
       Group G{};
@@ -427,6 +446,8 @@ int wv_batch_plan(const wv_batch_item *items, int n, int flags, wv_batch **out) 
       std::memcpy(G.steps, g.steps, sizeof G.steps);
       G.dtype = it.dtype;
       G.wavelet = it.wavelet;
+      G.src_sy = sy;
+      G.src_sz = sz;
       G.npass = forward_specs(g.padded, g.steps, G.spec);
       pass_buffers(G.spec, G.npass, G.dst_out);
       gidx[k] = (int)b->groups.size();
@@ -619,6 +640,7 @@ int wv_batch_submit(wv_batch *b, void *stream) {
   if (!b) { set_error("null batch"); return WV_ERR_ARG; }
   hipStream_t s = (hipStream_t)stream;
   b->last_stream = s;
+  b->submitted = true;
   const bool graphs = !(b->flags & WV_BATCH_NO_GRAPH);
   const int n = (int)b->items.size();
   auto run = [&](hipGraphExec_t *g, const EnqueueFn &fn) -> int {
@@ -681,7 +703,7 @@ int wv_batch_submit(wv_batch *b, void *stream) {
 
 int wv_batch_poll(wv_batch *b) {
   if (!b) { set_error("null batch"); return WV_ERR_ARG; }
-  if (!b->last_stream) return 1;
+  if (!b->submitted) return 1;
   const hipError_t e = hipStreamQuery(b->last_stream);
   if (e == hipSuccess) return 1;
   if (e == hipErrorNotReady) return 0;
@@ -691,7 +713,7 @@ int wv_batch_poll(wv_batch *b) {
 
 int wv_batch_wait(wv_batch *b) {
   if (!b) { set_error("null batch"); return WV_ERR_ARG; }
-  if (b->last_stream && hipStreamSynchronize(b->last_stream) != hipSuccess) {
+  if (b->submitted && hipStreamSynchronize(b->last_stream) != hipSuccess) {
     set_error("batch wait");
     return WV_ERR_HIP;
   }

@@ -226,14 +226,15 @@ struct Proto {
 
 // Cube::copyToCubeletBuffer for an uncompressed cube (offsets 0, type,
 // compression NONE, byte count); data_file_offset only in the index copy
-std::string raw_cubelet_msg(const int size[3], int dtype, uint64_t nbytes, uint64_t offset) {
+std::string raw_cubelet_msg(const int size[3], int dtype, uint64_t nbytes, uint64_t offset,
+                            const int *box = nullptr) {
   Proto m;
   m.u32(1, (uint32_t)size[0]);
   m.u32(2, (uint32_t)size[1]);
   m.u32(3, (uint32_t)size[2]);
-  m.u32(4, 0);
-  m.u32(5, 0);
-  m.u32(6, 0);
+  m.u32(4, box ? (uint32_t)box[0] : 0u);
+  m.u32(5, box ? (uint32_t)box[1] : 0u);
+  m.u32(6, box ? (uint32_t)box[2] : 0u);
   m.u32(7, (uint32_t)dtype);
   m.u32(8, (uint32_t)nbytes);
   m.u32(9, 0);
@@ -429,3 +430,94 @@ extern "C" int64_t wv_write_cube_raw(const int size[3], int dtype, const void *d
   std::memcpy(file_buf + head.size() + nbytes, tail.data(), tail.size());
   return total;
 }
+
+// ---- many cubelets in one stream (CubeletStreamWriter::addCubelet /
+// close, cubelet_file.cc:259-327): each record is its length, the
+// CubeletBuffer message and the data; the index repeats every message with
+// data_file_offset set
+struct wv_cube_stream {
+  std::string body;
+  std::vector<std::string> index;
+  wv_cube_stream() {
+    char header[64] = {0};
+    std::memcpy(header, "SCU cubelets 1.0\n", 17);
+    body.append(header, 64);
+  }
+};
+
+namespace {
+
+int stream_add_record(wv_cube_stream *w, const std::string &meta,
+                      const std::string &indexed, const void *data, uint64_t nbytes) {
+  const uint32_t mlen = (uint32_t)meta.size();
+  w->body.append((const char *)&mlen, 4);
+  w->body += meta;
+  if (nbytes) w->body.append((const char *)data, nbytes);
+  w->index.push_back(indexed);
+  return WV_OK;
+}
+
+std::string stream_tail(const wv_cube_stream *w) {
+  std::string tail;
+  const int32_t eof = -1;
+  tail.append((const char *)&eof, 4);
+  Proto index;
+  for (const std::string &m : w->index) index.bytes(5, m);
+  tail += index.b;
+  const int32_t ilen = (int32_t)index.b.size();
+  tail.append((const char *)&ilen, 4);
+  tail += "cube";
+  return tail;
+}
+
+}  // namespace
+
+extern "C" wv_cube_stream *wv_cube_stream_open(void) { return new wv_cube_stream(); }
+
+extern "C" int wv_cube_stream_add(wv_cube_stream *w, const wv_cube_desc *d, const int offset[3],
+                                  const wv_huff *book, const uint32_t *words, int64_t nwords) {
+  if (!w || !d || !book || !offset || nwords < 0 || (nwords && !words) ||
+      nwords > 0x3fffffffll) {
+    wv::set_error("wv_cube_stream_add: bad argument");
+    return WV_ERR_ARG;
+  }
+  const uint32_t nbytes = (uint32_t)(nwords * 4);
+  const uint64_t data_at = w->body.size() + 4;   // after the length word ...
+  const std::string meta = wv::compressed_cubelet_msg(d, book, nbytes, 0, offset);
+  const std::string indexed =
+      wv::compressed_cubelet_msg(d, book, nbytes, data_at + meta.size(), offset);   // ... and meta
+  return stream_add_record(w, meta, indexed, words, nbytes);
+}
+
+extern "C" int wv_cube_stream_add_raw(wv_cube_stream *w, const int size[3], int dtype,
+                                      const void *data, const int offset[3]) {
+  if (!w || !size || !data || !offset || size[0] <= 0 || size[1] <= 0 || size[2] <= 0 ||
+      (dtype != WV_DTYPE_UINT8 && dtype != WV_DTYPE_INT32 && dtype != WV_DTYPE_FLOAT32)) {
+    wv::set_error("wv_cube_stream_add_raw: bad argument");
+    return WV_ERR_ARG;
+  }
+  const uint64_t nbytes =
+      (uint64_t)size[0] * size[1] * size[2] * (dtype == WV_DTYPE_UINT8 ? 1 : 4);
+  const std::string meta = raw_cubelet_msg(size, dtype, nbytes, 0, offset);
+  const uint64_t data_at = w->body.size() + 4 + meta.size();
+  return stream_add_record(w, meta, raw_cubelet_msg(size, dtype, nbytes, data_at, offset), data,
+                           nbytes);
+}
+
+extern "C" int64_t wv_cube_stream_bytes(const wv_cube_stream *w, int with_footer) {
+  if (!w) return WV_ERR_ARG;
+  return (int64_t)(w->body.size() + (with_footer ? stream_tail(w).size() : 0));
+}
+
+extern "C" int64_t wv_cube_stream_finish(wv_cube_stream *w, int with_footer, uint8_t *file_buf,
+                                         int64_t cap) {
+  if (!w) return WV_ERR_ARG;
+  const std::string tail = with_footer ? stream_tail(w) : std::string();
+  const int64_t total = (int64_t)(w->body.size() + tail.size());
+  if (!file_buf || total > cap) return WV_ERR_NOMEM;
+  std::memcpy(file_buf, w->body.data(), w->body.size());
+  if (!tail.empty()) std::memcpy(file_buf + w->body.size(), tail.data(), tail.size());
+  return total;
+}
+
+extern "C" void wv_cube_stream_free(wv_cube_stream *w) { delete w; }

@@ -835,6 +835,10 @@ int plan_pass(DwtPass &P, const PassSpec &s, bool clamp) {
   int64_t si[3], so[3];
   if (clamp) layout_strides(s.lin, P.ox, P.oy, P.oz, si);
   else layout_strides(s.lin, P.X, P.Y, P.Z, si);
+  if (clamp && P.src_sy > 0 && s.lin == WV_LAYOUT_XYZ) {   // a box of a larger volume
+    si[1] = P.src_sy;
+    si[2] = P.src_sz;
+  }
   layout_strides(s.lout, P.X, P.Y, P.Z, so);
   P.ld_line = si[a] == 1;
   P.st_line = so[a] == 1;

@@ -279,14 +279,15 @@ static std::string params_msg(const wv_cube_desc *d, const Coder &c,
 }
 
 static std::string cubelet_msg(const wv_cube_desc *d, const std::string &params,
-                               uint32_t byte_count, uint64_t data_offset) {
+                               uint32_t byte_count, uint64_t data_offset,
+                               const int *offset = nullptr) {
   Proto m;
   m.u32(1, (uint32_t)d->out_size[0]);
   m.u32(2, (uint32_t)d->out_size[1]);
   m.u32(3, (uint32_t)d->out_size[2]);
-  m.u32(4, 0);
-  m.u32(5, 0);
-  m.u32(6, 0);
+  m.u32(4, offset ? (uint32_t)offset[0] : 0u);
+  m.u32(5, offset ? (uint32_t)offset[1] : 0u);
+  m.u32(6, offset ? (uint32_t)offset[2] : 0u);
   m.u32(7, 2);                  // INT32
   m.u32(8, byte_count);
   m.u32(9, 2);                  // WAVELET
@@ -345,6 +346,18 @@ struct wv_huff {
   wv_huff(const uint32_t *counts, int b, int dup, int64_t singles, int64_t dups)
       : coder(counts, b, dup, singles, dups), bin_count(b), payload_bits(0) {}
 };
+
+namespace wv {
+// the CubeletBuffer message of a compressed cubelet (copyToCubeletBuffer,
+// wavelet.cc:283-343) at box offset `offset`, data_file_offset when > 0
+std::string compressed_cubelet_msg(const wv_cube_desc *d, const wv_huff *book, uint32_t nbytes,
+                                   uint64_t data_offset, const int offset[3]) {
+  const int dup = book->coder.dup();
+  const int dup_key = dup >= 0 ? d->bin_count : -1;
+  const std::string params = wvh::params_msg(d, book->coder, dup_key, dup);
+  return wvh::cubelet_msg(d, params, nbytes, data_offset, offset);
+}
+}  // namespace wv
 
 extern "C" int64_t wv_write_cube(const wv_cube_desc *d, const int32_t *bins,
                                  int64_t n, const uint32_t *counts,

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include <string>
 #include <vector>
 #include "../../include/wv_hip.h"
 
@@ -49,6 +50,9 @@ struct DwtPass {
   // launch overrides (0: defaults): tile width cap, elements per thread of
   // line-contiguous passes, resident blocks per CU
   int tcap, ept0, bpc;
+  // first forward pass of a box inside a larger volume: source samples
+  // between rows and slices (0: the dense unpadded source)
+  int64_t src_sy, src_sz;
 };
 
 // Memory layouts of a cubelet, fastest axis first: x,y,z (the input and
@@ -169,6 +173,10 @@ void cube_desc_of(const wv_geom &g, const wv_options *opt, int dtype, const wv_t
                   int zero_bin, wv_cube_desc &d);
 
 void set_error(const char *fmt, ...);
+
+// the CubeletBuffer message of a compressed cubelet at a box offset
+std::string compressed_cubelet_msg(const wv_cube_desc *d, const wv_huff *book, uint32_t nbytes,
+                                   uint64_t data_offset, const int offset[3]);
 
 // compute units of the current device (cached per device)
 int cu_count();

@@ -1,0 +1,247 @@
+// wv_split.cpp -- cube split with overlap (cube split, cube_split_typed
+// cubelet_convert.cc:368-553) feeding the batched compression path.
+//
+// The reference cuts a large volume into cubelet files on the host and
+// then compresses each cubelet with its own run of test_compress_gpu.  Here
+// the volume stays in HBM: every box becomes a batch item whose first
+// transform pass reads the box in place through the volume's row and slice
+// strides (no extraction copy), the whole set runs as one batch (grouped
+// persistent pass launches, batched select and quantize), each cubelet is
+// Huffman-coded on the device, and the payloads are written as one .cube
+// stream with the boxes' offsets.
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+#include "wv_internal.h"
+
+using namespace wv;
+
+// cube_split_typed: x / y counts and offsets (cubelet_convert.cc:451-483),
+// z from the frame loop (:490-553): a chunk is flushed when it holds cub[2]
+// frames, then the loop backs up `overlap` frames unless it is at the last
+// frame; what is left at the end is one shorter chunk.  Write order: z
+// chunk, then the y-major, x-minor list of the x / y boxes.
+extern "C" int wv_split_plan(const int size[3], const int cub[3], int overlap,
+                             wv_split_box *boxes, int cap) {
+  if (!size || !cub || overlap < 0) {
+    set_error("wv_split_plan: bad argument");
+    return WV_ERR_ARG;
+  }
+  for (int a = 0; a < 3; a++)
+    if (size[a] <= 0 || cub[a] <= 0 || overlap >= cub[a]) {
+      set_error("wv_split_plan: volume %dx%dx%d, cubelets %dx%dx%d, overlap %d (needs "
+                "0 <= overlap < cubelet size)",
+                size[0], size[1], size[2], cub[0], cub[1], cub[2], overlap);
+      return WV_ERR_ARG;
+    }
+  // boxes per row / column: (size - overlap - 1) / (cub - overlap) + 1
+  // with C's truncating division
+  int count[2];
+  for (int a = 0; a < 2; a++) count[a] = (size[a] - overlap - 1) / (cub[a] - overlap) + 1;
+  std::vector<int> zoff, zlen;
+  int start = 0, depth = 0;
+  for (int f = 0; f < size[2]; f++) {
+    depth++;
+    if (depth == cub[2]) {
+      zoff.push_back(start);
+      zlen.push_back(depth);
+      start = f + 1 - overlap;
+      depth = 0;
+      if (f < size[2] - 1) f -= overlap;
+    }
+  }
+  if (depth > 0) {
+    zoff.push_back(start);
+    zlen.push_back(depth);
+  }
+  const int64_t total = (int64_t)zoff.size() * count[0] * count[1];
+  if (total > 0x7fffffff) {
+    set_error("wv_split_plan: %lld boxes", (long long)total);
+    return WV_ERR_ARG;
+  }
+  int k = 0;
+  for (size_t zc = 0; zc < zoff.size(); zc++)
+    for (int y = 0; y < count[1]; y++)
+      for (int x = 0; x < count[0]; x++, k++) {
+        if (!boxes || k >= cap) continue;
+        wv_split_box &b = boxes[k];
+        b.offset[0] = x * (cub[0] - overlap);
+        b.offset[1] = y * (cub[1] - overlap);
+        b.offset[2] = zoff[zc];
+        b.size[0] = std::min(cub[0], size[0] - b.offset[0]);
+        b.size[1] = std::min(cub[1], size[1] - b.offset[1]);
+        b.size[2] = zlen[zc];
+      }
+  return k;
+}
+
+namespace {
+
+int esize(int dtype) {
+  return dtype == WV_DTYPE_UINT8 ? 1 : dtype == WV_DTYPE_INT16 ? 2 : 4;
+}
+
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+struct BatchHandle {
+  wv_batch *b = nullptr;
+  ~BatchHandle() {
+    if (b) (void)wv_batch_destroy(b);
+  }
+};
+
+struct StreamHandle {
+  wv_cube_stream *w = nullptr;
+  ~StreamHandle() { wv_cube_stream_free(w); }
+};
+
+#define SCHK(call)                  \
+  do {                              \
+    int _rc = (call);               \
+    if (_rc != WV_OK) return _rc;   \
+  } while (0)
+#define SHIP(expr, what)                                         \
+  do {                                                           \
+    hipError_t _e = (expr);                                      \
+    if (_e != hipSuccess) {                                      \
+      set_error("%s: %s", what, hipGetErrorString(_e));          \
+      return WV_ERR_HIP;                                         \
+    }                                                            \
+  } while (0)
+
+double ms_since(std::chrono::steady_clock::time_point t) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t).count();
+}
+
+}  // namespace
+
+extern "C" int64_t wv_split_compress(const void *vol_d, int dtype, const int size[3],
+                                     const int cub[3], int overlap, const wv_options *opt,
+                                     int batch_flags, uint8_t *file_buf, int64_t cap,
+                                     wv_split_info *info, void *stream) {
+  const auto t0 = std::chrono::steady_clock::now();
+  if (!vol_d || !opt || !size || !cub) {
+    set_error("wv_split_compress: null argument");
+    return WV_ERR_ARG;
+  }
+  if (opt->quant_alg != WV_QUANT_UNIFORM && opt->quant_alg != WV_QUANT_LOG) {
+    set_error("wv_split_compress: quantizer %d (uniform or log)", opt->quant_alg);
+    return WV_ERR_UNSUPPORTED;
+  }
+  if (dtype != WV_DTYPE_UINT8 && dtype != WV_DTYPE_INT16 && dtype != WV_DTYPE_INT32 &&
+      dtype != WV_DTYPE_FLOAT32) {
+    set_error("wv_split_compress: dtype %d", dtype);
+    return WV_ERR_ARG;
+  }
+  const int nbox = wv_split_plan(size, cub, overlap, nullptr, 0);
+  if (nbox < 0) return nbox;
+  std::vector<wv_split_box> boxes((size_t)nbox);
+  wv_split_plan(size, cub, overlap, boxes.data(), nbox);
+  // per box: geometry, then coefficient / bins / counts regions of one
+  // device arena
+  std::vector<wv_geom> geoms((size_t)nbox);
+  std::vector<int64_t> off_coef(nbox), off_bins(nbox), off_counts(nbox);
+  auto al = [](int64_t v) { return (v + 255) & ~(int64_t)255; };
+  int64_t total = 0, max_count = 0;
+  for (int i = 0; i < nbox; i++) {
+    SCHK(wv_geom_init(&geoms[i], boxes[i].size, opt->steps, opt->wavelet, dtype));
+    const int64_t n = geoms[i].count;
+    max_count = std::max(max_count, n);
+    off_coef[i] = total; total += al(n * 4);
+    off_bins[i] = total; total += al(n * 4);
+    off_counts[i] = total; total += al(((int64_t)opt->bin_count + 1) * 4);
+  }
+  const int64_t enc_ws = wv_encode_ws_bytes(max_count, opt->bin_count);
+  const int64_t words_cap = wv_encode_words_bound(max_count, opt->bin_count);
+  if (enc_ws < 0 || words_cap < 0) {
+    set_error("wv_split_compress: %d bins", opt->bin_count);
+    return WV_ERR_ARG;
+  }
+  const int64_t off_ws = total;
+  total += al(enc_ws);
+  const int64_t off_words = total;
+  total += al(words_cap * 4);
+  DevBuf arena;
+  SHIP(hipMalloc(&arena.p, (size_t)total), "wv_split_compress: device buffers");
+  char *ar = (char *)arena.p;
+  const int es = esize(dtype);
+  std::vector<wv_batch_item> items((size_t)nbox);
+  for (int i = 0; i < nbox; i++) {
+    const wv_split_box &bx = boxes[i];
+    wv_batch_item &it = items[i];
+    it = wv_batch_item{};
+    const int64_t first = ((int64_t)bx.offset[2] * size[1] + bx.offset[1]) * size[0] + bx.offset[0];
+    it.in_d = (const char *)vol_d + first * es;
+    it.dtype = dtype;
+    std::memcpy(it.orig, bx.size, sizeof it.orig);
+    it.wavelet = opt->wavelet;
+    std::memcpy(it.steps, opt->steps, sizeof it.steps);
+    it.quant_alg = opt->quant_alg;
+    it.bin_count = opt->bin_count;
+    it.thresh_frac = opt->thresh_frac;
+    it.coef_d = (float *)(ar + off_coef[i]);
+    it.bins_d = (int32_t *)(ar + off_bins[i]);
+    it.counts_d = (uint32_t *)(ar + off_counts[i]);
+    it.in_stride[0] = size[0];
+    it.in_stride[1] = (int64_t)size[0] * size[1];
+  }
+  const double ms_plan = ms_since(t0);
+  const auto t1 = std::chrono::steady_clock::now();
+  BatchHandle bh;
+  SCHK(wv_batch_plan(items.data(), nbox, batch_flags, &bh.b));
+  SCHK(wv_batch_submit(bh.b, stream));
+  SCHK(wv_batch_wait(bh.b));
+  const double ms_batch = ms_since(t1);
+  // code each cubelet on the device and append it to the stream
+  const auto t2 = std::chrono::steady_clock::now();
+  hipStream_t s = (hipStream_t)stream;
+  StreamHandle sh;
+  sh.w = wv_cube_stream_open();
+  std::vector<uint32_t> words;
+  for (int i = 0; i < nbox; i++) {
+    wv_thresh_result res;
+    int zb = 0;
+    SCHK(wv_batch_result(bh.b, i, &res, &zb));
+    int64_t nwords = 0;
+    wv_huff *book = nullptr;
+    SCHK(wv_encode_device(items[i].bins_d, geoms[i].count, items[i].counts_d, opt->bin_count,
+                          opt->compress_zeros ? zb : -1, ar + off_ws,
+                          (uint32_t *)(ar + off_words), words_cap, &nwords, &book, stream));
+    words.resize((size_t)nwords);
+    hipError_t e = nwords ? hipMemcpyAsync(words.data(), ar + off_words, nwords * 4,
+                                           hipMemcpyDeviceToHost, s)
+                          : hipSuccess;
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+      wv_huff_free(book);
+      set_error("wv_split_compress: payload D2H: %s", hipGetErrorString(e));
+      return WV_ERR_HIP;
+    }
+    wv_cube_desc d;
+    cube_desc_of(geoms[i], opt, dtype, res, {}, {}, zb, d);
+    const int rc = wv_cube_stream_add(sh.w, &d, boxes[i].offset, book, words.data(), nwords);
+    wv_huff_free(book);
+    if (rc != WV_OK) return rc;
+  }
+  const int64_t need = wv_cube_stream_bytes(sh.w, opt->with_footer);
+  if (info) {
+    info->cubelets = nbox;
+    info->need = need;
+    info->ms_plan = ms_plan;
+    info->ms_batch = ms_batch;
+    info->ms_encode = ms_since(t2);
+    info->ms_total = ms_since(t0);
+  }
+  if (!file_buf || cap < need) {
+    set_error("wv_split_compress: file buffer %lld < %lld", (long long)cap, (long long)need);
+    return WV_ERR_NOMEM;
+  }
+  return wv_cube_stream_finish(sh.w, opt->with_footer, file_buf, cap);
+}
