@@ -54,8 +54,12 @@ def parse():
                     help="first run a plain device copy of 4 N bytes (N = samples of the "
                          "single cubelet) a few times: the reference kernel for scaling "
                          "FETCH_SIZE / WRITE_SIZE counters")
-    ap.add_argument("--extras", default="e2e,serving",
-                    help="comma list: e2e (single-cubelet one-call), serving")
+    ap.add_argument("--extras", default="e2e,serving,split",
+                    help="comma list: e2e (single-cubelet one-call), serving, split "
+                         "(the single cubelet's volume split into overlapping cubelets, "
+                         "all compressed by the batched path)")
+    ap.add_argument("--split-cube", type=int, default=64, help="split: cubelet edge")
+    ap.add_argument("--split-overlap", type=int, default=8, help="split: overlap")
     ap.add_argument("--serve-count", type=int, default=300, help="requests of the serving run")
     ap.add_argument("--serve-slots", type=int, default=8, help="serving device workers")
     ap.add_argument("--serve-load", type=float, default=0.7,
@@ -386,6 +390,42 @@ def mixed_stream(args, wv, torch, dev, steps, warmup):
     return out
 
 
+def split_leg(args, wv, torch, dev, vol):
+    """cube split with overlap feeding the batched path: the single
+    cubelet's volume (device-resident) cut into cubelets, every box read in
+    place by the batched transform, coded on the device, written as one
+    .cube stream."""
+    vd = torch.from_numpy(np.ascontiguousarray(vol)).to(dev)
+    cub = [args.split_cube] * 3
+    ov = args.split_overlap
+    f, info = wv.split_compress(vd, cub, ov, wv.CDF97, [3, 3, 3], wv.UNIFORM, 1024, 0.95)
+    torch.cuda.synchronize()
+    runs = []
+    for _ in range(3):
+        t0 = time.time()
+        f, info = wv.split_compress(vd, cub, ov, wv.CDF97, [3, 3, 3], wv.UNIFORM, 1024, 0.95)
+        runs.append(((time.time() - t0) * 1e3, info.ms_plan, info.ms_batch, info.ms_encode))
+    runs.sort()
+    ms, ms_plan, ms_batch, ms_enc = runs[len(runs) // 2]
+    nbytes = vd.numel() * 4
+    log("split: %d cubelets of %d^3 (overlap %d) in %.1f ms (batch %.1f, coding %.1f), %.1f GB/s"
+        % (info.cubelets, args.split_cube, ov, ms, ms_batch, ms_enc, nbytes / ms / 1e6))
+    out = {
+        "workload": "%d^3 float32 volume split into %d^3 cubelets with overlap %d, CDF97 3,3,3, "
+                    "thresh 0.95, uniform 1024, one .cube stream" % (
+                        vol.shape[0], args.split_cube, ov),
+        "cubelets": info.cubelets,
+        "ms_total": round(ms, 3), "ms_batch": round(ms_batch, 3),
+        "ms_coding": round(ms_enc, 3), "ms_plan": round(ms_plan, 3),
+        "gbps_input": round(nbytes / ms / 1e6, 3),
+        "gbps_input_batch_only": round(nbytes / ms_batch / 1e6, 3),
+        "file_bytes": len(f), "ratio": round(nbytes / len(f), 2),
+    }
+    del vd
+    torch.cuda.empty_cache()
+    return out
+
+
 def serving(args, wv, torch, dev):
     """The serving loop on a Poisson request stream at a fixed offered rate.
     Requests: edges lognormal (median 128, sigma 0.6) clipped to [32, 512],
@@ -471,6 +511,9 @@ def main():
     serve = None
     if "serving" in args.extras and args.workload == "batched":
         serve = serving(args, wv, torch, dev)
+    split = None
+    if "split" in args.extras:
+        split = split_leg(args, wv, torch, dev, vol)
 
     cpu = None
     if not args.no_cpu:
@@ -508,6 +551,8 @@ def main():
         line["batched"] = batched
     if serve is not None:
         line["serving"] = serve
+    if split is not None:
+        line["split"] = split
     print(json.dumps(line), flush=True)
 
 

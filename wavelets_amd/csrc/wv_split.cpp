@@ -8,11 +8,17 @@
 // strides (no extraction copy), the whole set runs as one batch (grouped
 // persistent pass launches, batched select and quantize), each cubelet is
 // Huffman-coded on the device, and the payloads are written as one .cube
-// stream with the boxes' offsets.
+// stream with the boxes' offsets.  The coding runs on a pool of host
+// threads, each with its own stream and workspace: one box's coding is a
+// short chain of small launches and synchronisations, so boxes are coded
+// side by side.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cstring>
+#include <string>
+#include <thread>
 #include <vector>
 #include "wv_internal.h"
 
@@ -164,10 +170,12 @@ extern "C" int64_t wv_split_compress(const void *vol_d, int dtype, const int siz
     set_error("wv_split_compress: %d bins", opt->bin_count);
     return WV_ERR_ARG;
   }
+  // coding workers: one stream, workspace and payload buffer each
+  const int nworkers = std::max(1, std::min(nbox, 16));
   const int64_t off_ws = total;
-  total += al(enc_ws);
+  total += al(enc_ws) * nworkers;
   const int64_t off_words = total;
-  total += al(words_cap * 4);
+  total += al(words_cap * 4) * nworkers;
   DevBuf arena;
   SHIP(hipMalloc(&arena.p, (size_t)total), "wv_split_compress: device buffers");
   char *ar = (char *)arena.p;
@@ -195,41 +203,79 @@ extern "C" int64_t wv_split_compress(const void *vol_d, int dtype, const int siz
   const double ms_plan = ms_since(t0);
   const auto t1 = std::chrono::steady_clock::now();
   BatchHandle bh;
-  SCHK(wv_batch_plan(items.data(), nbox, batch_flags, &bh.b));
+  // one submit: launched directly (a graph would first need an eager run)
+  SCHK(wv_batch_plan(items.data(), nbox, batch_flags | WV_BATCH_NO_GRAPH, &bh.b));
   SCHK(wv_batch_submit(bh.b, stream));
   SCHK(wv_batch_wait(bh.b));
   const double ms_batch = ms_since(t1);
-  // code each cubelet on the device and append it to the stream
+  // code the boxes on the workers (box k on worker k % nworkers), then
+  // append them to the stream in box order
   const auto t2 = std::chrono::steady_clock::now();
-  hipStream_t s = (hipStream_t)stream;
-  StreamHandle sh;
-  sh.w = wv_cube_stream_open();
-  std::vector<uint32_t> words;
-  for (int i = 0; i < nbox; i++) {
+  struct Coded {
+    std::vector<uint32_t> words;
+    wv_huff *book = nullptr;
     wv_thresh_result res;
     int zb = 0;
-    SCHK(wv_batch_result(bh.b, i, &res, &zb));
-    int64_t nwords = 0;
-    wv_huff *book = nullptr;
-    SCHK(wv_encode_device(items[i].bins_d, geoms[i].count, items[i].counts_d, opt->bin_count,
-                          opt->compress_zeros ? zb : -1, ar + off_ws,
-                          (uint32_t *)(ar + off_words), words_cap, &nwords, &book, stream));
-    words.resize((size_t)nwords);
-    hipError_t e = nwords ? hipMemcpyAsync(words.data(), ar + off_words, nwords * 4,
-                                           hipMemcpyDeviceToHost, s)
-                          : hipSuccess;
-    if (e == hipSuccess) e = hipStreamSynchronize(s);
-    if (e != hipSuccess) {
-      wv_huff_free(book);
-      set_error("wv_split_compress: payload D2H: %s", hipGetErrorString(e));
-      return WV_ERR_HIP;
+  };
+  std::vector<Coded> coded((size_t)nbox);
+  std::vector<int> rcs((size_t)nworkers, WV_OK);
+  std::vector<std::string> errs((size_t)nworkers);   // error texts are per thread
+  std::vector<std::thread> pool;
+  for (int t = 0; t < nworkers; t++)
+    pool.emplace_back([&, t] {
+      hipStream_t ws = nullptr;
+      if (hipStreamCreateWithFlags(&ws, hipStreamNonBlocking) != hipSuccess) {
+        rcs[t] = WV_ERR_HIP;
+        errs[t] = "wv_split_compress: coding stream";
+        return;
+      }
+      char *my_ws = ar + off_ws + al(enc_ws) * t;
+      uint32_t *my_words = (uint32_t *)(ar + off_words + al(words_cap * 4) * t);
+      for (int i = t; i < nbox && rcs[t] == WV_OK; i += nworkers) {
+        Coded &c = coded[i];
+        int rc = wv_batch_result(bh.b, i, &c.res, &c.zb);
+        int64_t nwords = 0;
+        if (rc == WV_OK)
+          rc = wv_encode_device(items[i].bins_d, geoms[i].count, items[i].counts_d,
+                                opt->bin_count, opt->compress_zeros ? c.zb : -1, my_ws, my_words,
+                                words_cap, &nwords, &c.book, ws);
+        if (rc == WV_OK) {
+          c.words.resize((size_t)nwords);
+          hipError_t e = nwords ? hipMemcpyAsync(c.words.data(), my_words, nwords * 4,
+                                                 hipMemcpyDeviceToHost, ws)
+                                : hipSuccess;
+          if (e == hipSuccess) e = hipStreamSynchronize(ws);
+          if (e != hipSuccess) {
+            set_error("wv_split_compress: payload D2H: %s", hipGetErrorString(e));
+            rc = WV_ERR_HIP;
+          }
+        }
+        rcs[t] = rc;
+        if (rc != WV_OK) errs[t] = wv_last_error();
+      }
+      (void)hipStreamDestroy(ws);
+    });
+  for (auto &th : pool) th.join();
+  int rc = WV_OK;
+  for (int t = 0; t < nworkers; t++)
+    if (rcs[t] != WV_OK) {
+      rc = rcs[t];
+      set_error("%s", errs[t].c_str());
     }
-    wv_cube_desc d;
-    cube_desc_of(geoms[i], opt, dtype, res, {}, {}, zb, d);
-    const int rc = wv_cube_stream_add(sh.w, &d, boxes[i].offset, book, words.data(), nwords);
-    wv_huff_free(book);
-    if (rc != WV_OK) return rc;
+  StreamHandle sh;
+  sh.w = wv_cube_stream_open();
+  for (int i = 0; i < nbox; i++) {
+    Coded &c = coded[i];
+    if (rc == WV_OK) {
+      wv_cube_desc d;
+      cube_desc_of(geoms[i], opt, dtype, c.res, {}, {}, c.zb, d);
+      rc = wv_cube_stream_add(sh.w, &d, boxes[i].offset, c.book, c.words.data(),
+                              (int64_t)c.words.size());
+    }
+    wv_huff_free(c.book);
+    c.book = nullptr;
   }
+  if (rc != WV_OK) return rc;
   const int64_t need = wv_cube_stream_bytes(sh.w, opt->with_footer);
   if (info) {
     info->cubelets = nbox;
