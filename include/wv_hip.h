@@ -552,6 +552,37 @@ int64_t wv_split_compress(const void *vol_d, int dtype, const int size[3], const
                           uint8_t *file_buf, int64_t cap, wv_split_info *info,
                           void *stream);
 
+/* ------------------------------------------------------------------ *
+ * Parameter optimisation (-opt): OptimizationData + testParameters
+ * (optimize.h:6-145, optimize_gpu.cu:15-202), the interface the
+ * reference's error-control sweeps (optimize.cc:4-133) use.  create runs the
+ * forward transform once, takes min / max / max |x| and sorts |x| on the
+ * device; in_d (the original samples) must stay valid until destroy.  A
+ * test point quantizes with threshold T and bin_count bins (uniform or
+ * log, max = max |x|; bin_count <= 0: only |x| <= T -> 0), inverts, and
+ * compares with the original over its extent (restore rules of
+ * computeErrorRatesAfterDequant).  size_bytes is
+ * Huffman::totalEncodedLengthBytes of that quantization (huffman.cc:
+ * 241-252), 0 without quantization.  Error sums are double, deterministic.
+ * ------------------------------------------------------------------ */
+typedef struct wv_opt wv_opt;
+typedef struct {
+  int64_t size_bytes;
+  float l1, l2, mse, psnr, rel;   /* getL1Error, getL2Error, getMeanSquaredError,
+                                     getPeakSignalToNoiseRatio, getRelativeError */
+  int zero_bin;                   /* quant(0); -1 without quantization */
+} wv_opt_result;
+int wv_opt_create(const void *in_d, int dtype, const int orig[3], const int steps[3],
+                  int wavelet, int max_possible, int compress_zeros, wv_opt **out,
+                  void *stream);
+int64_t wv_opt_count(const wv_opt *o);
+int wv_opt_stats(const wv_opt *o, float *min_val, float *max_val, float *max_abs);
+float wv_opt_sorted(const wv_opt *o, int64_t index);        /* getSorted */
+int64_t wv_opt_find_sorted(const wv_opt *o, float value);  /* findSorted */
+int wv_opt_test(wv_opt *o, float threshold, int bin_count, int quant_alg, wv_opt_result *r,
+                void *stream);
+int wv_opt_destroy(wv_opt *o);
+
 #ifdef __cplusplus
 }
 #endif

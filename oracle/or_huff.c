@@ -227,6 +227,17 @@ void or_huff_get_counts(const or_huffman *h, int64_t *out) {
 }
 
 int or_huff_code_length(const or_huffman *h, int v) { return h->code_len[v]; }
+
+/* Huffman::totalEncodedLengthBytes, huffman.cc:241-252: sum over values of
+   code length times count, plus the dup key's length per dup string (the
+   3-bit length fields are not counted), rounded up to 32-bit words.  The
+   reference sums in int; 64 bits here. */
+int64_t or_huff_total_encoded_bytes(const or_huffman *h) {
+  int64_t total = 0;
+  for (int v = 0; v < h->size; v++) total += (int64_t)h->code_len[v] * h->counts[v];
+  if (h->dup_value >= 0) total += h->dup_value_count * h->code_len[h->size];
+  return (total + 31) / 32 * 4;
+}
 int or_huff_longest(const or_huffman *h) { return h->longest; }
 
 int or_huff_decode_table(const or_huffman *h, int32_t *table, int cap) {

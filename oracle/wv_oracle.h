@@ -125,6 +125,8 @@ void or_huff_free(or_huffman *h);
 void or_huff_get_counts(const or_huffman *h, int64_t *counts_out);
 int or_huff_code_length(const or_huffman *h, int value);
 int or_huff_longest(const or_huffman *h);
+/* Huffman::totalEncodedLengthBytes (huffman.cc:241-252) */
+int64_t or_huff_total_encoded_bytes(const or_huffman *h);
 /* decode table (BFS order pairs); returns its length in ints */
 int or_huff_decode_table(const or_huffman *h, int32_t *table, int cap);
 /* encodes data; returns number of 32-bit words written (cap in words) or
@@ -218,5 +220,18 @@ double or_now(void);
    packed copies of the boxes' samples */
 int or_split(const int size[3], const int cub[3], int overlap, int *boxes, int cap,
              const void *vol, int esize, void *out);
+
+/* one test point of the parameter optimisation (testParameters,
+   optimize_gpu.cu:52-202, with the CPU path's double error sums): forward
+   transform of the original, then bin_count <= 0: |x| <= threshold -> 0;
+   else quantize (uniform / log, max = max |x|), Huffman size
+   (totalEncodedLengthBytes), dequantize; inverse; error over the original
+   extent.  Returns the size in bytes (0 without quantization). */
+int64_t or_opt_test(const void *in, int dtype, const int orig[3], const int steps[3],
+                    int wavelet, int max_possible, int compress_zeros, float threshold,
+                    int bin_count, int quant_alg, or_error_acc *acc);
+/* the sorted |x| of the transformed data (getSorted's array), n entries */
+int or_opt_sorted(const void *in, int dtype, const int orig[3], const int steps[3],
+                  int wavelet, float *sorted_out, int64_t cap);
 
 #endif

@@ -85,6 +85,9 @@ def _declare(L):
     L.or_padded_length.argtypes = [I, I]
     L.or_set_steps.argtypes = [P, P]
     L.or_split.argtypes = [P, P, I, P, I, P, I, P]
+    L.or_opt_test.restype = C.c_int64
+    L.or_opt_test.argtypes = [P, I, P, P, I, I, I, F, I, I, C.POINTER(ErrorAcc)]
+    L.or_opt_sorted.argtypes = [P, I, P, P, I, P, C.c_int64]
     L.or_pad_convert.argtypes = [P, I, P, P, P]
     for f in ("or_cdf97_fwd", "or_cdf97_inv", "or_haar_fwd", "or_haar_inv"):
         getattr(L, f).argtypes = [I, P, I, P]
@@ -281,6 +284,28 @@ def huff_decode(table, dup_key, dup_value, words, count):
     out = np.zeros(count, np.int32)
     n = lib().or_huff_decode(_p(t), len(t), dup_key, dup_value, _p(w), len(w),
                              _p(out), count)
+    return out[:n]
+
+
+def opt_test(vol, steps, wavelet, max_possible, compress_zeros, threshold, bins, alg):
+    """One parameter-optimisation test point: (size bytes, mse, psnr, acc)."""
+    v = np.ascontiguousarray(vol)
+    orig = [v.shape[2], v.shape[1], v.shape[0]]
+    acc = ErrorAcc()
+    nb = lib().or_opt_test(_p(v), dtype_id(v), _p(i3(orig)), _p(i3(steps)), wavelet,
+                           max_possible, int(compress_zeros), F(threshold), bins, alg,
+                           C.byref(acc))
+    return nb, lib().or_error_mse(C.byref(acc)), lib().or_error_psnr(C.byref(acc)), acc
+
+
+def opt_sorted(vol, steps, wavelet):
+    """The ascending |x| of the transformed volume."""
+    v = np.ascontiguousarray(vol)
+    orig = [v.shape[2], v.shape[1], v.shape[0]]
+    cap = int(np.prod(padded_size(orig, set_steps(steps, orig))))
+    out = np.empty(cap, np.float32)
+    n = lib().or_opt_sorted(_p(v), dtype_id(v), _p(i3(orig)), _p(i3(steps)), wavelet,
+                            _p(out), cap)
     return out[:n]
 
 

@@ -162,3 +162,22 @@ def test_huffman_golden_vectors():
     words, bits = h.encode()
     assert np.array_equal(words, g["words"])
     assert np.array_equal(h.decode_table(), g["decode"])
+
+
+def test_total_encoded_length_matches_written_payload():
+    """totalEncodedLengthBytes (huffman.cc:241-252) against the bytes the
+    coder actually writes: without dup strings every bit is a code bit, so
+    the two agree exactly; with them the payload also holds 3 length bits
+    per dup string, so it is at least as long."""
+    rng = np.random.default_rng(4)
+    vol = (rng.standard_normal((16, 24, 32)) * 30).astype(np.float32)
+    for zeros in (False, True):
+        res = O.compress(vol, 2, 2, 256, [3, 3, 3], 0.9, compress_zeros=zeros)
+        t = res["res"].threshold
+        size, _, _, _ = O.opt_test(vol, [3, 3, 3], 2, 100, zeros, t, 256, 2)
+        _, cubes, _, _ = __import__("_cubeproto").parse_cube_file(res["file"])
+        payload = len(cubes[0][2])
+        if zeros:
+            assert 0 < size <= payload
+        else:
+            assert size == payload

@@ -93,6 +93,12 @@ class SplitBox(C.Structure):
     _fields_ = [("offset", C.c_int * 3), ("size", C.c_int * 3)]
 
 
+class OptResult(C.Structure):
+    _fields_ = [("size_bytes", C.c_int64), ("l1", C.c_float), ("l2", C.c_float),
+                ("mse", C.c_float), ("psnr", C.c_float), ("rel", C.c_float),
+                ("zero_bin", C.c_int)]
+
+
 class SplitInfo(C.Structure):
     _fields_ = [("cubelets", C.c_int), ("need", C.c_int64), ("ms_plan", C.c_double),
                 ("ms_batch", C.c_double), ("ms_encode", C.c_double), ("ms_total", C.c_double)]
@@ -178,6 +184,18 @@ def _declare(L):
         L.wv_split_compress.restype = I64
         L.wv_split_compress.argtypes = [P, C.c_int, P, P, C.c_int, C.POINTER(Options), C.c_int,
                                         P, I64, C.POINTER(SplitInfo), P]
+    if hasattr(L, "wv_opt_create"):
+        L.wv_opt_create.argtypes = [P, C.c_int, P, P, C.c_int, C.c_int, C.c_int,
+                                    C.POINTER(C.c_void_p), P]
+        L.wv_opt_count.restype = I64
+        L.wv_opt_count.argtypes = [P]
+        L.wv_opt_stats.argtypes = [P, P, P, P]
+        L.wv_opt_sorted.restype = C.c_float
+        L.wv_opt_sorted.argtypes = [P, I64]
+        L.wv_opt_find_sorted.restype = I64
+        L.wv_opt_find_sorted.argtypes = [P, C.c_float]
+        L.wv_opt_test.argtypes = [P, C.c_float, C.c_int, C.c_int, C.POINTER(OptResult), P]
+        L.wv_opt_destroy.argtypes = [P]
     if hasattr(L, "wv_huff_decode_device"):
         L.wv_huff_decode_ws_bytes.restype = I64
         L.wv_huff_decode_ws_bytes.argtypes = [I64, C.c_int]
@@ -597,6 +615,75 @@ def write_cube_raw(vol):
     n = lib().wv_write_cube_raw(size, dt, v.ctypes.data_as(P), buf.ctypes.data_as(P), cap)
     check(n if n < 0 else 0)
     return buf[:n].tobytes()
+
+
+class Opt:
+    """Parameter optimisation data (wv_opt_*): OptimizationData and
+    testParameters of the reference's -opt (optimize.h, optimize_gpu.cu).
+    vol: cuda tensor [z, y, x], kept alive by this object."""
+
+    def __init__(self, vol, steps, wavelet, max_possible=0, compress_zeros=True, stream=None):
+        if not vol.is_contiguous():
+            raise ValueError("Opt: the volume must be contiguous")
+        self.vol = vol
+        self.h = C.c_void_p()
+        orig = [vol.shape[2], vol.shape[1], vol.shape[0]]
+        check(lib().wv_opt_create(_ptr(vol), dtype_id(vol), _i3(orig), _i3(steps), wavelet,
+                                  max_possible, int(compress_zeros), C.byref(self.h),
+                                  _stream(stream)))
+
+    def count(self):
+        return check_size(lib().wv_opt_count(self.h))
+
+    def stats(self):
+        v = [C.c_float() for _ in range(3)]
+        check(lib().wv_opt_stats(self.h, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def sorted(self, index):
+        """getSorted: element index of the ascending |x| (0 out of range)."""
+        return lib().wv_opt_sorted(self.h, int(index))
+
+    def find_sorted(self, value):
+        """findSorted: first index whose |x| >= value."""
+        return check_size(lib().wv_opt_find_sorted(self.h, C.c_float(value)))
+
+    def test(self, threshold, bins, alg=LOG, stream=None):
+        """testParameters: OptResult (size_bytes, l1, l2, mse, psnr, rel)."""
+        r = OptResult()
+        check(lib().wv_opt_test(self.h, C.c_float(threshold), int(bins), int(alg), C.byref(r),
+                                _stream(stream)))
+        return r
+
+    def close(self):
+        if self.h:
+            lib().wv_opt_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def opt_plot_all(o, alg=LOG, stream=None):
+    """plotAll (optimize.cc:99-122), the reference's default optimisation:
+    for threshold fractions .5, .51, ... (float steps, below .999) and bin
+    counts 100, 110, 121, ... (x1.1 truncated, up to 2000) the size and PSNR
+    of each point; rows (frac, bins, size, psnr)."""
+    rows = []
+    n = o.count()
+    frac = np.float32(0.5)
+    while frac < np.float32(0.999):
+        t = o.sorted(int(np.float32(n) * frac))
+        b = 100
+        while b <= 2000:
+            r = o.test(t, b, alg, stream)
+            rows.append((float(frac), b, int(r.size_bytes), float(r.psnr)))
+            b = int(b * 1.1)
+        frac = np.float32(frac + np.float32(0.01))
+    return rows
 
 
 def split_plan(size, cub, overlap):

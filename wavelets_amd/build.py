@@ -28,7 +28,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-munsafe-fp-atomics"]
 
 SOURCES = ["wv_dwt.hip", "wv_select.hip", "wv_quant.hip", "wv_lloyd.hip", "wv_encode.hip", "wv_decode.hip", "wv_restore.hip",
            "wv_api.cpp", "wv_compress.cpp", "wv_params.cpp", "wv_huffman.cpp",
-           "wv_batch.cpp", "wv_serve.cpp", "wv_cube.cpp", "wv_split.cpp"]
+           "wv_batch.cpp", "wv_serve.cpp", "wv_cube.cpp", "wv_split.cpp", "wv_opt.hip"]
 
 
 def _compile(src):

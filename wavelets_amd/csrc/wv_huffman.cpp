@@ -357,6 +357,19 @@ std::string compressed_cubelet_msg(const wv_cube_desc *d, const wv_huff *book, u
   const std::string params = wvh::params_msg(d, book->coder, dup_key, dup);
   return wvh::cubelet_msg(d, params, nbytes, data_offset, offset);
 }
+
+// Huffman::totalEncodedLengthBytes (huffman.cc:241-252): every value's code
+// length times its count (the dup value's count being its singles), plus
+// the dup key's code per dup string -- its 3 length bits are not counted
+// -- rounded up to whole 32-bit words (summed in 64 bits)
+int64_t huff_total_encoded_bytes(const uint32_t *counts, int bin_count, int dup,
+                                 int64_t singles, int64_t dups) {
+  const wvh::Coder c(counts, bin_count, dup >= 0 && dup < bin_count ? dup : -1, singles, dups);
+  int64_t total = 0;
+  for (int v = 0; v < bin_count; v++) total += (int64_t)c.code(v).len * c.freq(v);
+  if (c.dup() >= 0) total += dups * (int64_t)c.code(bin_count).len;
+  return (total + 31) / 32 * 4;
+}
 }  // namespace wv
 
 extern "C" int64_t wv_write_cube(const wv_cube_desc *d, const int32_t *bins,

@@ -174,6 +174,11 @@ void cube_desc_of(const wv_geom &g, const wv_options *opt, int dtype, const wv_t
 
 void set_error(const char *fmt, ...);
 
+// Huffman::totalEncodedLengthBytes of a histogram (counts: bin_count
+// entries; dup < 0: no dup strings)
+int64_t huff_total_encoded_bytes(const uint32_t *counts, int bin_count, int dup,
+                                 int64_t singles, int64_t dups);
+
 // the CubeletBuffer message of a compressed cubelet at a box offset
 std::string compressed_cubelet_msg(const wv_cube_desc *d, const wv_huff *book, uint32_t nbytes,
                                    uint64_t data_offset, const int offset[3]);
