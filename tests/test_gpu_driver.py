@@ -116,3 +116,35 @@ def test_driver_decompress_matches_oracle(tmp_path, shape, dtype, wave, qalg, bi
     assert r.returncode == 0, r.stderr
     want = wv.write_cube_raw(oracle_decompress(comp))
     assert open(dst, "rb").read() == want
+
+
+def test_driver_opt_table_matches_oracle(tmp_path):
+    """-opt prints plotAll's table (optimize.cc:99-122): threshold fraction,
+    bin count, size and pSNR of the log quantizer; no output file.  Rows
+    spot-checked against the oracle's test points."""
+    vol = _vol((8, 32, 48), np.uint8, 11)
+    src, dst = str(tmp_path / "in.cube"), str(tmp_path / "out.cube")
+    _write_input_cube(src, vol)
+    r = subprocess.run([DRIVER, "-opt", "-steps", "3", src, dst], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert not os.path.exists(dst)
+    lines = r.stdout.strip().splitlines()
+    head = lines.index("Threshold\tBins\tsize\tpSNR")
+    rows = [l.split("\t") for l in lines[head + 1:]]
+    assert len(rows) == 50 * 32
+    srt = O.opt_sorted(vol, [3, 3, 3], O.CDF97)
+    n = np.float32(srt.size)
+    frac = np.float32(0.5)
+    fracs = []
+    while frac < np.float32(0.999):
+        fracs.append(frac)
+        frac = np.float32(frac + np.float32(0.01))
+    for k in (0, 5, 31, 32 * 17 + 3, len(rows) - 1):
+        f = fracs[k // 32]
+        assert rows[k][0] == "%.3f" % f
+        b = int(rows[k][1])
+        t = srt[int(n * f)]
+        size, _, psnr, _ = O.opt_test(vol, [3, 3, 3], O.CDF97, 0, True, t, b, O.LOG)
+        assert int(rows[k][2]) == size, k
+        assert abs(float(rows[k][3]) - psnr) <= 1e-4 * abs(psnr), k

@@ -198,6 +198,8 @@ void usage() {
           "\n  wv_compress_gpu [opts] <input.cube> <output.cube>\n"
           "    -steps <n|x,y|x,y,z> -thresh <f> -qcount <n> -wave <haar|cdf97>\n"
           "    -qalg <uniform|log|lloyd> -noz -err -lloyd-iters <n> -q\n"
+          "    -opt : parameter optimisation (the reference's default sweep,\n"
+          "           threshold fraction x log bin count -> size, pSNR); no output file\n"
           "  wv_compress_gpu -d [-q] <compressed.cube> <output.cube>\n\n");
   exit(1);
 }
@@ -224,6 +226,48 @@ double now_ms() {
              std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+// -opt: the device OptimizationData of the input, then plotAll
+// (optimize.cc:99-122, the reference's default optimizeParameters): for
+// threshold fractions .5, .51, ... (float steps, below .999) and bin counts
+// 100, 110, 121, ... ((int)(n * 1.1), up to 2000) the log quantizer's size
+// and pSNR.  Like the reference, no compressed file is written.
+int optimize(const InputCube &cube, const wv_options &opt, bool quiet) {
+  const int *orig = cube.size;
+  const int maxp = cube.max_value > 0 ? cube.max_value : 0;
+  if (cube.dtype != WV_DTYPE_UINT8 && maxp <= 0) {
+    fprintf(stderr, "Parameter optimisation needs maximum_value in the input cubelet.\n");
+    return 1;
+  }
+  void *in_d = nullptr;
+  HOK(hipMalloc(&in_d, cube.data.size()));
+  HOK(hipMemcpy(in_d, cube.data.data(), cube.data.size(), hipMemcpyHostToDevice));
+  const double t0 = now_ms();
+  wv_opt *o = nullptr;
+  WOK(wv_opt_create(in_d, cube.dtype, orig, opt.steps, opt.wavelet, maxp, opt.compress_zeros,
+                    &o, nullptr));
+  const double t1 = now_ms();
+  const int64_t n = wv_opt_count(o);
+  printf("Threshold\tBins\tsize\tpSNR\n");
+  int points = 0;
+  for (float frac = .5f; frac < .999f; frac += 0.01f) {
+    const float t = wv_opt_sorted(o, (int64_t)(int)((float)n * frac));
+    for (int bins = 100; bins <= 2000; bins = (int)(bins * 1.1)) {
+      wv_opt_result r;
+      WOK(wv_opt_test(o, t, bins, WV_QUANT_LOG, &r, nullptr));
+      printf("%.3f\t%d\t%lld\t%.6g\n", frac, bins, (long long)r.size_bytes, r.psnr);
+      points++;
+    }
+  }
+  fflush(stdout);
+  const double t2 = now_ms();
+  if (!quiet)
+    fprintf(stderr, "transform + sort %.2f ms; %d test points in %.1f ms (%.3f ms each)\n",
+            t1 - t0, points, t2 - t1, (t2 - t1) / points);
+  wv_opt_destroy(o);
+  (void)hipFree(in_d);
+  return 0;
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -237,7 +281,7 @@ int main(int argc, char **argv) {
   opt.compress_zeros = 1;
   opt.lloyd_max_iters = -1;
   opt.with_footer = 1;                     // the GPU driver writes the index
-  bool quiet = false, do_err = false, do_decompress = false;
+  bool quiet = false, do_err = false, do_decompress = false, do_opt = false;
   int a = 1;
   for (; a < argc && argv[a][0] == '-'; a++) {
     const char *s = argv[a];
@@ -277,6 +321,8 @@ int main(int argc, char **argv) {
       opt.compress_zeros = 0;
     } else if (!strcmp(s, "-err")) {
       do_err = true;
+    } else if (!strcmp(s, "-opt")) {
+      do_opt = true;
     } else if (!strcmp(s, "-lloyd-iters")) {
       if (++a >= argc) usage();
       opt.lloyd_max_iters = atoi(argv[a]);
@@ -305,6 +351,8 @@ int main(int argc, char **argv) {
   WOK(wv_geom_init(&g, orig, opt.steps, opt.wavelet, cube.dtype));
   if (!quiet)
     printf("Read %dx%dx%d data file: %.2f ms\n", orig[0], orig[1], orig[2], now_ms() - t0);
+
+  if (do_opt) return optimize(cube, opt, quiet);
 
   void *in_d = nullptr, *work_d = nullptr;
   const int64_t work_bytes = wv_compress_work_bytes(orig, &opt);
