@@ -54,10 +54,11 @@ def parse():
                     help="first run a plain device copy of 4 N bytes (N = samples of the "
                          "single cubelet) a few times: the reference kernel for scaling "
                          "FETCH_SIZE / WRITE_SIZE counters")
-    ap.add_argument("--extras", default="e2e,serving,split",
+    ap.add_argument("--extras", default="e2e,serving,split,opt",
                     help="comma list: e2e (single-cubelet one-call), serving, split "
                          "(the single cubelet's volume split into overlapping cubelets, "
-                         "all compressed by the batched path)")
+                         "all compressed by the batched path), opt (the -opt sweep on "
+                         "the 2048^2 u8 image)")
     ap.add_argument("--split-cube", type=int, default=64, help="split: cubelet edge")
     ap.add_argument("--split-overlap", type=int, default=8, help="split: overlap")
     ap.add_argument("--serve-count", type=int, default=300, help="requests of the serving run")
@@ -426,6 +427,38 @@ def split_leg(args, wv, torch, dev, vol):
     return out
 
 
+def opt_leg(args, wv, torch, dev):
+    """-opt on the 2048 x 2048 u8 image (CDF97 11,11,0): the optimisation
+    data (transform + device sort) once, then the reference's default
+    plotAll sweep of 50 threshold fractions x 32 log bin counts."""
+    from wavelets_amd import synth
+    img = synth.smooth_field_u8(2048, 2048, seed=1)[None, :, :]
+    vd = torch.from_numpy(np.ascontiguousarray(img)).to(dev)
+    o = wv.Opt(vd, [11, 11, 0], wv.CDF97)
+    o.test(o.sorted(o.count() // 2), 256, wv.LOG)      # warm
+    torch.cuda.synchronize()
+    t0 = time.time()
+    o2 = wv.Opt(vd, [11, 11, 0], wv.CDF97)
+    torch.cuda.synchronize()
+    ms_create = (time.time() - t0) * 1e3
+    t0 = time.time()
+    rows = wv.opt_plot_all(o2)
+    ms_sweep = (time.time() - t0) * 1e3
+    best = max(rows, key=lambda r: r[3] - 1e-6 * r[2])
+    log("opt: create %.2f ms, %d points in %.1f ms (%.3f ms each)"
+        % (ms_create, len(rows), ms_sweep, ms_sweep / len(rows)))
+    o.close()
+    o2.close()
+    return {
+        "workload": "2048x2048 u8 image, CDF97 11,11,0, plotAll: 50 threshold fractions x 32 "
+                    "log bin counts (test point = quantize, coder size, dequantize, inverse, "
+                    "error sums)",
+        "ms_create": round(ms_create, 3), "points": len(rows),
+        "ms_sweep": round(ms_sweep, 2), "ms_per_point": round(ms_sweep / len(rows), 4),
+        "best_psnr_row": list(best),
+    }
+
+
 def serving(args, wv, torch, dev):
     """The serving loop on a Poisson request stream at a fixed offered rate.
     Requests: edges lognormal (median 128, sigma 0.6) clipped to [32, 512],
@@ -514,6 +547,9 @@ def main():
     split = None
     if "split" in args.extras:
         split = split_leg(args, wv, torch, dev, vol)
+    opt = None
+    if "opt" in args.extras:
+        opt = opt_leg(args, wv, torch, dev)
 
     cpu = None
     if not args.no_cpu:
@@ -553,6 +589,8 @@ def main():
         line["serving"] = serve
     if split is not None:
         line["split"] = split
+    if opt is not None:
+        line["opt"] = opt
     print(json.dumps(line), flush=True)
 
 
