@@ -45,24 +45,26 @@ def _fields(msg):
     return text_format.MessageToString(msg).splitlines()
 
 
-@pytest.mark.parametrize("shape,dtype,cub,overlap,wave,alg,bins,frac", [
-    ((28, 36, 40), np.uint8, [16, 16, 16], 4, 2, 2, 256, 0.9),
-    ((20, 24, 33), np.float32, [16, 12, 8], 3, 1, 1, 1024, 0.95),
-    ((9, 30, 30), np.int16, [16, 16, 6], 0, 2, 2, 256, 0.8),
+@pytest.mark.parametrize("shape,dtype,cub,overlap,wave,alg,bins,frac,zeros", [
+    ((28, 36, 40), np.uint8, [16, 16, 16], 4, 2, 2, 256, 0.9, True),
+    ((20, 24, 33), np.float32, [16, 12, 8], 3, 1, 1, 1024, 0.95, True),
+    ((9, 30, 30), np.int16, [16, 16, 6], 0, 2, 2, 256, 0.8, True),
+    ((24, 40, 40), np.float32, [32, 32, 24], 6, 2, 1, 256, 0.9, False),   # no dup strings
+    ((16, 48, 64), np.float32, [64, 48, 16], 5, 2, 1, 8192, 0.5, True),   # code book past LDS
 ])
 def test_split_compress_records_match_oracle(wv, shape, dtype, cub, overlap, wave, alg, bins,
-                                             frac):
+                                             frac, zeros):
     vol = _vol(shape, dtype, 4)
-    print("case", shape, dtype.__name__, cub, overlap, flush=True)
+    print("case", shape, dtype.__name__, cub, overlap, bins, zeros, flush=True)
     f, info = wv.split_compress(torch.from_numpy(vol).cuda(), cub, overlap, wave, [3, 3, 3],
-                                alg, bins, frac)
+                                alg, bins, frac, compress_zeros=zeros)
     ov = vol.astype(np.int32) if dtype == np.int16 else vol
     boxes, datas = O.split([shape[2], shape[1], shape[0]], cub, overlap, ov)
     _, cubes, index, _ = P.parse_cube_file(f)
     assert info.cubelets == len(boxes) == len(cubes) == len(index.cubelets)
     for k, ((cb, raw, payload), (off, s), d) in enumerate(zip(cubes, boxes, datas)):
         assert (cb.x_offset, cb.y_offset, cb.z_offset) == off, k
-        want = O.compress(d, wave, alg, bins, [3, 3, 3], frac)["file"]
+        want = O.compress(d, wave, alg, bins, [3, 3, 3], frac, compress_zeros=zeros)["file"]
         _, wcubes, _, _ = P.parse_cube_file(want)
         wcb, wraw, wpayload = wcubes[0]
         assert payload == wpayload, k

@@ -54,24 +54,23 @@ namespace wv {
                                    // by the whole block
 #define ENC_MAX_RANGES 2048        // blocks per launch (8 per CU)
 
-struct EncArgs {
-  const int32_t *bins;
-  int64_t n;
-  int64_t nchunks;
-  int zb;                    // duplicate value; -1: no duplicate coding
-  int nb1;                   // code-book entries (the dup key is the last)
-  int vec;                   // bins are 16-byte aligned
-  int nranges;               // blocks; block b owns chunks [b per, (b+1) per)
-  int64_t per;
-  int64_t *range_last;       // per range: its last non-duplicate index, -1
-  uint64_t *range_bits;      // per range: output bits
-  unsigned long long *range_sd;   // per range: single codes, dup codes
-  uint32_t *range_hist;      // per range: histogram of the coded values
-  const uint64_t *book;      // codes: bits | length << 56
-  uint32_t *words;
-  uint64_t nbits_cap;        // bits the payload buffer holds
-  unsigned long long *sd;    // [0] single codes of zb, [1] dup codes
-};
+// EncArgs (one coding job) is declared in wv_internal.h
+
+
+// A launch over many jobs: block b serves the last job j with blk0[j] <= b
+// (job tables on the device); a single job is passed by value with blk0 0.
+__device__ inline int enc_job(const int *blk0, int njobs, int b) {
+  int lo = 0, hi = njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (blk0[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+// this block's range within its job
+__device__ inline int range_id(const EncArgs &A) { return (int)blockIdx.x - A.blk0; }
 
 __device__ inline int code_len(uint64_t c) { return (int)(c >> 56); }
 __device__ inline uint64_t code_bits(uint64_t c) { return c & ((1ull << 56) - 1); }
@@ -194,7 +193,7 @@ __device__ inline uint64_t block_reduce(const uint64_t *a, int m, uint64_t *red)
 }
 
 __device__ inline void range_of(const EncArgs &A, int64_t &cb, int64_t &ce) {
-  cb = (int64_t)blockIdx.x * A.per;
+  cb = (int64_t)range_id(A) * A.per;
   ce = cb + A.per < A.nchunks ? cb + A.per : A.nchunks;
 }
 
@@ -371,7 +370,9 @@ __device__ inline int clamp_bin(const EncArgs &A, int64_t v) {
 }
 
 // last: per range, its last non-duplicate bin, reading chunks backwards
-__global__ void __launch_bounds__(ENC_NT) enc_range_last(EncArgs A) {
+template <bool BATCH>
+__global__ void __launch_bounds__(ENC_NT) enc_range_last(EncArgs A0, const EncArgs *J, const int *JB, int nj) {
+  const EncArgs A = BATCH ? J[enc_job(JB, nj, (int)blockIdx.x)] : A0;
   __shared__ int first[ENC_NT];
   __shared__ int red[ENC_NW];
   int64_t cb, ce;
@@ -389,14 +390,15 @@ __global__ void __launch_bounds__(ENC_NT) enc_range_last(EncArgs A) {
       break;
     }
   }
-  if (threadIdx.x == 0) A.range_last[blockIdx.x] = found;
+  if (threadIdx.x == 0) A.range_last[range_id(A)] = found;
 }
 
 // count: run statistics per range and in total; with HIST also the
 // range's histogram of the other values (dynamic LDS of nb1 entries).
 // One barrier per chunk.
-template <bool HIST>
-__global__ void __launch_bounds__(ENC_NT) enc_count(EncArgs A) {
+template <bool HIST, bool BATCH>
+__global__ void __launch_bounds__(ENC_NT) enc_count(EncArgs A0, const EncArgs *J, const int *JB, int nj) {
+  const EncArgs A = BATCH ? J[enc_job(JB, nj, (int)blockIdx.x)] : A0;
   extern __shared__ uint32_t hist[];
   __shared__ int first[2][ENC_NT];
   __shared__ int red[2][ENC_NW];
@@ -404,7 +406,7 @@ __global__ void __launch_bounds__(ENC_NT) enc_count(EncArgs A) {
   if (HIST)
     for (int i = threadIdx.x; i < A.nb1; i += ENC_NT) hist[i] = 0;
   int64_t prev = A.zb >= 0 ? (int64_t)block_reduce<true>((const uint64_t *)A.range_last,
-                                                           blockIdx.x, red64)
+                                                           range_id(A), red64)
                            : -1;
   __syncthreads();
   uint64_t singles = 0, dups = 0;
@@ -429,14 +431,14 @@ __global__ void __launch_bounds__(ENC_NT) enc_count(EncArgs A) {
   const uint64_t S = block_all<false>(singles, red64);
   const uint64_t D = block_all<false>(dups, red64);
   if (threadIdx.x == 0) {
-    A.range_sd[2 * blockIdx.x] = S;
-    A.range_sd[2 * blockIdx.x + 1] = D;
+    A.range_sd[2 * range_id(A)] = S;
+    A.range_sd[2 * range_id(A) + 1] = D;
     if (S) atomicAdd(&A.sd[0], (unsigned long long)S);
     if (D) atomicAdd(&A.sd[1], (unsigned long long)D);
   }
   if (HIST)
     for (int i = threadIdx.x; i < A.nb1; i += ENC_NT)
-      A.range_hist[(int64_t)blockIdx.x * A.nb1 + i] = hist[i];
+      A.range_hist[(int64_t)range_id(A) * A.nb1 + i] = hist[i];
 }
 
 // Code book (LDS copy when it fits) and the two codes a run uses.
@@ -525,9 +527,11 @@ __device__ inline uint32_t thread_bits(const EncArgs &A, const Book &b, int64_t 
 }
 
 // bits per range from its histogram and run sums
-__global__ void __launch_bounds__(ENC_NT) enc_range_bits_hist(EncArgs A) {
+template <bool BATCH>
+__global__ void __launch_bounds__(ENC_NT) enc_range_bits_hist(EncArgs A0, const EncArgs *J, const int *JB, int nj) {
+  const EncArgs A = BATCH ? J[enc_job(JB, nj, (int)blockIdx.x)] : A0;
   __shared__ uint64_t red64[ENC_NW];
-  const int64_t b = blockIdx.x;
+  const int64_t b = range_id(A);
   uint64_t acc = 0;
   for (int v = threadIdx.x; v < A.nb1; v += ENC_NT)
     acc += (uint64_t)A.range_hist[b * A.nb1 + v] * (uint64_t)code_len(A.book[v]);
@@ -548,13 +552,15 @@ __global__ void __launch_bounds__(ENC_NT) enc_range_bits_hist(EncArgs A) {
 }
 
 // bits per range by walking it (code books too large for LDS)
-__global__ void __launch_bounds__(ENC_NT) enc_range_bits_walk(EncArgs A) {
+template <bool BATCH>
+__global__ void __launch_bounds__(ENC_NT) enc_range_bits_walk(EncArgs A0, const EncArgs *J, const int *JB, int nj) {
+  const EncArgs A = BATCH ? J[enc_job(JB, nj, (int)blockIdx.x)] : A0;
   __shared__ int first[ENC_NT];
   __shared__ int red[ENC_NW];
   __shared__ uint64_t red64[ENC_NW];
   const Book bk = load_book<false>(A, nullptr);
   int64_t prev = A.zb >= 0 ? (int64_t)block_reduce<true>((const uint64_t *)A.range_last,
-                                                           blockIdx.x, red64)
+                                                           range_id(A), red64)
                            : -1;
   uint64_t bits = 0;
   int64_t cb, ce;
@@ -570,7 +576,7 @@ __global__ void __launch_bounds__(ENC_NT) enc_range_bits_walk(EncArgs A) {
     if (tot >= 0) prev = c * ENC_C + tot;
   }
   const uint64_t t = block_all<false>(bits, red64);
-  if (threadIdx.x == 0) A.range_bits[blockIdx.x] = t;
+  if (threadIdx.x == 0) A.range_bits[range_id(A)] = t;
 }
 
 // q copies of the u-bit string unit from bit p of dst, one word per thread
@@ -700,8 +706,9 @@ __device__ inline void emit_bins(const EncArgs &A, const Book &b, int64_t prev, 
 // offsets, one after coding into the staged words (plus one when a long
 // run's dup codes are written by the block); staged words are cleared as
 // they are copied out, so the next chunk finds them zero.
-template <bool TAB_LDS, bool PREFETCH>
-__global__ void __launch_bounds__(ENC_NT) enc_write(EncArgs A) {
+template <bool TAB_LDS, bool PREFETCH, bool BATCH>
+__global__ void __launch_bounds__(ENC_NT) enc_write(EncArgs A0, const EncArgs *J, const int *JB, int nj) {
+  const EncArgs A = BATCH ? J[enc_job(JB, nj, (int)blockIdx.x)] : A0;
   extern __shared__ uint64_t tab[];
   __shared__ uint32_t stage[ENC_STAGE];
   __shared__ int first[2][ENC_NT];
@@ -718,9 +725,9 @@ __global__ void __launch_bounds__(ENC_NT) enc_write(EncArgs A) {
   for (int i = threadIdx.x; i < ENC_STAGE; i += ENC_NT) stage[i] = 0;
   if (threadIdx.x == 0) long_q = 0;
   int64_t prev = A.zb >= 0 ? (int64_t)block_reduce<true>((const uint64_t *)A.range_last,
-                                                           blockIdx.x, red64)
+                                                           range_id(A), red64)
                            : -1;
-  uint64_t off = block_reduce<false>(A.range_bits, blockIdx.x, red64);
+  uint64_t off = block_reduce<false>(A.range_bits, range_id(A), red64);
   int64_t cb, ce;
   range_of(A, cb, ce);
   for_chunks<PREFETCH>(A, cb, ce, [&](int64_t c, const Chunk &k, int par) {
@@ -825,7 +832,9 @@ __device__ inline void walk_events(const EncArgs &A, const uint64_t *ctab, int r
   }
 }
 
-__global__ void __launch_bounds__(ENC_NT) enc_write_fast(EncArgs A) {
+template <bool BATCH>
+__global__ void __launch_bounds__(ENC_NT) enc_write_fast(EncArgs A0, const EncArgs *J, const int *JB, int nj) {
+  const EncArgs A = BATCH ? J[enc_job(JB, nj, (int)blockIdx.x)] : A0;
   extern __shared__ uint64_t ctab[];
   __shared__ uint32_t stage[ENC_STAGE];
   __shared__ int first[2][ENC_NT];
@@ -864,9 +873,9 @@ __global__ void __launch_bounds__(ENC_NT) enc_write_fast(EncArgs A) {
   for (int i = threadIdx.x; i < ENC_STAGE; i += ENC_NT) stage[i] = 0;
   if (threadIdx.x == 0) long_q = 0;
   int64_t prev = A.zb >= 0 ? (int64_t)block_reduce<true>((const uint64_t *)A.range_last,
-                                                           blockIdx.x, red64)
+                                                           range_id(A), red64)
                            : -1;
-  uint64_t off = block_reduce<false>(A.range_bits, blockIdx.x, red64);
+  uint64_t off = block_reduce<false>(A.range_bits, range_id(A), red64);
   int64_t cb, ce;
   range_of(A, cb, ce);
   for_chunks<false>(A, cb, ce, [&](int64_t c, const Chunk &k, int par) {
@@ -938,6 +947,20 @@ __global__ void enc_zero(uint32_t *p, int64_t n) {
 
 __global__ void enc_zero64(unsigned long long *p, int n) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
+}
+
+// batches: the run-statistics of job j go to sd[2j..2j+1]; its histogram
+// (counts[j], m entries) is packed into dst + j * m
+__global__ void enc_zero_sd_batch(const EncArgs *J, int nj) {
+  for (int j = blockIdx.x * blockDim.x + threadIdx.x; j < nj; j += gridDim.x * blockDim.x) {
+    J[j].sd[0] = 0;
+    J[j].sd[1] = 0;
+  }
+}
+
+__global__ void enc_gather_counts(const uint32_t *const *src, int m, uint32_t *dst) {
+  const uint32_t *c = src[blockIdx.x];
+  for (int i = threadIdx.x; i < m; i += blockDim.x) dst[(int64_t)blockIdx.x * m + i] = c[i];
 }
 
 // ---- host side ----------------------------------------------------------
@@ -1017,11 +1040,11 @@ hipError_t enc_count_runs(const int32_t *bins, int64_t n, int zb, int bin_count,
   const EncArgs A = enc_args(bins, n, zb, ws, bin_count, L);
   hipLaunchKernelGGL(enc_zero64, dim3(1), dim3(64), 0, s, A.sd, 2);
   if (zb >= 0)
-    hipLaunchKernelGGL(enc_range_last, dim3(L.nranges), dim3(ENC_NT), 0, s, A);
+    hipLaunchKernelGGL(enc_range_last<false>, dim3(L.nranges), dim3(ENC_NT), 0, s, A, nullptr, nullptr, 0);
   if (L.hist)
-    hipLaunchKernelGGL(enc_count<true>, dim3(L.nranges), dim3(ENC_NT), A.nb1 * 4, s, A);
+    hipLaunchKernelGGL((enc_count<true, false>), dim3(L.nranges), dim3(ENC_NT), A.nb1 * 4, s, A, nullptr, nullptr, 0);
   else if (zb >= 0)
-    hipLaunchKernelGGL(enc_count<false>, dim3(L.nranges), dim3(ENC_NT), 0, s, A);
+    hipLaunchKernelGGL((enc_count<false, false>), dim3(L.nranges), dim3(ENC_NT), 0, s, A, nullptr, nullptr, 0);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   e = hipMemcpyAsync(sd_host, A.sd, 16, hipMemcpyDeviceToHost, s);
@@ -1043,20 +1066,139 @@ hipError_t enc_write_payload(const int32_t *bins, int64_t n, int zb, int bin_cou
   hipLaunchKernelGGL(enc_zero, dim3((unsigned)zg), dim3(256), 0, s, words, nwords);
   const bool pf = enc_prefetch();
   if (L.hist && short_runs && !pf) {
-    hipLaunchKernelGGL(enc_range_bits_hist, dim3(L.nranges), dim3(ENC_NT), 0, s, A);
+    hipLaunchKernelGGL(enc_range_bits_hist<false>, dim3(L.nranges), dim3(ENC_NT), 0, s, A, nullptr, nullptr, 0);
     const size_t lds = (size_t)(((nb1 + 1) & ~1) + 2048) * 8;
-    hipLaunchKernelGGL(enc_write_fast, dim3(L.nranges), dim3(ENC_NT), lds, s, A);
+    hipLaunchKernelGGL(enc_write_fast<false>, dim3(L.nranges), dim3(ENC_NT), lds, s, A, nullptr, nullptr, 0);
   } else if (L.hist) {
-    hipLaunchKernelGGL(enc_range_bits_hist, dim3(L.nranges), dim3(ENC_NT), 0, s, A);
+    hipLaunchKernelGGL(enc_range_bits_hist<false>, dim3(L.nranges), dim3(ENC_NT), 0, s, A, nullptr, nullptr, 0);
     if (pf)
-      hipLaunchKernelGGL((enc_write<true, true>), dim3(L.nranges), dim3(ENC_NT),
-                         (size_t)nb1 * 8, s, A);
+      hipLaunchKernelGGL((enc_write<true, true, false>), dim3(L.nranges), dim3(ENC_NT),
+                         (size_t)nb1 * 8, s, A, nullptr, nullptr, 0);
     else
-      hipLaunchKernelGGL((enc_write<true, false>), dim3(L.nranges), dim3(ENC_NT),
-                         (size_t)nb1 * 8, s, A);
+      hipLaunchKernelGGL((enc_write<true, false, false>), dim3(L.nranges), dim3(ENC_NT),
+                         (size_t)nb1 * 8, s, A, nullptr, nullptr, 0);
   } else {
-    hipLaunchKernelGGL(enc_range_bits_walk, dim3(L.nranges), dim3(ENC_NT), 0, s, A);
-    hipLaunchKernelGGL((enc_write<false, false>), dim3(L.nranges), dim3(ENC_NT), 0, s, A);
+    hipLaunchKernelGGL(enc_range_bits_walk<false>, dim3(L.nranges), dim3(ENC_NT), 0, s, A, nullptr, nullptr, 0);
+    hipLaunchKernelGGL((enc_write<false, false, false>), dim3(L.nranges), dim3(ENC_NT), 0, s, A, nullptr, nullptr, 0);
+  }
+  return hipGetLastError();
+}
+
+// ---- batches of jobs with one code-book size --------------------------
+// ws: [job table][blk0][sd pairs][count pointers][packed counts][packed
+// books][per-job workspaces].  Every phase is one launch over all jobs'
+// ranges (block b serves the job with the last blk0 <= b).
+EncBatch enc_batch_plan(const int32_t *const *bins, const int64_t *n, const int *zb, int nj,
+                        int bin_count, void *ws) {
+  EncBatch P;
+  P.nj = nj;
+  P.bin_count = bin_count;
+  P.nb1 = nj > 0 && zb[0] >= 0 ? bin_count + 1 : bin_count;
+  P.hist = P.nb1 <= ENC_TAB;
+  int64_t o = 0;
+  P.o_jobs = o; o += al256((int64_t)nj * sizeof(EncArgs));
+  P.o_blk0 = o; o += al256((int64_t)nj * 4);
+  P.o_sd = o; o += al256((int64_t)nj * 16);
+  P.o_ptrs = o; o += al256((int64_t)nj * 8);
+  P.o_counts = o; o += al256((int64_t)nj * (bin_count + 1) * 4);
+  P.o_books = o; o += al256((int64_t)nj * P.nb1 * 8);
+  P.jobs.resize((size_t)nj);
+  P.blk0.resize((size_t)nj);
+  P.ok = true;
+  int blocks = 0;
+  char *w = (char *)ws;
+  for (int j = 0; j < nj; j++) {
+    const EncLayout L = enc_layout(n[j], bin_count);
+    EncArgs A = enc_args(bins[j], n[j], zb[j], w ? w + o : nullptr, bin_count, L);
+    if (A.nb1 != P.nb1) P.ok = false;   // every job with or every job without a dup key
+    A.sd = w ? (unsigned long long *)(w + P.o_sd) + 2 * j : nullptr;
+    A.book = w ? (const uint64_t *)(w + P.o_books) + (int64_t)j * P.nb1 : nullptr;
+    A.blk0 = blocks;
+    P.blk0[j] = blocks;
+    blocks += L.nranges;
+    P.jobs[j] = A;
+    o += al256(L.total);
+  }
+  P.blocks = blocks;
+  P.total = o;
+  return P;
+}
+
+hipError_t enc_batch_counts(EncBatch &P, const uint32_t *const *counts_d, void *ws,
+                            uint32_t *counts_host, unsigned long long *sd_host, hipStream_t s) {
+  if (!P.ok || P.nj <= 0) return hipErrorInvalidValue;
+  char *w = (char *)ws;
+  EncArgs *J = (EncArgs *)(w + P.o_jobs);
+  const int *B = (const int *)(w + P.o_blk0);
+  const int m = P.bin_count + 1;
+  hipError_t e = hipMemcpyAsync(J, P.jobs.data(), P.jobs.size() * sizeof(EncArgs),
+                                hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(w + P.o_blk0, P.blk0.data(), P.blk0.size() * 4, hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(w + P.o_ptrs, counts_d, (size_t)P.nj * 8, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return e;
+  const bool dup = P.jobs[0].zb >= 0;
+  hipLaunchKernelGGL(enc_zero_sd_batch, dim3((P.nj + 255) / 256), dim3(256), 0, s, J, P.nj);
+  if (dup)
+    hipLaunchKernelGGL(enc_range_last<true>, dim3(P.blocks), dim3(ENC_NT), 0, s, EncArgs{}, J, B,
+                       P.nj);
+  if (P.hist)
+    hipLaunchKernelGGL((enc_count<true, true>), dim3(P.blocks), dim3(ENC_NT), P.nb1 * 4, s,
+                       EncArgs{}, J, B, P.nj);
+  else if (dup)
+    hipLaunchKernelGGL((enc_count<false, true>), dim3(P.blocks), dim3(ENC_NT), 0, s, EncArgs{},
+                       J, B, P.nj);
+  hipLaunchKernelGGL(enc_gather_counts, dim3(P.nj), dim3(256), 0, s,
+                     (const uint32_t *const *)(w + P.o_ptrs), m, (uint32_t *)(w + P.o_counts));
+  e = hipGetLastError();
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(counts_host, w + P.o_counts, (size_t)P.nj * m * 4, hipMemcpyDeviceToHost,
+                       s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(sd_host, w + P.o_sd, (size_t)P.nj * 16, hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  return e;
+}
+
+hipError_t enc_batch_payload(EncBatch &P, void *ws, const uint64_t *books_host, bool short_runs,
+                             uint32_t *words, const int64_t *woff, const int64_t *nwords,
+                             int64_t total_words, hipStream_t s) {
+  if (!P.ok || P.nj <= 0) return hipErrorInvalidValue;
+  char *w = (char *)ws;
+  EncArgs *J = (EncArgs *)(w + P.o_jobs);
+  const int *B = (const int *)(w + P.o_blk0);
+  for (int j = 0; j < P.nj; j++) {
+    P.jobs[j].words = words + woff[j];
+    P.jobs[j].nbits_cap = (uint64_t)nwords[j] * 32u;
+  }
+  hipError_t e = hipMemcpyAsync(J, P.jobs.data(), P.jobs.size() * sizeof(EncArgs),
+                                hipMemcpyHostToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(w + P.o_books, books_host, (size_t)P.nj * P.nb1 * 8,
+                       hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return e;
+  int64_t zg = (total_words + 255) / 256;
+  if (zg > 4096) zg = 4096;
+  if (zg < 1) zg = 1;
+  hipLaunchKernelGGL(enc_zero, dim3((unsigned)zg), dim3(256), 0, s, words, total_words);
+  const int nb1 = P.nb1;
+  if (P.hist && short_runs) {
+    hipLaunchKernelGGL(enc_range_bits_hist<true>, dim3(P.blocks), dim3(ENC_NT), 0, s, EncArgs{},
+                       J, B, P.nj);
+    const size_t lds = (size_t)(((nb1 + 1) & ~1) + 2048) * 8;
+    hipLaunchKernelGGL(enc_write_fast<true>, dim3(P.blocks), dim3(ENC_NT), lds, s, EncArgs{}, J,
+                       B, P.nj);
+  } else if (P.hist) {
+    hipLaunchKernelGGL(enc_range_bits_hist<true>, dim3(P.blocks), dim3(ENC_NT), 0, s, EncArgs{},
+                       J, B, P.nj);
+    hipLaunchKernelGGL((enc_write<true, false, true>), dim3(P.blocks), dim3(ENC_NT),
+                       (size_t)nb1 * 8, s, EncArgs{}, J, B, P.nj);
+  } else {
+    hipLaunchKernelGGL(enc_range_bits_walk<true>, dim3(P.blocks), dim3(ENC_NT), 0, s, EncArgs{},
+                       J, B, P.nj);
+    hipLaunchKernelGGL((enc_write<false, false, true>), dim3(P.blocks), dim3(ENC_NT), 0, s,
+                       EncArgs{}, J, B, P.nj);
   }
   return hipGetLastError();
 }

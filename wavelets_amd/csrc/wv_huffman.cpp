@@ -14,6 +14,7 @@
 #include <cstring>
 #include <queue>
 #include <string>
+#include <thread>
 #include <vector>
 #include "wv_internal.h"
 
@@ -485,6 +486,145 @@ extern "C" int wv_encode_device(const int32_t *bins_d, int64_t n, const uint32_t
   else delete book;
   return WV_OK;
 }
+
+namespace wv {
+
+int64_t encode_batch_ws_bytes(const int64_t *n, int nj, int bin_count, int dup) {
+  std::vector<const int32_t *> bins((size_t)nj, nullptr);
+  std::vector<int> zb((size_t)nj, dup);
+  return enc_batch_plan(bins.data(), n, zb.data(), nj, bin_count, nullptr).total;
+}
+
+// Device Huffman coding of nj bins arrays with one bin count and one dup
+// value: the histograms and run statistics of all of them in one launch
+// per phase and one synchronisation, the code books on host threads, all
+// payloads in one launch per phase, one D2H.  The same books and words as
+// wv_encode_device gives each array alone.
+int encode_batch(const int32_t *const *bins_d, const int64_t *n, const uint32_t *const *counts_d,
+                 int nj, int bin_count, int dup, void *ws_d, int64_t ws_bytes, uint32_t *words_d,
+                 int64_t words_cap, std::vector<std::vector<uint32_t>> &words_host,
+                 std::vector<wv_huff *> &books, hipStream_t s) {
+  words_host.assign((size_t)nj, {});
+  books.assign((size_t)nj, nullptr);
+  if (nj <= 0) return WV_OK;
+  if (bin_count < 1 || bin_count > WV_MAX_BINS) {
+    set_error("encode_batch: %d bins", bin_count);
+    return WV_ERR_ARG;
+  }
+  dup = dup >= 0 && dup < bin_count ? dup : -1;
+  std::vector<int> zbs((size_t)nj, dup);
+  EncBatch P = enc_batch_plan(bins_d, n, zbs.data(), nj, bin_count, ws_d);
+  if (P.total > ws_bytes) {
+    set_error("encode_batch: workspace %lld < %lld", (long long)ws_bytes, (long long)P.total);
+    return WV_ERR_NOMEM;
+  }
+  const int m = bin_count + 1;
+  std::vector<uint32_t> counts((size_t)nj * m);
+  std::vector<unsigned long long> sd((size_t)nj * 2);
+  hipError_t e = enc_batch_counts(P, counts_d, ws_d, counts.data(), sd.data(), s);
+  if (e != hipSuccess) {
+    set_error("encode_batch: run statistics: %s", hipGetErrorString(e));
+    return WV_ERR_HIP;
+  }
+  // code books on host threads
+  const int nb1 = P.nb1;
+  std::vector<uint64_t> packed((size_t)nj * nb1);
+  std::vector<int64_t> nwords((size_t)nj), bits((size_t)nj);
+  std::vector<char> shorts((size_t)nj, 1);
+  std::vector<int> rcs((size_t)nj, WV_OK);
+  std::vector<std::string> errs((size_t)nj);
+  auto book_of = [&](int j) {
+    const uint32_t *cj = counts.data() + (size_t)j * m;
+    if (cj[bin_count] && dup < 0) {
+      rcs[j] = WV_ERR_ARG;
+      errs[j] = "bins outside [0, bin_count) and no dup key";
+      return;
+    }
+    wv_huff *book = new wv_huff(cj, bin_count, dup, (int64_t)sd[2 * j], (int64_t)sd[2 * j + 1]);
+    const wvh::Coder &c = book->coder;
+    if (c.leaves() != nb1 || c.longest() > 56) {
+      delete book;
+      rcs[j] = WV_ERR_ARG;
+      errs[j] = "code longer than 56 bits";
+      return;
+    }
+    uint64_t b = 0;
+    uint64_t *pk = packed.data() + (size_t)j * nb1;
+    for (int v = 0; v < nb1; v++) {
+      const wvh::Code &cd = c.code(v);
+      const uint64_t cb =
+          cd.len ? (cd.words[0] | (cd.len > 32 ? (uint64_t)cd.words[1] << 32 : 0)) : 0;
+      pk[v] = cb | ((uint64_t)cd.len << 56);
+      b += (uint64_t)c.freq(v) * (uint64_t)cd.len;
+    }
+    if (dup >= 0) {
+      b += 3ull * sd[2 * j + 1] + (uint64_t)cj[bin_count] * (uint64_t)c.code(bin_count).len;
+      const int u = c.code(bin_count).len + 3, zl = c.code(dup).len;
+      bool sr = 8 * zl <= 120;
+      for (int r = 9; r < 1024 && sr; r++) {
+        int hi = r >> 3, singles = r & 7;
+        if ((hi & 1) && singles == 0) { hi &= ~1; singles = 8; }
+        sr = __builtin_popcount(hi) * u + singles * zl <= 120;
+      }
+      shorts[j] = sr;
+    }
+    bits[j] = (int64_t)b;
+    nwords[j] = (int64_t)((b + 31) / 32);
+    book->payload_bits = (int64_t)b;
+    books[j] = book;
+  };
+  {
+    const int nt = std::max(1, std::min(16, nj / 8));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++)
+      pool.emplace_back([&, t] {
+        for (int j = t; j < nj; j += nt) book_of(j);
+      });
+    for (auto &th : pool) th.join();
+  }
+  for (int j = 0; j < nj; j++)
+    if (rcs[j] != WV_OK) {
+      for (auto *b : books) delete b;
+      books.assign((size_t)nj, nullptr);
+      set_error("encode_batch: job %d: %s", j, errs[j].c_str());
+      return rcs[j];
+    }
+  std::vector<int64_t> woff((size_t)nj);
+  int64_t total = 0;
+  bool all_short = true;
+  for (int j = 0; j < nj; j++) {
+    woff[j] = total;
+    total += nwords[j];
+    all_short = all_short && shorts[j];
+  }
+  if (total > words_cap) {
+    for (auto *b : books) delete b;
+    books.assign((size_t)nj, nullptr);
+    set_error("encode_batch: payloads need %lld words, buffer holds %lld", (long long)total,
+              (long long)words_cap);
+    return WV_ERR_NOMEM;
+  }
+  std::vector<uint32_t> all((size_t)total);
+  e = hipSuccess;
+  if (total > 0) {
+    e = enc_batch_payload(P, ws_d, packed.data(), all_short, words_d, woff.data(), nwords.data(),
+                          total, s);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(all.data(), words_d, (size_t)total * 4, hipMemcpyDeviceToHost, s);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);   // packed / all are locals
+  if (e != hipSuccess) {
+    for (auto *b : books) delete b;
+    books.assign((size_t)nj, nullptr);
+    set_error("encode_batch: payloads: %s", hipGetErrorString(e));
+    return WV_ERR_HIP;
+  }
+  for (int j = 0; j < nj; j++)
+    words_host[j].assign(all.begin() + woff[j], all.begin() + woff[j] + nwords[j]);
+  return WV_OK;
+}
+
+}  // namespace wv
 
 extern "C" int64_t wv_huff_payload_bits(const wv_huff *book) {
   return book ? book->payload_bits : -1;

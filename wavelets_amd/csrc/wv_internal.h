@@ -164,6 +164,55 @@ hipError_t enc_write_payload(const int32_t *bins, int64_t n, int zb, int bin_cou
                              bool short_runs, void *ws, uint32_t *words, int64_t nwords,
                              hipStream_t s);
 
+// batches of jobs with one bin count (every job with a dup key or every
+// job without): plan (job tables and workspace layout; ws may be null to
+// size it), histograms + run statistics of all jobs (one launch per phase,
+// one synchronisation), then all payloads from the packed host books
+// (nb1 codes per job) into words + woff[j]
+// one coding job: its bins, ranges and workspace tables
+struct EncArgs {
+  const int32_t *bins;
+  int64_t n;
+  int64_t nchunks;
+  int zb;                    // duplicate value; -1: no duplicate coding
+  int nb1;                   // code-book entries (the dup key is the last)
+  int vec;                   // bins are 16-byte aligned
+  int nranges;               // blocks; block b owns chunks [b per, (b+1) per)
+  int64_t per;
+  int64_t *range_last;       // per range: its last non-duplicate index, -1
+  uint64_t *range_bits;      // per range: output bits
+  unsigned long long *range_sd;   // per range: single codes, dup codes
+  uint32_t *range_hist;      // per range: histogram of the coded values
+  const uint64_t *book;      // codes: bits | length << 56
+  uint32_t *words;
+  uint64_t nbits_cap;        // bits the payload buffer holds
+  unsigned long long *sd;    // [0] single codes of zb, [1] dup codes
+  int blk0;                  // first block of this job in its launch (batches)
+};
+
+struct EncBatch {
+  int nj = 0, bin_count = 0, nb1 = 0, blocks = 0;
+  bool hist = false, ok = false;
+  int64_t o_jobs = 0, o_blk0 = 0, o_sd = 0, o_ptrs = 0, o_counts = 0, o_books = 0, total = 0;
+  std::vector<EncArgs> jobs;
+  std::vector<int> blk0;
+};
+EncBatch enc_batch_plan(const int32_t *const *bins, const int64_t *n, const int *zb, int nj,
+                        int bin_count, void *ws);
+hipError_t enc_batch_counts(EncBatch &P, const uint32_t *const *counts_d, void *ws,
+                            uint32_t *counts_host, unsigned long long *sd_host, hipStream_t s);
+hipError_t enc_batch_payload(EncBatch &P, void *ws, const uint64_t *books_host, bool short_runs,
+                             uint32_t *words, const int64_t *woff, const int64_t *nwords,
+                             int64_t total_words, hipStream_t s);
+
+// the whole batch coding (wv_huffman.cpp): books and host payload words
+// per job; ws of encode_batch_ws_bytes, words_d of the summed payloads
+int64_t encode_batch_ws_bytes(const int64_t *n, int nj, int bin_count, int dup);
+int encode_batch(const int32_t *const *bins_d, const int64_t *n, const uint32_t *const *counts_d,
+                 int nj, int bin_count, int dup, void *ws_d, int64_t ws_bytes, uint32_t *words_d,
+                 int64_t words_cap, std::vector<std::vector<uint32_t>> &words_host,
+                 std::vector<wv_huff *> &books, hipStream_t s);
+
 // after wv_compress_device on work_d: device Huffman coding of its bins,
 // payload copied to words_host (synchronises); and the container fields
 int encode_work(const wv_geom &g, const wv_options *opt, int zero_bin, void *work_d,

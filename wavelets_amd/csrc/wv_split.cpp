@@ -8,10 +8,10 @@
 // strides (no extraction copy), the whole set runs as one batch (grouped
 // persistent pass launches, batched select and quantize), each cubelet is
 // Huffman-coded on the device, and the payloads are written as one .cube
-// stream with the boxes' offsets.  The coding runs on a pool of host
-// threads, each with its own stream and workspace: one box's coding is a
-// short chain of small launches and synchronisations, so boxes are coded
-// side by side.
+// stream with the boxes' offsets.  All boxes are coded together: their
+// run statistics in one launch per phase, the code books on host threads,
+// all payloads in one launch per phase (a box coded alone is a chain of
+// small launches and two synchronisations, which left the device idle).
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <atomic>
@@ -164,18 +164,25 @@ extern "C" int64_t wv_split_compress(const void *vol_d, int dtype, const int siz
     off_bins[i] = total; total += al(n * 4);
     off_counts[i] = total; total += al(((int64_t)opt->bin_count + 1) * 4);
   }
-  const int64_t enc_ws = wv_encode_ws_bytes(max_count, opt->bin_count);
-  const int64_t words_cap = wv_encode_words_bound(max_count, opt->bin_count);
-  if (enc_ws < 0 || words_cap < 0) {
-    set_error("wv_split_compress: %d bins", opt->bin_count);
-    return WV_ERR_ARG;
+  // coding: one workspace and one payload buffer for all boxes; every box
+  // has the same zero bin (quant(0) = (bins - 1) >> 1 for uniform and log)
+  const int dup = opt->compress_zeros ? (opt->bin_count - 1) >> 1 : -1;
+  std::vector<int64_t> counts_n(nbox);
+  int64_t words_cap = 0;
+  for (int i = 0; i < nbox; i++) {
+    counts_n[i] = geoms[i].count;
+    const int64_t wb = wv_encode_words_bound(geoms[i].count, opt->bin_count);
+    if (wb < 0) {
+      set_error("wv_split_compress: %d bins", opt->bin_count);
+      return WV_ERR_ARG;
+    }
+    words_cap += wb;
   }
-  // coding workers: one stream, workspace and payload buffer each
-  const int nworkers = std::max(1, std::min(nbox, 16));
+  const int64_t enc_ws = encode_batch_ws_bytes(counts_n.data(), nbox, opt->bin_count, dup);
   const int64_t off_ws = total;
-  total += al(enc_ws) * nworkers;
+  total += al(enc_ws);
   const int64_t off_words = total;
-  total += al(words_cap * 4) * nworkers;
+  total += al(words_cap * 4);
   DevBuf arena;
   SHIP(hipMalloc(&arena.p, (size_t)total), "wv_split_compress: device buffers");
   char *ar = (char *)arena.p;
@@ -208,72 +215,35 @@ extern "C" int64_t wv_split_compress(const void *vol_d, int dtype, const int siz
   SCHK(wv_batch_submit(bh.b, stream));
   SCHK(wv_batch_wait(bh.b));
   const double ms_batch = ms_since(t1);
-  // code the boxes on the workers (box k on worker k % nworkers), then
-  // append them to the stream in box order
+  // code all boxes together, then append them to the stream in box order
   const auto t2 = std::chrono::steady_clock::now();
-  struct Coded {
-    std::vector<uint32_t> words;
-    wv_huff *book = nullptr;
-    wv_thresh_result res;
-    int zb = 0;
-  };
-  std::vector<Coded> coded((size_t)nbox);
-  std::vector<int> rcs((size_t)nworkers, WV_OK);
-  std::vector<std::string> errs((size_t)nworkers);   // error texts are per thread
-  std::vector<std::thread> pool;
-  for (int t = 0; t < nworkers; t++)
-    pool.emplace_back([&, t] {
-      hipStream_t ws = nullptr;
-      if (hipStreamCreateWithFlags(&ws, hipStreamNonBlocking) != hipSuccess) {
-        rcs[t] = WV_ERR_HIP;
-        errs[t] = "wv_split_compress: coding stream";
-        return;
-      }
-      char *my_ws = ar + off_ws + al(enc_ws) * t;
-      uint32_t *my_words = (uint32_t *)(ar + off_words + al(words_cap * 4) * t);
-      for (int i = t; i < nbox && rcs[t] == WV_OK; i += nworkers) {
-        Coded &c = coded[i];
-        int rc = wv_batch_result(bh.b, i, &c.res, &c.zb);
-        int64_t nwords = 0;
-        if (rc == WV_OK)
-          rc = wv_encode_device(items[i].bins_d, geoms[i].count, items[i].counts_d,
-                                opt->bin_count, opt->compress_zeros ? c.zb : -1, my_ws, my_words,
-                                words_cap, &nwords, &c.book, ws);
-        if (rc == WV_OK) {
-          c.words.resize((size_t)nwords);
-          hipError_t e = nwords ? hipMemcpyAsync(c.words.data(), my_words, nwords * 4,
-                                                 hipMemcpyDeviceToHost, ws)
-                                : hipSuccess;
-          if (e == hipSuccess) e = hipStreamSynchronize(ws);
-          if (e != hipSuccess) {
-            set_error("wv_split_compress: payload D2H: %s", hipGetErrorString(e));
-            rc = WV_ERR_HIP;
-          }
-        }
-        rcs[t] = rc;
-        if (rc != WV_OK) errs[t] = wv_last_error();
-      }
-      (void)hipStreamDestroy(ws);
-    });
-  for (auto &th : pool) th.join();
-  int rc = WV_OK;
-  for (int t = 0; t < nworkers; t++)
-    if (rcs[t] != WV_OK) {
-      rc = rcs[t];
-      set_error("%s", errs[t].c_str());
-    }
+  std::vector<const int32_t *> bins_p(nbox);
+  std::vector<const uint32_t *> counts_p(nbox);
+  for (int i = 0; i < nbox; i++) {
+    bins_p[i] = items[i].bins_d;
+    counts_p[i] = items[i].counts_d;
+  }
+  std::vector<std::vector<uint32_t>> words;
+  std::vector<wv_huff *> books;
+  SCHK(encode_batch(bins_p.data(), counts_n.data(), counts_p.data(), nbox, opt->bin_count, dup,
+                    ar + off_ws, enc_ws, (uint32_t *)(ar + off_words), words_cap, words, books,
+                    (hipStream_t)stream));
   StreamHandle sh;
   sh.w = wv_cube_stream_open();
+  int rc = WV_OK;
   for (int i = 0; i < nbox; i++) {
-    Coded &c = coded[i];
     if (rc == WV_OK) {
-      wv_cube_desc d;
-      cube_desc_of(geoms[i], opt, dtype, c.res, {}, {}, c.zb, d);
-      rc = wv_cube_stream_add(sh.w, &d, boxes[i].offset, c.book, c.words.data(),
-                              (int64_t)c.words.size());
+      wv_thresh_result res;
+      int zb = 0;
+      rc = wv_batch_result(bh.b, i, &res, &zb);
+      if (rc == WV_OK) {
+        wv_cube_desc d;
+        cube_desc_of(geoms[i], opt, dtype, res, {}, {}, zb, d);
+        rc = wv_cube_stream_add(sh.w, &d, boxes[i].offset, books[i], words[i].data(),
+                                (int64_t)words[i].size());
+      }
     }
-    wv_huff_free(c.book);
-    c.book = nullptr;
+    wv_huff_free(books[i]);
   }
   if (rc != WV_OK) return rc;
   const int64_t need = wv_cube_stream_bytes(sh.w, opt->with_footer);
