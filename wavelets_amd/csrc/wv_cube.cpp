@@ -474,19 +474,29 @@ std::string stream_tail(const wv_cube_stream *w) {
 
 extern "C" wv_cube_stream *wv_cube_stream_open(void) { return new wv_cube_stream(); }
 
-extern "C" int wv_cube_stream_add(wv_cube_stream *w, const wv_cube_desc *d, const int offset[3],
-                                  const wv_huff *book, const uint32_t *words, int64_t nwords) {
-  if (!w || !d || !book || !offset || nwords < 0 || (nwords && !words) ||
-      nwords > 0x3fffffffll) {
-    wv::set_error("wv_cube_stream_add: bad argument");
+namespace wv {
+int cube_stream_add_params(wv_cube_stream *w, const wv_cube_desc *d, const std::string &params,
+                           const int offset[3], const uint32_t *words, int64_t nwords) {
+  if (!w || !d || !offset || nwords < 0 || (nwords && !words) || nwords > 0x3fffffffll) {
+    set_error("wv_cube_stream_add: bad argument");
     return WV_ERR_ARG;
   }
   const uint32_t nbytes = (uint32_t)(nwords * 4);
-  const uint64_t data_at = w->body.size() + 4;   // after the length word ...
-  const std::string meta = wv::compressed_cubelet_msg(d, book, nbytes, 0, offset);
-  const std::string indexed =
-      wv::compressed_cubelet_msg(d, book, nbytes, data_at + meta.size(), offset);   // ... and meta
+  std::string meta, indexed;
+  // the data follows the length word and the record's message
+  compressed_cubelet_msgs(d, params, nbytes, w->body.size() + 4, offset, meta, indexed);
   return stream_add_record(w, meta, indexed, words, nbytes);
+}
+}  // namespace wv
+
+extern "C" int wv_cube_stream_add(wv_cube_stream *w, const wv_cube_desc *d, const int offset[3],
+                                  const wv_huff *book, const uint32_t *words, int64_t nwords) {
+  if (!d || !book) {
+    wv::set_error("wv_cube_stream_add: bad argument");
+    return WV_ERR_ARG;
+  }
+  return wv::cube_stream_add_params(w, d, wv::compressed_params_msg(d, book), offset, words,
+                                    nwords);
 }
 
 extern "C" int wv_cube_stream_add_raw(wv_cube_stream *w, const int size[3], int dtype,

@@ -350,13 +350,20 @@ struct wv_huff {
 
 namespace wv {
 // the CubeletBuffer message of a compressed cubelet (copyToCubeletBuffer,
-// wavelet.cc:283-343) at box offset `offset`, data_file_offset when > 0
-std::string compressed_cubelet_msg(const wv_cube_desc *d, const wv_huff *book, uint32_t nbytes,
-                                   uint64_t data_offset, const int offset[3]) {
+// wavelet.cc:283-343) at box offset `offset`, without and with
+// data_file_offset (the record's copy and the index's); the parameters
+// message (with the decode table) is built once for both
+std::string compressed_params_msg(const wv_cube_desc *d, const wv_huff *book) {
   const int dup = book->coder.dup();
   const int dup_key = dup >= 0 ? d->bin_count : -1;
-  const std::string params = wvh::params_msg(d, book->coder, dup_key, dup);
-  return wvh::cubelet_msg(d, params, nbytes, data_offset, offset);
+  return wvh::params_msg(d, book->coder, dup_key, dup);
+}
+
+void compressed_cubelet_msgs(const wv_cube_desc *d, const std::string &params, uint32_t nbytes,
+                             uint64_t data_offset_after_meta, const int offset[3],
+                             std::string &meta, std::string &indexed) {
+  meta = wvh::cubelet_msg(d, params, nbytes, 0, offset);
+  indexed = wvh::cubelet_msg(d, params, nbytes, data_offset_after_meta + meta.size(), offset);
 }
 
 // Huffman::totalEncodedLengthBytes (huffman.cc:241-252): every value's code

@@ -228,9 +228,18 @@ void set_error(const char *fmt, ...);
 int64_t huff_total_encoded_bytes(const uint32_t *counts, int bin_count, int dup,
                                  int64_t singles, int64_t dups);
 
-// the CubeletBuffer message of a compressed cubelet at a box offset
-std::string compressed_cubelet_msg(const wv_cube_desc *d, const wv_huff *book, uint32_t nbytes,
-                                   uint64_t data_offset, const int offset[3]);
+// the CubeletBuffer messages of a compressed cubelet at a box offset: the
+// record's (no data_file_offset) and the index's (data at
+// data_offset_after_meta + the record message's length)
+// (the parameters message, with the decode table, built first and
+// separately: it does not depend on the record's place in the stream)
+std::string compressed_params_msg(const wv_cube_desc *d, const wv_huff *book);
+void compressed_cubelet_msgs(const wv_cube_desc *d, const std::string &params, uint32_t nbytes,
+                             uint64_t data_offset_after_meta, const int offset[3],
+                             std::string &meta, std::string &indexed);
+// appends a compressed record whose parameters message is built
+int cube_stream_add_params(wv_cube_stream *w, const wv_cube_desc *d, const std::string &params,
+                           const int offset[3], const uint32_t *words, int64_t nwords);
 
 // compute units of the current device (cached per device)
 int cu_count();

@@ -228,24 +228,39 @@ extern "C" int64_t wv_split_compress(const void *vol_d, int dtype, const int siz
   SCHK(encode_batch(bins_p.data(), counts_n.data(), counts_p.data(), nbox, opt->bin_count, dup,
                     ar + off_ws, enc_ws, (uint32_t *)(ar + off_words), words_cap, words, books,
                     (hipStream_t)stream));
-  StreamHandle sh;
-  sh.w = wv_cube_stream_open();
+  // records: descriptions, then the parameters messages (decode tables)
+  // and the freeing of the code books on host threads, then the stream in
+  // box order
+  std::vector<wv_cube_desc> descs((size_t)nbox);
+  std::vector<std::string> params((size_t)nbox);
   int rc = WV_OK;
-  for (int i = 0; i < nbox; i++) {
-    if (rc == WV_OK) {
-      wv_thresh_result res;
-      int zb = 0;
-      rc = wv_batch_result(bh.b, i, &res, &zb);
-      if (rc == WV_OK) {
-        wv_cube_desc d;
-        cube_desc_of(geoms[i], opt, dtype, res, {}, {}, zb, d);
-        rc = wv_cube_stream_add(sh.w, &d, boxes[i].offset, books[i], words[i].data(),
-                                (int64_t)words[i].size());
-      }
-    }
-    wv_huff_free(books[i]);
+  for (int i = 0; i < nbox && rc == WV_OK; i++) {
+    wv_thresh_result res;
+    int zb = 0;
+    rc = wv_batch_result(bh.b, i, &res, &zb);
+    if (rc == WV_OK) cube_desc_of(geoms[i], opt, dtype, res, {}, {}, zb, descs[i]);
+  }
+  {
+    const int nt = std::max(1, std::min(16, nbox / 16));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++)
+      pool.emplace_back([&, t] {
+        for (int i = t; i < nbox; i += nt) {
+          if (rc == WV_OK) params[i] = compressed_params_msg(&descs[i], books[i]);
+          wv_huff_free(books[i]);
+          books[i] = nullptr;
+        }
+      });
+    for (auto &th : pool) th.join();
   }
   if (rc != WV_OK) return rc;
+  StreamHandle sh;
+  sh.w = wv_cube_stream_open();
+  for (int i = 0; i < nbox; i++) {
+    rc = cube_stream_add_params(sh.w, &descs[i], params[i], boxes[i].offset, words[i].data(),
+                                (int64_t)words[i].size());
+    if (rc != WV_OK) return rc;
+  }
   const int64_t need = wv_cube_stream_bytes(sh.w, opt->with_footer);
   if (info) {
     info->cubelets = nbox;
