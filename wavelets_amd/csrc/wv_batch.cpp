@@ -177,6 +177,7 @@ struct wv_batch {
   int part_begin[3] = {0, 0, 0};
   int sel_total_p[2] = {0, 0}, q_total_p[2] = {0, 0};
   HostPool *pool = nullptr;
+  int max_bins = 2;
   hipStream_t side[BATCH_STREAMS] = {};
   hipStream_t cap = nullptr;   // graphs are captured here (the caller's
                                // stream may be the null stream)
@@ -256,7 +257,8 @@ static int enqueue_A(wv_batch *b, hipStream_t s, int part) {
 
 static int enqueue_B(wv_batch *b, hipStream_t s, int part) {
   const int pb = b->part_begin[part], pe = b->part_begin[part + 1];
-  hipError_t e = quantize_hist_batch(b->q_d + pb, b->qblk_d + pb, pe - pb, b->q_total_p[part], s);
+  hipError_t e = quantize_hist_batch(b->q_d + pb, b->qblk_d + pb, pe - pb, b->q_total_p[part],
+                                     b->max_bins, s);
   if (e != hipSuccess) { set_error("batch quantize: %s", hipGetErrorString(e)); return WV_ERR_HIP; }
   return WV_OK;
 }
@@ -501,6 +503,7 @@ int wv_batch_plan(const wv_batch_item *items, int n, int flags, wv_batch **out) 
   const size_t o_qblk = off; off = up(off + sizeof(int) * n);
   const size_t o_qdev = off; off = up(off + sizeof(QuantDev) * n);
   b->qtab_stride = (max_bins + 63) & ~63;
+  b->max_bins = max_bins;
   const size_t o_qtab = off; off = up(off + sizeof(float) * (size_t)b->qtab_stride * n);
   const size_t o_res = off; off = up(off + sizeof(wv_thresh_result) * n);
   const size_t o_tabs = off;

@@ -146,7 +146,7 @@ struct BQuantJob {
 int quant_blocks_for(int64_t n);
 QuantDev quant_dev_of(const wv_quant_params *p);   // table = p->table_d
 hipError_t quantize_hist_batch(const BQuantJob *jobs_d, const int *blk0_d, int njobs,
-                               int total_blocks, hipStream_t s);
+                               int total_blocks, int max_bins, hipStream_t s);
 hipError_t dequantize_table(const int32_t *bins, int64_t n, const float *table,
                             int bin_count, float *out, hipStream_t s);
 

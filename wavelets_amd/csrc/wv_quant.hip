@@ -159,25 +159,26 @@ __device__ inline int quant_one(const QuantDev &q, const float *tab, float x) {
 
 // One block's share (block bidx of nblk) of the fused quantize + histogram
 // pass over coef[0, n).
-// tab / h are the caller's LDS arrays of WV_LDS_TABLE / WV_LDS_HIST entries
+// tab / h are the caller's LDS arrays of tab_cap / hist_cap entries
 // (declared once per kernel, so instantiations for several quantizers in
 // one kernel share them).
 template <int ALG>
 __device__ __forceinline__ void quant_hist_body(const float *coef, int64_t n, const QuantDev &q,
                                 int32_t *bins, uint32_t *__restrict__ counts,
-                                int bidx, int nblk, float *tab, uint32_t *h) {
+                                int bidx, int nblk, float *tab, uint32_t *h,
+                                int tab_cap = WV_LDS_TABLE, int hist_cap = WV_LDS_HIST) {
   const int ntab = q.n_pos + (ALG == WV_QUANT_LOG ? q.n_neg : 0);
-  const bool tab_in_lds = ntab <= WV_LDS_TABLE;
+  const bool tab_in_lds = ntab <= tab_cap;
   if (ALG != WV_QUANT_UNIFORM && tab_in_lds)
     for (int i = threadIdx.x; i < ntab; i += WV_BLOCK) tab[i] = q.table[i];
-  const bool hist_in_lds = q.bin_count <= WV_LDS_HIST;
+  const bool hist_in_lds = q.bin_count <= hist_cap;
   if (hist_in_lds)
     for (int i = threadIdx.x; i < q.bin_count; i += WV_BLOCK) h[i] = 0;
   __syncthreads();
   const float *t = tab_in_lds ? tab : q.table;
   // log: slot tables after the edges when they leave room
   LogSide LP{}, LN{};
-  const int per = tab_in_lds ? min((WV_LDS_TABLE - ntab) / 2, LOG_LUT_MAX) : 0;
+  const int per = tab_in_lds ? min((tab_cap - ntab) / 2, LOG_LUT_MAX) : 0;
   const bool use_lut = ALG == WV_QUANT_LOG && per >= 64;
   int *lut = (int *)(tab + ntab);
   if (use_lut) {
@@ -496,20 +497,27 @@ __global__ void bquant_prepare(const BQuantJob *J) {
   }
 }
 
+// LDS sized for the batch's largest bin count (dynamic: tab_cap floats of
+// edges + slot tables, then hist_cap counters), so batches of small books
+// keep more blocks resident than the 32 KB of the largest
 __global__ void __launch_bounds__(WV_BLOCK)
-bquant_hist(const BQuantJob *J, const int *blk0, int njobs) {
+bquant_hist(const BQuantJob *J, const int *blk0, int njobs, int tab_cap, int hist_cap) {
   const int jb = find_qjob(blk0, njobs, blockIdx.x);
-  __shared__ float tab[WV_LDS_TABLE];
-  __shared__ uint32_t h[WV_LDS_HIST];
+  extern __shared__ uint32_t qlds[];
+  float *tab = (float *)qlds;
+  uint32_t *h = qlds + tab_cap;
   const BQuantJob &j = J[jb];
   const QuantDev q = *j.q;
   const int bidx = (int)blockIdx.x - j.blk0;
   if (q.alg == WV_QUANT_UNIFORM)
-    quant_hist_body<WV_QUANT_UNIFORM>(j.coef, j.n, q, j.bins, j.counts, bidx, j.nblk, tab, h);
+    quant_hist_body<WV_QUANT_UNIFORM>(j.coef, j.n, q, j.bins, j.counts, bidx, j.nblk, tab, h,
+                                      tab_cap, hist_cap);
   else if (q.alg == WV_QUANT_LOG)
-    quant_hist_body<WV_QUANT_LOG>(j.coef, j.n, q, j.bins, j.counts, bidx, j.nblk, tab, h);
+    quant_hist_body<WV_QUANT_LOG>(j.coef, j.n, q, j.bins, j.counts, bidx, j.nblk, tab, h,
+                                  tab_cap, hist_cap);
   else
-    quant_hist_body<WV_QUANT_LLOYD>(j.coef, j.n, q, j.bins, j.counts, bidx, j.nblk, tab, h);
+    quant_hist_body<WV_QUANT_LLOYD>(j.coef, j.n, q, j.bins, j.counts, bidx, j.nblk, tab, h,
+                                    tab_cap, hist_cap);
 }
 
 // blocks of a batch job: 64 K samples each (a block's fixed costs -- table
@@ -521,11 +529,15 @@ int quant_blocks_for(int64_t n) {
 }
 
 hipError_t quantize_hist_batch(const BQuantJob *jobs_d, const int *blk0_d, int njobs,
-                               int total_blocks, hipStream_t s) {
+                               int total_blocks, int max_bins, hipStream_t s) {
   if (njobs <= 0) return hipSuccess;
+  // edges (< max_bins) plus the log slot tables (2 x 1024), and the counters
+  const int tab_cap = min(WV_LDS_TABLE, ((max_bins + 2 * 1024 + 63) & ~63));
+  const int hist_cap = min(WV_LDS_HIST, ((max_bins + 63) & ~63));
   hipLaunchKernelGGL(bquant_prepare, dim3(njobs), dim3(256), 0, s, jobs_d);
-  hipLaunchKernelGGL(bquant_hist, dim3(total_blocks), dim3(WV_BLOCK), 0, s, jobs_d, blk0_d,
-                     njobs);
+  hipLaunchKernelGGL(bquant_hist, dim3(total_blocks), dim3(WV_BLOCK),
+                     (size_t)(tab_cap + hist_cap) * 4, s, jobs_d, blk0_d, njobs, tab_cap,
+                     hist_cap);
   return hipGetLastError();
 }
 
