@@ -278,7 +278,8 @@ def test_threshold_zero_ties_need_no_fallback(wv, zero_frac):
 
 def _sample_positions(n, m=16384):
     """The select's sample: one key per stratum [i n/m, (i+1) n/m), at the
-    offset a 32-bit mixer (lowbias32 constants) gives for i."""
+    offset (h * width) >> 32 for the 32-bit mixer value h of i (lowbias32
+    constants)."""
     M = np.uint64(0xffffffff)
     i = np.arange(m, dtype=np.uint64)
     h = i.copy()
@@ -289,7 +290,7 @@ def _sample_positions(n, m=16384):
     h ^= h >> np.uint64(16)
     lo = i * np.uint64(n) // np.uint64(m)
     hi = (i + np.uint64(1)) * np.uint64(n) // np.uint64(m)
-    return (lo + h % (hi - lo)).astype(np.int64)
+    return (lo + ((h * (hi - lo)) >> np.uint64(32))).astype(np.int64)
 
 
 def test_threshold_exact_fallback_when_sample_misleads(wv):

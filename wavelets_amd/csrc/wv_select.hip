@@ -7,7 +7,8 @@
 // 31-bit keys:
 //   sample   one block reads 16K keys spread over the array (jittered
 //            strata: key i from a hashed position inside the i-th 1/16K of
-//            the array, so the sample cannot alias with the sub-band
+//            the array -- the 32-bit hash of i times the stratum's width,
+//            shifted down 32 bits -- so the sample cannot alias with the sub-band
 //            layout of the coefficients), and from their two-level
 //            histogram picks a key window [lo, hi] around the target rank
 //            with a margin of 1.25% of the ranks on each side (the sampled
@@ -211,9 +212,14 @@ __device__ void sel_sample_dev(const float *__restrict__ coef, int64_t n, float 
     for (int i = t; i < m; i += 1024)
       keys[i] = __builtin_bit_cast(uint32_t, coef[i]) & 0x7fffffffu;
   } else {
-    for (int i = t; i < m; i += 1024) {
-      const int64_t lo = (int64_t)i * n / m, hi = (int64_t)(i + 1) * n / m;
-      const int64_t pos = lo + (int64_t)(sample_hash((uint32_t)i) % (uint64_t)(hi - lo));
+    // all 16 loads of a thread issued before any is used; the offset in
+    // the stratum is the hash scaled to its width (multiply-shift)
+#pragma unroll
+    for (int k = 0; k < SEL_SAMPLE / 1024; k++) {
+      const int i = t + k * 1024;
+      const int64_t lo = (int64_t)i * n / SEL_SAMPLE, hi = (int64_t)(i + 1) * n / SEL_SAMPLE;
+      const uint64_t w = (uint64_t)(hi - lo);
+      const int64_t pos = lo + (int64_t)(((uint64_t)sample_hash((uint32_t)i) * w) >> 32);
       keys[i] = __builtin_bit_cast(uint32_t, coef[pos]) & 0x7fffffffu;
     }
   }
