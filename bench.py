@@ -432,7 +432,7 @@ def opt_leg(args, wv, torch, dev):
     data (transform + device sort) once, then the reference's default
     plotAll sweep of 50 threshold fractions x 32 log bin counts."""
     from wavelets_amd import synth
-    img = synth.smooth_field_u8(2048, 2048, seed=1)[None, :, :]
+    img = synth.smooth_field_u8(2048, 2048, seed=1)      # [1, 2048, 2048]
     vd = torch.from_numpy(np.ascontiguousarray(img)).to(dev)
     o = wv.Opt(vd, [11, 11, 0], wv.CDF97)
     o.test(o.sorted(o.count() // 2), 256, wv.LOG)      # warm

@@ -623,8 +623,8 @@ class Opt:
     vol: cuda tensor [z, y, x], kept alive by this object."""
 
     def __init__(self, vol, steps, wavelet, max_possible=0, compress_zeros=True, stream=None):
-        if not vol.is_contiguous():
-            raise ValueError("Opt: the volume must be contiguous")
+        if vol.dim() != 3 or not vol.is_contiguous():
+            raise ValueError("Opt: needs a contiguous [z, y, x] volume")
         self.vol = vol
         self.h = C.c_void_p()
         orig = [vol.shape[2], vol.shape[1], vol.shape[0]]
@@ -721,8 +721,8 @@ def split_compress(vol, cub, overlap, wavelet, steps, alg, bins, frac, flags=0,
     """Split a device volume (cuda tensor [z, y, x]) into boxes and compress
     them all through the batched path into one .cube stream
     (wv_split_compress).  Returns (file bytes, SplitInfo)."""
-    if not vol.is_contiguous():
-        raise ValueError("split_compress: the volume must be contiguous")
+    if vol.dim() != 3 or not vol.is_contiguous():
+        raise ValueError("split_compress: needs a contiguous [z, y, x] volume")
     opt = Options(wavelet, alg, bins, _i3(steps), C.c_float(frac), int(compress_zeros), 0,
                   int(with_footer))
     size = _i3([vol.shape[2], vol.shape[1], vol.shape[0]])

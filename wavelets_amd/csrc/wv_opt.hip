@@ -128,6 +128,7 @@ struct wv_opt {
   char *enc_ws = nullptr, *sort_tmp, *table;
   int64_t enc_cap = 0;   // enc_ws: its own buffer, grown to what a test needs
   double *partial;
+  double *sums_h = nullptr;   // pinned: the four error sums
   size_t sort_bytes;
   int64_t table_cap;
   float min_val, max_val, max_abs;
@@ -157,8 +158,9 @@ unsigned grid_of(int64_t n) {
   return (unsigned)(b < 1 ? 1 : (b > cap ? cap : b));
 }
 
-// the error sums of o->rec against the original -> host
-int error_sums(wv_opt *o, double out[4], hipStream_t s) {
+// the error sums of o->rec against the original -> o->sums_h (pinned),
+// enqueued only: the caller synchronises
+int error_sums(wv_opt *o, hipStream_t s) {
   const wv_geom &g = o->g;
   const int ox = g.orig[0], oy = g.orig[1], oz = g.orig[2];
   const int64_t rows = (int64_t)oy * oz;
@@ -183,10 +185,9 @@ int error_sums(wv_opt *o, double out[4], hipStream_t s) {
   }
   hipLaunchKernelGGL(error_final_kernel, dim3(1), dim3(256), 0, s, o->partial, nblk);
   OHIP(hipGetLastError(), "wv_opt: error sums");
-  OHIP(hipMemcpyAsync(out, o->partial + (int64_t)nblk * 4, 4 * sizeof(double),
+  OHIP(hipMemcpyAsync(o->sums_h, o->partial + (int64_t)nblk * 4, 4 * sizeof(double),
                       hipMemcpyDeviceToHost, s),
        "wv_opt: error sums D2H");
-  OHIP(hipStreamSynchronize(s), "wv_opt: sync");
   return WV_OK;
 }
 
@@ -234,8 +235,11 @@ extern "C" int wv_opt_create(const void *in_d, int dtype, const int orig[3], con
   const int64_t o_table = off; off += o->table_cap;
   const int64_t o_part = off; off += al256((OPT_ERR_BLOCKS + 1) * 4 * 8);
   if (hipMalloc((void **)&o->arena, (size_t)off) != hipSuccess ||
-      hipMalloc((void **)&o->enc_ws, (size_t)o->enc_cap) != hipSuccess) {
+      hipMalloc((void **)&o->enc_ws, (size_t)o->enc_cap) != hipSuccess ||
+      hipHostMalloc((void **)&o->sums_h, 4 * sizeof(double), 0) != hipSuccess) {
     if (o->arena) (void)hipFree(o->arena);
+    if (o->enc_ws) (void)hipFree(o->enc_ws);
+    if (o->sums_h) (void)hipHostFree(o->sums_h);
     delete o;
     set_error("wv_opt_create: %lld device bytes", (long long)off);
     return WV_ERR_NOMEM;
@@ -276,6 +280,7 @@ extern "C" int wv_opt_create(const void *in_d, int dtype, const int orig[3], con
   if (rc != WV_OK) {
     (void)hipFree(o->arena);
     (void)hipFree(o->enc_ws);
+    (void)hipHostFree(o->sums_h);
     delete o;
     return rc;
   }
@@ -341,6 +346,9 @@ extern "C" int wv_opt_test(wv_opt *o, float threshold, int bin_count, int quant_
     hipLaunchKernelGGL(apply_threshold_kernel, dim3(grid_of(n)), dim3(256), 0, s, o->coef, n,
                        threshold, o->work);
     OHIP(hipGetLastError(), "wv_opt_test: threshold");
+    OCHK(wv_dwt3d_inverse(&o->g, inv_in, o->rec, o->ws, stream));
+    OCHK(error_sums(o, s));
+    OHIP(hipStreamSynchronize(s), "wv_opt_test: sync");
   } else {
     if (quant_alg != WV_QUANT_UNIFORM && quant_alg != WV_QUANT_LOG) {
       set_error("wv_opt_test: quantizer %d (uniform or log)", quant_alg);
@@ -382,19 +390,22 @@ extern "C" int wv_opt_test(wv_opt *o, float threshold, int bin_count, int quant_
     if (rc == WV_OK &&
         enc_count_runs(o->bins, n, dup, bin_count, o->enc_ws, sd, s) != hipSuccess)
       rc = WV_ERR_HIP;
+    // the device goes on (dequantize, inverse, error sums) while the host
+    // builds the code book for the size
     if (rc == WV_OK) rc = wv_dequantize(o->bins, n, &p, o->work, stream);
-    if (rc == WV_OK && hipStreamSynchronize(s) != hipSuccess) rc = WV_ERR_HIP;
-    wv_quant_params_free(&p);
+    if (rc == WV_OK) rc = wv_dwt3d_inverse(&o->g, inv_in, o->rec, o->ws, stream);
+    if (rc == WV_OK) rc = error_sums(o, s);
+    if (rc == WV_OK)
+      r->size_bytes = huff_total_encoded_bytes(counts.data(), bin_count, dup, (int64_t)sd[0],
+                                               (int64_t)sd[1]);
+    if (hipStreamSynchronize(s) != hipSuccess && rc == WV_OK) rc = WV_ERR_HIP;
+    wv_quant_params_free(&p);   // its tables were copied by the stream
     if (rc != WV_OK) {
-      if (rc == WV_ERR_HIP) set_error("wv_opt_test: quantize / histogram / dequantize");
+      if (rc == WV_ERR_HIP) set_error("wv_opt_test: quantize / dequantize / inverse / error");
       return rc;
     }
-    r->size_bytes = huff_total_encoded_bytes(counts.data(), bin_count, dup, (int64_t)sd[0],
-                                             (int64_t)sd[1]);
   }
-  OCHK(wv_dwt3d_inverse(&o->g, inv_in, o->rec, o->ws, stream));
-  double sums[4];
-  OCHK(error_sums(o, sums, s));
+  const double *sums = o->sums_h;
   // ErrorAccumulator getters (test_compress_common.h:135-159)
   const int64_t cnt = (int64_t)o->g.orig[0] * o->g.orig[1] * o->g.orig[2];
   float maxv = 0.f;
@@ -412,6 +423,7 @@ extern "C" int wv_opt_destroy(wv_opt *o) {
   if (!o) return WV_OK;
   if (o->arena) (void)hipFree(o->arena);
   if (o->enc_ws) (void)hipFree(o->enc_ws);
+  if (o->sums_h) (void)hipHostFree(o->sums_h);
   delete o;
   return WV_OK;
 }
