@@ -444,6 +444,15 @@ int wv_batch_groups(const wv_batch *b);
 int wv_batch_group_tuning(const wv_batch *b, int g, int *tile_cap, int *ept0, int *bpc,
                           int *ncub);
 int wv_batch_destroy(wv_batch *b);
+/* after a submit: every item Huffman-coded on the device (items grouped by
+   bin count, each group in one launch per phase), written as one .cube
+   stream of compressed cubelets in item order, at offsets[3 * i] (NULL:
+   0, 0, 0).  Waits for the batch; allocates its coding buffers;
+   synchronises.  Returns the bytes (WV_ERR_NOMEM with *need set when cap is
+   short).  No counterpart in the reference (one file per run there). */
+int64_t wv_batch_write_stream(wv_batch *b, const int *offsets, int compress_zeros,
+                              int with_footer, uint8_t *file_buf, int64_t cap, int64_t *need,
+                              void *stream);
 
 /* ------------------------------------------------------------------ *
  * Serving loop.  No counterpart in the reference (one file per run of

@@ -211,3 +211,23 @@ def test_batch_wait_on_null_stream_before_any_other_sync(wv):
         for i, (r, w) in enumerate(zip(got, want)):
             assert r.threshold == w.threshold and r.max_abs == w.max_abs, i
     b.close()
+
+
+def test_batch_write_stream_records_match_oracle(wv):
+    """The whole batch coded on the device (grouped by bin count) into one
+    .cube stream: every record equals the oracle's own file record for that
+    cubelet (parameters message with the decode table, payload)."""
+    import _cubeproto as P
+    cubes, vols = _cubes(MIX, seed0=400)
+    b = wv.Batch(cubes)
+    b.submit()
+    f = b.write_stream()
+    _, recs, index, _ = P.parse_cube_file(f)
+    assert len(recs) == len(cubes) == len(index.cubelets)
+    for i, ((cb, raw, payload), v, c) in enumerate(zip(recs, vols, cubes)):
+        ov = v.astype(np.int32) if v.dtype == np.int16 else v
+        want = O.compress(ov, c["wavelet"], c["alg"], c["bins"], c["steps"], c["frac"])["file"]
+        _, wrecs, _, _ = P.parse_cube_file(want)
+        assert payload == wrecs[0][2], i
+        assert raw == wrecs[0][1], i
+    b.close()

@@ -181,6 +181,9 @@ def _declare(L):
         L.wv_cube_stream_finish.restype = I64
         L.wv_cube_stream_finish.argtypes = [P, C.c_int, P, I64]
         L.wv_cube_stream_free.argtypes = [P]
+        L.wv_batch_write_stream.restype = I64
+        L.wv_batch_write_stream.argtypes = [P, P, C.c_int, C.c_int, P, I64,
+                                            C.POINTER(C.c_int64), P]
         L.wv_split_compress.restype = I64
         L.wv_split_compress.argtypes = [P, C.c_int, P, P, C.c_int, C.POINTER(Options), C.c_int,
                                         P, I64, C.POINTER(SplitInfo), P]
@@ -798,6 +801,21 @@ class Batch:
         zb = C.c_int()
         check(lib().wv_batch_result(self.h, i, C.byref(r), C.byref(zb)))
         return r, zb.value
+
+    def write_stream(self, compress_zeros=True, with_footer=True, stream=None):
+        """Every item coded on the device into one .cube stream (bytes)."""
+        need = C.c_int64()
+        cap = 2 * sum(int(g.count) for g in self.geoms) + (1 << 20)
+        for _ in range(2):
+            buf = np.empty(cap, np.uint8)
+            n = lib().wv_batch_write_stream(self.h, None, int(compress_zeros), int(with_footer),
+                                            buf.ctypes.data_as(P), cap, C.byref(need),
+                                            _stream(stream))
+            if n != -4 or need.value <= cap:
+                break
+            cap = need.value
+        check_size(n)
+        return buf[:n].tobytes()
 
     def groups(self):
         out = []

@@ -745,6 +745,14 @@ int wv_batch_result(const wv_batch *b, int i, wv_thresh_result *res, int *zero_b
   return WV_OK;
 }
 
+}  // extern "C"
+
+namespace wv {
+const wv_batch_item *batch_items(const wv_batch *b) { return b ? b->items.data() : nullptr; }
+}  // namespace wv
+
+extern "C" {
+
 int wv_batch_destroy(wv_batch *b) {
   free_batch(b);
   return WV_OK;

@@ -205,6 +205,9 @@ hipError_t enc_batch_payload(EncBatch &P, void *ws, const uint64_t *books_host, 
                              uint32_t *words, const int64_t *woff, const int64_t *nwords,
                              int64_t total_words, hipStream_t s);
 
+// the items a batch was planned with (wv_batch.cpp)
+const wv_batch_item *batch_items(const wv_batch *b);
+
 // the whole batch coding (wv_huffman.cpp): books and host payload words
 // per job; ws of encode_batch_ws_bytes, words_d of the summed payloads
 int64_t encode_batch_ws_bytes(const int64_t *n, int nj, int bin_count, int dup);

@@ -14,6 +14,7 @@
 // small launches and two synchronisations, which left the device idle).
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <map>
 #include <atomic>
 #include <chrono>
 #include <cstring>
@@ -275,4 +276,104 @@ extern "C" int64_t wv_split_compress(const void *vol_d, int dtype, const int siz
     return WV_ERR_NOMEM;
   }
   return wv_cube_stream_finish(sh.w, opt->with_footer, file_buf, cap);
+}
+
+// A finished batch as one .cube stream: items grouped by (bin count, dup
+// value), each group coded together (encode_batch), records in item order
+// with the given box offsets (NULL: all 0).
+extern "C" int64_t wv_batch_write_stream(wv_batch *b, const int *offsets, int compress_zeros,
+                                         int with_footer, uint8_t *file_buf, int64_t cap,
+                                         int64_t *need_out, void *stream) {
+  if (!b) {
+    set_error("wv_batch_write_stream: null batch");
+    return WV_ERR_ARG;
+  }
+  SCHK(wv_batch_wait(b));
+  const int n = wv_batch_size(b);
+  const wv_batch_item *items = batch_items(b);
+  std::vector<wv_geom> geoms((size_t)n);
+  std::vector<wv_thresh_result> res((size_t)n);
+  std::vector<int> zbs((size_t)n);
+  std::map<std::pair<int, int>, std::vector<int>> groups;
+  for (int i = 0; i < n; i++) {
+    SCHK(wv_geom_init(&geoms[i], items[i].orig, items[i].steps, items[i].wavelet, items[i].dtype));
+    SCHK(wv_batch_result(b, i, &res[i], &zbs[i]));
+    groups[{items[i].bin_count, compress_zeros ? zbs[i] : -1}].push_back(i);
+  }
+  std::vector<std::vector<uint32_t>> words((size_t)n);
+  std::vector<wv_huff *> books((size_t)n, nullptr);
+  auto free_books = [&] {
+    for (auto *bk : books) wv_huff_free(bk);
+  };
+  for (auto &kv : groups) {
+    const std::vector<int> &ix = kv.second;
+    const int nj = (int)ix.size(), bins = kv.first.first, dup = kv.first.second;
+    std::vector<const int32_t *> bp(nj);
+    std::vector<const uint32_t *> cp(nj);
+    std::vector<int64_t> cnt(nj);
+    int64_t wcap = 0;
+    for (int k = 0; k < nj; k++) {
+      bp[k] = items[ix[k]].bins_d;
+      cp[k] = items[ix[k]].counts_d;
+      cnt[k] = geoms[ix[k]].count;
+      wcap += wv_encode_words_bound(cnt[k], bins);
+    }
+    const int64_t wsb = encode_batch_ws_bytes(cnt.data(), nj, bins, dup);
+    DevBuf ws, wd;
+    if (hipMalloc(&ws.p, (size_t)wsb) != hipSuccess ||
+        hipMalloc(&wd.p, (size_t)(wcap > 0 ? wcap : 1) * 4) != hipSuccess) {
+      free_books();
+      set_error("wv_batch_write_stream: coding buffers");
+      return WV_ERR_NOMEM;
+    }
+    std::vector<std::vector<uint32_t>> gw;
+    std::vector<wv_huff *> gb;
+    const int rc = encode_batch(bp.data(), cnt.data(), cp.data(), nj, bins, dup, ws.p, wsb,
+                                (uint32_t *)wd.p, wcap, gw, gb, (hipStream_t)stream);
+    if (rc != WV_OK) {
+      free_books();
+      return rc;
+    }
+    for (int k = 0; k < nj; k++) {
+      words[ix[k]].swap(gw[k]);
+      books[ix[k]] = gb[k];
+    }
+  }
+  std::vector<wv_cube_desc> descs((size_t)n);
+  std::vector<std::string> params((size_t)n);
+  {
+    const int nt = std::max(1, std::min(16, n / 16));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++)
+      pool.emplace_back([&, t] {
+        for (int i = t; i < n; i += nt) {
+          wv_options opt{};
+          opt.wavelet = items[i].wavelet;
+          opt.quant_alg = items[i].quant_alg;
+          opt.bin_count = items[i].bin_count;
+          std::memcpy(opt.steps, items[i].steps, sizeof opt.steps);
+          opt.thresh_frac = items[i].thresh_frac;
+          opt.compress_zeros = compress_zeros;
+          opt.with_footer = with_footer;
+          cube_desc_of(geoms[i], &opt, items[i].dtype, res[i], {}, {}, zbs[i], descs[i]);
+          params[i] = compressed_params_msg(&descs[i], books[i]);
+          wv_huff_free(books[i]);
+          books[i] = nullptr;
+        }
+      });
+    for (auto &th : pool) th.join();
+  }
+  StreamHandle sh;
+  sh.w = wv_cube_stream_open();
+  static const int zero3[3] = {0, 0, 0};
+  for (int i = 0; i < n; i++)
+    SCHK(cube_stream_add_params(sh.w, &descs[i], params[i], offsets ? offsets + 3 * i : zero3,
+                                words[i].data(), (int64_t)words[i].size()));
+  const int64_t need = wv_cube_stream_bytes(sh.w, with_footer);
+  if (need_out) *need_out = need;
+  if (!file_buf || cap < need) {
+    set_error("wv_batch_write_stream: file buffer %lld < %lld", (long long)cap, (long long)need);
+    return WV_ERR_NOMEM;
+  }
+  return wv_cube_stream_finish(sh.w, with_footer, file_buf, cap);
 }
