@@ -366,6 +366,19 @@ def mixed_stream(args, wv, torch, dev, steps, warmup):
         pb = work[:n * 4].view(torch.int32)
         if not torch.equal(pb, b.bins[i]):
             same = False
+    # the whole stream into one .cube (coding on the device, payload D2H,
+    # container on the host), after a batch submit
+    b.submit()
+    f = b.write_stream()
+    t0 = time.time()
+    b.submit()
+    b.wait()
+    t1 = time.time()
+    f = b.write_stream()
+    t2 = time.time()
+    ms_cube = (t2 - t1) * 1e3
+    log("batched to .cube: submit %.1f ms + coding/stream %.1f ms, %d bytes"
+        % ((t1 - t0) * 1e3, ms_cube, len(f)))
     # select statistics: candidates examined, exact fallbacks (top bit)
     nc = [b.result(i)[0].n_candidates for i in range(len(cubes))]
     fallbacks = sum(1 for v in nc if v < 0)
@@ -382,6 +395,11 @@ def mixed_stream(args, wv, torch, dev, steps, warmup):
         "batched_over_plain": round(batched_gbps / plain_gbps, 3),
         "plain_bins_equal_batched": same,
         "groups": len(b.groups()),
+        "to_cube_ms": round(ms_cube, 3),
+        "to_cube_file_bytes": len(f),
+        "to_cube_gbps": round(in_bytes / ((ms_batch + ms_cube) * 1e-3) / 1e9, 3),
+        "to_cube_note": "batched bins (ms_per_stream) plus coding every cubelet on the device, "
+                        "payload D2H and the one .cube stream on the host",
         "select_fallbacks": fallbacks,
         "select_max_candidates": max(v & 0x7fffffff for v in nc),
     }

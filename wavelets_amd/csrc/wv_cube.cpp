@@ -475,6 +475,10 @@ std::string stream_tail(const wv_cube_stream *w) {
 extern "C" wv_cube_stream *wv_cube_stream_open(void) { return new wv_cube_stream(); }
 
 namespace wv {
+void cube_stream_reserve(wv_cube_stream *w, size_t bytes) {
+  if (w) w->body.reserve(w->body.size() + bytes);
+}
+
 int cube_stream_add_params(wv_cube_stream *w, const wv_cube_desc *d, const std::string &params,
                            const int offset[3], const uint32_t *words, int64_t nwords) {
   if (!w || !d || !offset || nwords < 0 || (nwords && !words) || nwords > 0x3fffffffll) {

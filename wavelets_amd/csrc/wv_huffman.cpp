@@ -509,10 +509,12 @@ int64_t encode_batch_ws_bytes(const int64_t *n, int nj, int bin_count, int dup) 
 // wv_encode_device gives each array alone.
 int encode_batch(const int32_t *const *bins_d, const int64_t *n, const uint32_t *const *counts_d,
                  int nj, int bin_count, int dup, void *ws_d, int64_t ws_bytes, uint32_t *words_d,
-                 int64_t words_cap, std::vector<std::vector<uint32_t>> &words_host,
-                 std::vector<wv_huff *> &books, hipStream_t s) {
-  words_host.assign((size_t)nj, {});
+                 int64_t words_cap, std::vector<uint32_t> &all, std::vector<int64_t> &woff,
+                 std::vector<int64_t> &nwords, std::vector<wv_huff *> &books, hipStream_t s) {
   books.assign((size_t)nj, nullptr);
+  woff.assign((size_t)nj, 0);
+  nwords.assign((size_t)nj, 0);
+  all.clear();
   if (nj <= 0) return WV_OK;
   if (bin_count < 1 || bin_count > WV_MAX_BINS) {
     set_error("encode_batch: %d bins", bin_count);
@@ -536,7 +538,7 @@ int encode_batch(const int32_t *const *bins_d, const int64_t *n, const uint32_t 
   // code books on host threads
   const int nb1 = P.nb1;
   std::vector<uint64_t> packed((size_t)nj * nb1);
-  std::vector<int64_t> nwords((size_t)nj), bits((size_t)nj);
+  std::vector<int64_t> bits((size_t)nj);
   std::vector<char> shorts((size_t)nj, 1);
   std::vector<int> rcs((size_t)nj, WV_OK);
   std::vector<std::string> errs((size_t)nj);
@@ -596,13 +598,24 @@ int encode_batch(const int32_t *const *bins_d, const int64_t *n, const uint32_t 
       set_error("encode_batch: job %d: %s", j, errs[j].c_str());
       return rcs[j];
     }
-  std::vector<int64_t> woff((size_t)nj);
   int64_t total = 0;
   bool all_short = true;
   for (int j = 0; j < nj; j++) {
     woff[j] = total;
     total += nwords[j];
     all_short = all_short && shorts[j];
+  }
+  // words_d null: a buffer of exactly the payloads' size, freed here
+  void *own = nullptr;
+  if (!words_d && total > 0) {
+    if (hipMalloc(&own, (size_t)total * 4) != hipSuccess) {
+      for (auto *b : books) delete b;
+      books.assign((size_t)nj, nullptr);
+      set_error("encode_batch: %lld payload words", (long long)total);
+      return WV_ERR_NOMEM;
+    }
+    words_d = (uint32_t *)own;
+    words_cap = total;
   }
   if (total > words_cap) {
     for (auto *b : books) delete b;
@@ -611,7 +624,7 @@ int encode_batch(const int32_t *const *bins_d, const int64_t *n, const uint32_t 
               (long long)words_cap);
     return WV_ERR_NOMEM;
   }
-  std::vector<uint32_t> all((size_t)total);
+  all.resize((size_t)total);
   e = hipSuccess;
   if (total > 0) {
     e = enc_batch_payload(P, ws_d, packed.data(), all_short, words_d, woff.data(), nwords.data(),
@@ -619,15 +632,14 @@ int encode_batch(const int32_t *const *bins_d, const int64_t *n, const uint32_t 
     if (e == hipSuccess)
       e = hipMemcpyAsync(all.data(), words_d, (size_t)total * 4, hipMemcpyDeviceToHost, s);
   }
-  if (e == hipSuccess) e = hipStreamSynchronize(s);   // packed / all are locals
+  if (e == hipSuccess) e = hipStreamSynchronize(s);   // packed is a local
+  if (own) (void)hipFree(own);
   if (e != hipSuccess) {
     for (auto *b : books) delete b;
     books.assign((size_t)nj, nullptr);
     set_error("encode_batch: payloads: %s", hipGetErrorString(e));
     return WV_ERR_HIP;
   }
-  for (int j = 0; j < nj; j++)
-    words_host[j].assign(all.begin() + woff[j], all.begin() + woff[j] + nwords[j]);
   return WV_OK;
 }
 

@@ -208,13 +208,17 @@ hipError_t enc_batch_payload(EncBatch &P, void *ws, const uint64_t *books_host, 
 // the items a batch was planned with (wv_batch.cpp)
 const wv_batch_item *batch_items(const wv_batch *b);
 
-// the whole batch coding (wv_huffman.cpp): books and host payload words
-// per job; ws of encode_batch_ws_bytes, words_d of the summed payloads
+// the whole batch coding (wv_huffman.cpp): books per job and all payload
+// words on the host (job j's nwords[j] words at all + woff[j]); ws of
+// encode_batch_ws_bytes; words_d of the summed payloads, or null to have
+// one of the exact size allocated and freed inside
 int64_t encode_batch_ws_bytes(const int64_t *n, int nj, int bin_count, int dup);
 int encode_batch(const int32_t *const *bins_d, const int64_t *n, const uint32_t *const *counts_d,
                  int nj, int bin_count, int dup, void *ws_d, int64_t ws_bytes, uint32_t *words_d,
-                 int64_t words_cap, std::vector<std::vector<uint32_t>> &words_host,
-                 std::vector<wv_huff *> &books, hipStream_t s);
+                 int64_t words_cap, std::vector<uint32_t> &all, std::vector<int64_t> &woff,
+                 std::vector<int64_t> &nwords, std::vector<wv_huff *> &books, hipStream_t s);
+// grows a stream's buffer ahead of many records
+void cube_stream_reserve(wv_cube_stream *w, size_t bytes);
 
 // after wv_compress_device on work_d: device Huffman coding of its bins,
 // payload copied to words_host (synchronises); and the container fields

@@ -169,21 +169,10 @@ extern "C" int64_t wv_split_compress(const void *vol_d, int dtype, const int siz
   // has the same zero bin (quant(0) = (bins - 1) >> 1 for uniform and log)
   const int dup = opt->compress_zeros ? (opt->bin_count - 1) >> 1 : -1;
   std::vector<int64_t> counts_n(nbox);
-  int64_t words_cap = 0;
-  for (int i = 0; i < nbox; i++) {
-    counts_n[i] = geoms[i].count;
-    const int64_t wb = wv_encode_words_bound(geoms[i].count, opt->bin_count);
-    if (wb < 0) {
-      set_error("wv_split_compress: %d bins", opt->bin_count);
-      return WV_ERR_ARG;
-    }
-    words_cap += wb;
-  }
+  for (int i = 0; i < nbox; i++) counts_n[i] = geoms[i].count;
   const int64_t enc_ws = encode_batch_ws_bytes(counts_n.data(), nbox, opt->bin_count, dup);
   const int64_t off_ws = total;
   total += al(enc_ws);
-  const int64_t off_words = total;
-  total += al(words_cap * 4);
   DevBuf arena;
   SHIP(hipMalloc(&arena.p, (size_t)total), "wv_split_compress: device buffers");
   char *ar = (char *)arena.p;
@@ -224,10 +213,11 @@ extern "C" int64_t wv_split_compress(const void *vol_d, int dtype, const int siz
     bins_p[i] = items[i].bins_d;
     counts_p[i] = items[i].counts_d;
   }
-  std::vector<std::vector<uint32_t>> words;
+  std::vector<uint32_t> words;
+  std::vector<int64_t> woff, nwords;
   std::vector<wv_huff *> books;
   SCHK(encode_batch(bins_p.data(), counts_n.data(), counts_p.data(), nbox, opt->bin_count, dup,
-                    ar + off_ws, enc_ws, (uint32_t *)(ar + off_words), words_cap, words, books,
+                    ar + off_ws, enc_ws, nullptr, 0, words, woff, nwords, books,
                     (hipStream_t)stream));
   // records: descriptions, then the parameters messages (decode tables)
   // and the freeing of the code books on host threads, then the stream in
@@ -257,9 +247,10 @@ extern "C" int64_t wv_split_compress(const void *vol_d, int dtype, const int siz
   if (rc != WV_OK) return rc;
   StreamHandle sh;
   sh.w = wv_cube_stream_open();
+  cube_stream_reserve(sh.w, words.size() * 4 + (size_t)nbox * (params.empty() ? 0 : params[0].size() + 256));
   for (int i = 0; i < nbox; i++) {
-    rc = cube_stream_add_params(sh.w, &descs[i], params[i], boxes[i].offset, words[i].data(),
-                                (int64_t)words[i].size());
+    rc = cube_stream_add_params(sh.w, &descs[i], params[i], boxes[i].offset,
+                                words.data() + woff[i], nwords[i]);
     if (rc != WV_OK) return rc;
   }
   const int64_t need = wv_cube_stream_bytes(sh.w, opt->with_footer);
@@ -300,7 +291,10 @@ extern "C" int64_t wv_batch_write_stream(wv_batch *b, const int *offsets, int co
     SCHK(wv_batch_result(b, i, &res[i], &zbs[i]));
     groups[{items[i].bin_count, compress_zeros ? zbs[i] : -1}].push_back(i);
   }
-  std::vector<std::vector<uint32_t>> words((size_t)n);
+  // per group: its payload words; per item: group, offset and length
+  std::vector<std::vector<uint32_t>> gwords;
+  std::vector<int> gof((size_t)n);
+  std::vector<int64_t> iwoff((size_t)n), inw((size_t)n);
   std::vector<wv_huff *> books((size_t)n, nullptr);
   auto free_books = [&] {
     for (auto *bk : books) wv_huff_free(bk);
@@ -311,33 +305,35 @@ extern "C" int64_t wv_batch_write_stream(wv_batch *b, const int *offsets, int co
     std::vector<const int32_t *> bp(nj);
     std::vector<const uint32_t *> cp(nj);
     std::vector<int64_t> cnt(nj);
-    int64_t wcap = 0;
     for (int k = 0; k < nj; k++) {
       bp[k] = items[ix[k]].bins_d;
       cp[k] = items[ix[k]].counts_d;
       cnt[k] = geoms[ix[k]].count;
-      wcap += wv_encode_words_bound(cnt[k], bins);
     }
     const int64_t wsb = encode_batch_ws_bytes(cnt.data(), nj, bins, dup);
-    DevBuf ws, wd;
-    if (hipMalloc(&ws.p, (size_t)wsb) != hipSuccess ||
-        hipMalloc(&wd.p, (size_t)(wcap > 0 ? wcap : 1) * 4) != hipSuccess) {
+    DevBuf ws;
+    if (hipMalloc(&ws.p, (size_t)wsb) != hipSuccess) {
       free_books();
-      set_error("wv_batch_write_stream: coding buffers");
+      set_error("wv_batch_write_stream: coding workspace");
       return WV_ERR_NOMEM;
     }
-    std::vector<std::vector<uint32_t>> gw;
+    std::vector<uint32_t> gw;
+    std::vector<int64_t> gwoff, gnw;
     std::vector<wv_huff *> gb;
     const int rc = encode_batch(bp.data(), cnt.data(), cp.data(), nj, bins, dup, ws.p, wsb,
-                                (uint32_t *)wd.p, wcap, gw, gb, (hipStream_t)stream);
+                                nullptr, 0, gw, gwoff, gnw, gb, (hipStream_t)stream);
     if (rc != WV_OK) {
       free_books();
       return rc;
     }
     for (int k = 0; k < nj; k++) {
-      words[ix[k]].swap(gw[k]);
+      gof[ix[k]] = (int)gwords.size();
+      iwoff[ix[k]] = gwoff[k];
+      inw[ix[k]] = gnw[k];
       books[ix[k]] = gb[k];
     }
+    gwords.emplace_back();
+    gwords.back().swap(gw);
   }
   std::vector<wv_cube_desc> descs((size_t)n);
   std::vector<std::string> params((size_t)n);
@@ -365,10 +361,14 @@ extern "C" int64_t wv_batch_write_stream(wv_batch *b, const int *offsets, int co
   }
   StreamHandle sh;
   sh.w = wv_cube_stream_open();
+  size_t est = 0;
+  for (auto &g : gwords) est += g.size() * 4;
+  for (auto &pm : params) est += pm.size() * 2 + 256;
+  cube_stream_reserve(sh.w, est);
   static const int zero3[3] = {0, 0, 0};
   for (int i = 0; i < n; i++)
     SCHK(cube_stream_add_params(sh.w, &descs[i], params[i], offsets ? offsets + 3 * i : zero3,
-                                words[i].data(), (int64_t)words[i].size()));
+                                gwords[gof[i]].data() + iwoff[i], inw[i]));
   const int64_t need = wv_cube_stream_bytes(sh.w, with_footer);
   if (need_out) *need_out = need;
   if (!file_buf || cap < need) {
