@@ -405,7 +405,9 @@ int64_t wv_compress_counts_offset(const int orig[3], const wv_options *opt);
  * the persistent pass kernel over all (cubelet, tile) items; select and
  * quantize run a few launches for the whole batch; both sequences are
  * replayed as hipGraphs.  Log quantizer tables are built on the host
- * between the two (one synchronisation per submit).
+ * (host logf); a batch with log jobs runs in two parts so that the host
+ * builds one part's tables while the device transforms or quantizes the
+ * other (one event wait per part).
  * ------------------------------------------------------------------ */
 typedef struct {
   const void *in_d;        /* device input: orig[0]*orig[1]*orig[2] samples */
