@@ -435,7 +435,8 @@ typedef struct wv_batch wv_batch;
    autotunes when asked (runs kernels; synchronises) */
 int wv_batch_plan(const wv_batch_item *items, int n, int flags, wv_batch **out);
 /* enqueues the batch on stream; returns after the quantize stage is
-   enqueued (log quantizers synchronise once for the host tables) */
+   enqueued (with log quantizers, after waiting for each part's threshold
+   records, from which the host builds that part's tables) */
 int wv_batch_submit(wv_batch *b, void *stream);
 int wv_batch_poll(wv_batch *b);          /* 1 done, 0 running, <0 error */
 int wv_batch_wait(wv_batch *b);
