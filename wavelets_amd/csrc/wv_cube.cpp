@@ -475,6 +475,12 @@ std::string stream_tail(const wv_cube_stream *w) {
 extern "C" wv_cube_stream *wv_cube_stream_open(void) { return new wv_cube_stream(); }
 
 namespace wv {
+std::string cube_index_tail(const std::vector<std::string> &index) {
+  wv_cube_stream w;
+  w.index = index;
+  return stream_tail(&w);
+}
+
 void cube_stream_reserve(wv_cube_stream *w, size_t bytes) {
   if (w) w->body.reserve(w->body.size() + bytes);
 }

@@ -510,7 +510,9 @@ int64_t encode_batch_ws_bytes(const int64_t *n, int nj, int bin_count, int dup) 
 int encode_batch(const int32_t *const *bins_d, const int64_t *n, const uint32_t *const *counts_d,
                  int nj, int bin_count, int dup, void *ws_d, int64_t ws_bytes, uint32_t *words_d,
                  int64_t words_cap, std::vector<uint32_t> &all, std::vector<int64_t> &woff,
-                 std::vector<int64_t> &nwords, std::vector<wv_huff *> &books, hipStream_t s) {
+                 std::vector<int64_t> &nwords, std::vector<wv_huff *> &books, hipStream_t s,
+                 uint32_t **words_dev_out) {
+  if (words_dev_out) *words_dev_out = nullptr;
   books.assign((size_t)nj, nullptr);
   woff.assign((size_t)nj, 0);
   nwords.assign((size_t)nj, 0);
@@ -624,15 +626,19 @@ int encode_batch(const int32_t *const *bins_d, const int64_t *n, const uint32_t 
               (long long)words_cap);
     return WV_ERR_NOMEM;
   }
-  all.resize((size_t)total);
+  if (!words_dev_out) all.resize((size_t)total);
   e = hipSuccess;
   if (total > 0) {
     e = enc_batch_payload(P, ws_d, packed.data(), all_short, words_d, woff.data(), nwords.data(),
                           total, s);
-    if (e == hipSuccess)
+    if (e == hipSuccess && !words_dev_out)
       e = hipMemcpyAsync(all.data(), words_d, (size_t)total * 4, hipMemcpyDeviceToHost, s);
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s);   // packed is a local
+  if (words_dev_out && e == hipSuccess) {
+    *words_dev_out = words_d;   // the payloads stay on the device (own: the caller frees)
+    own = nullptr;
+  }
   if (own) (void)hipFree(own);
   if (e != hipSuccess) {
     for (auto *b : books) delete b;

@@ -216,7 +216,13 @@ int64_t encode_batch_ws_bytes(const int64_t *n, int nj, int bin_count, int dup);
 int encode_batch(const int32_t *const *bins_d, const int64_t *n, const uint32_t *const *counts_d,
                  int nj, int bin_count, int dup, void *ws_d, int64_t ws_bytes, uint32_t *words_d,
                  int64_t words_cap, std::vector<uint32_t> &all, std::vector<int64_t> &woff,
-                 std::vector<int64_t> &nwords, std::vector<wv_huff *> &books, hipStream_t s);
+                 std::vector<int64_t> &nwords, std::vector<wv_huff *> &books, hipStream_t s,
+                 uint32_t **words_dev_out = nullptr);
+// (words_dev_out: the payloads stay on the device, all stays empty, and
+// *words_dev_out is their buffer -- allocated here when words_d is null,
+// then the caller's to free)
+// the index footer of a stream whose records' indexed messages are given
+std::string cube_index_tail(const std::vector<std::string> &index);
 // grows a stream's buffer ahead of many records
 void cube_stream_reserve(wv_cube_stream *w, size_t bytes);
 

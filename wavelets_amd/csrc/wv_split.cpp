@@ -291,14 +291,16 @@ extern "C" int64_t wv_batch_write_stream(wv_batch *b, const int *offsets, int co
     SCHK(wv_batch_result(b, i, &res[i], &zbs[i]));
     groups[{items[i].bin_count, compress_zeros ? zbs[i] : -1}].push_back(i);
   }
-  // per group: its payload words; per item: group, offset and length
-  std::vector<std::vector<uint32_t>> gwords;
+  // per group: its payloads (left on the device); per item: group, offset
+  // and length
+  std::vector<DevBuf> gdev(groups.size());
   std::vector<int> gof((size_t)n);
   std::vector<int64_t> iwoff((size_t)n), inw((size_t)n);
   std::vector<wv_huff *> books((size_t)n, nullptr);
   auto free_books = [&] {
     for (auto *bk : books) wv_huff_free(bk);
   };
+  int gi = 0;
   for (auto &kv : groups) {
     const std::vector<int> &ix = kv.second;
     const int nj = (int)ix.size(), bins = kv.first.first, dup = kv.first.second;
@@ -317,23 +319,24 @@ extern "C" int64_t wv_batch_write_stream(wv_batch *b, const int *offsets, int co
       set_error("wv_batch_write_stream: coding workspace");
       return WV_ERR_NOMEM;
     }
-    std::vector<uint32_t> gw;
+    std::vector<uint32_t> none;
     std::vector<int64_t> gwoff, gnw;
     std::vector<wv_huff *> gb;
+    uint32_t *dev = nullptr;
     const int rc = encode_batch(bp.data(), cnt.data(), cp.data(), nj, bins, dup, ws.p, wsb,
-                                nullptr, 0, gw, gwoff, gnw, gb, (hipStream_t)stream);
+                                nullptr, 0, none, gwoff, gnw, gb, (hipStream_t)stream, &dev);
     if (rc != WV_OK) {
       free_books();
       return rc;
     }
+    gdev[gi].p = dev;
     for (int k = 0; k < nj; k++) {
-      gof[ix[k]] = (int)gwords.size();
+      gof[ix[k]] = gi;
       iwoff[ix[k]] = gwoff[k];
       inw[ix[k]] = gnw[k];
       books[ix[k]] = gb[k];
     }
-    gwords.emplace_back();
-    gwords.back().swap(gw);
+    gi++;
   }
   std::vector<wv_cube_desc> descs((size_t)n);
   std::vector<std::string> params((size_t)n);
@@ -359,21 +362,45 @@ extern "C" int64_t wv_batch_write_stream(wv_batch *b, const int *offsets, int co
       });
     for (auto &th : pool) th.join();
   }
-  StreamHandle sh;
-  sh.w = wv_cube_stream_open();
-  size_t est = 0;
-  for (auto &g : gwords) est += g.size() * 4;
-  for (auto &pm : params) est += pm.size() * 2 + 256;
-  cube_stream_reserve(sh.w, est);
+  // the layout of the stream: header, then per record its length, message
+  // and payload, then the index footer
   static const int zero3[3] = {0, 0, 0};
-  for (int i = 0; i < n; i++)
-    SCHK(cube_stream_add_params(sh.w, &descs[i], params[i], offsets ? offsets + 3 * i : zero3,
-                                gwords[gof[i]].data() + iwoff[i], inw[i]));
-  const int64_t need = wv_cube_stream_bytes(sh.w, with_footer);
+  std::vector<std::string> metas((size_t)n), indexed((size_t)n);
+  std::vector<int64_t> data_at((size_t)n);
+  int64_t pos = 64;
+  for (int i = 0; i < n; i++) {
+    compressed_cubelet_msgs(&descs[i], params[i], (uint32_t)(inw[i] * 4), (uint64_t)pos + 4,
+                            offsets ? offsets + 3 * i : zero3, metas[i], indexed[i]);
+    data_at[i] = pos + 4 + (int64_t)metas[i].size();
+    pos = data_at[i] + inw[i] * 4;
+  }
+  const std::string tail = with_footer ? cube_index_tail(indexed) : std::string();
+  const int64_t need = pos + (int64_t)tail.size();
   if (need_out) *need_out = need;
   if (!file_buf || cap < need) {
     set_error("wv_batch_write_stream: file buffer %lld < %lld", (long long)cap, (long long)need);
     return WV_ERR_NOMEM;
   }
-  return wv_cube_stream_finish(sh.w, with_footer, file_buf, cap);
+  // the payloads go from the device straight into the file image
+  hipStream_t s = (hipStream_t)stream;
+  std::memset(file_buf, 0, 64);
+  std::memcpy(file_buf, "SCU cubelets 1.0\n", 17);
+  for (int i = 0; i < n; i++) {
+    const uint32_t mlen = (uint32_t)metas[i].size();
+    uint8_t *r = file_buf + data_at[i] - 4 - mlen;
+    std::memcpy(r, &mlen, 4);
+    std::memcpy(r + 4, metas[i].data(), mlen);
+    if (inw[i] > 0 &&
+        hipMemcpyAsync(file_buf + data_at[i], (const uint32_t *)gdev[gof[i]].p + iwoff[i],
+                       (size_t)inw[i] * 4, hipMemcpyDeviceToHost, s) != hipSuccess) {
+      set_error("wv_batch_write_stream: payload D2H");
+      return WV_ERR_HIP;
+    }
+  }
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    set_error("wv_batch_write_stream: sync");
+    return WV_ERR_HIP;
+  }
+  if (!tail.empty()) std::memcpy(file_buf + pos, tail.data(), tail.size());
+  return need;
 }
