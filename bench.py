@@ -543,7 +543,7 @@ def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(cubelets shard across GPUs with no exchange; see DESIGN.md)",
+              "(cubelets shard across GPUs with no exchange)",
               file=sys.stderr)
         sys.exit(2)
     import torch
