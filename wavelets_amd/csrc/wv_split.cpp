@@ -8,10 +8,12 @@
 // strides (no extraction copy), the whole set runs as one batch (grouped
 // persistent pass launches, batched select and quantize), each cubelet is
 // Huffman-coded on the device, and the payloads are written as one .cube
-// stream with the boxes' offsets.  All boxes are coded together: their
-// run statistics in one launch per phase, the code books on host threads,
-// all payloads in one launch per phase (a box coded alone is a chain of
-// small launches and two synchronisations, which left the device idle).
+// stream with the boxes' offsets (wv_batch_write_stream).  All boxes are
+// coded together: their run statistics in one launch per phase, the code
+// books on host threads, all payloads in one launch per phase (a box coded
+// alone is a chain of small launches and two synchronisations, which left
+// the device idle), and the payloads go from the device straight into the
+// file image.
 #include <hip/hip_runtime.h>
 #include <algorithm>
 #include <map>
@@ -104,11 +106,6 @@ struct BatchHandle {
   }
 };
 
-struct StreamHandle {
-  wv_cube_stream *w = nullptr;
-  ~StreamHandle() { wv_cube_stream_free(w); }
-};
-
 #define SCHK(call)                  \
   do {                              \
     int _rc = (call);               \
@@ -165,14 +162,6 @@ extern "C" int64_t wv_split_compress(const void *vol_d, int dtype, const int siz
     off_bins[i] = total; total += al(n * 4);
     off_counts[i] = total; total += al(((int64_t)opt->bin_count + 1) * 4);
   }
-  // coding: one workspace and one payload buffer for all boxes; every box
-  // has the same zero bin (quant(0) = (bins - 1) >> 1 for uniform and log)
-  const int dup = opt->compress_zeros ? (opt->bin_count - 1) >> 1 : -1;
-  std::vector<int64_t> counts_n(nbox);
-  for (int i = 0; i < nbox; i++) counts_n[i] = geoms[i].count;
-  const int64_t enc_ws = encode_batch_ws_bytes(counts_n.data(), nbox, opt->bin_count, dup);
-  const int64_t off_ws = total;
-  total += al(enc_ws);
   DevBuf arena;
   SHIP(hipMalloc(&arena.p, (size_t)total), "wv_split_compress: device buffers");
   char *ar = (char *)arena.p;
@@ -205,55 +194,14 @@ extern "C" int64_t wv_split_compress(const void *vol_d, int dtype, const int siz
   SCHK(wv_batch_submit(bh.b, stream));
   SCHK(wv_batch_wait(bh.b));
   const double ms_batch = ms_since(t1);
-  // code all boxes together, then append them to the stream in box order
+  // every box coded on the device together and written with its offset,
+  // the payloads copied from the device straight into file_buf
   const auto t2 = std::chrono::steady_clock::now();
-  std::vector<const int32_t *> bins_p(nbox);
-  std::vector<const uint32_t *> counts_p(nbox);
-  for (int i = 0; i < nbox; i++) {
-    bins_p[i] = items[i].bins_d;
-    counts_p[i] = items[i].counts_d;
-  }
-  std::vector<uint32_t> words;
-  std::vector<int64_t> woff, nwords;
-  std::vector<wv_huff *> books;
-  SCHK(encode_batch(bins_p.data(), counts_n.data(), counts_p.data(), nbox, opt->bin_count, dup,
-                    ar + off_ws, enc_ws, nullptr, 0, words, woff, nwords, books,
-                    (hipStream_t)stream));
-  // records: descriptions, then the parameters messages (decode tables)
-  // and the freeing of the code books on host threads, then the stream in
-  // box order
-  std::vector<wv_cube_desc> descs((size_t)nbox);
-  std::vector<std::string> params((size_t)nbox);
-  int rc = WV_OK;
-  for (int i = 0; i < nbox && rc == WV_OK; i++) {
-    wv_thresh_result res;
-    int zb = 0;
-    rc = wv_batch_result(bh.b, i, &res, &zb);
-    if (rc == WV_OK) cube_desc_of(geoms[i], opt, dtype, res, {}, {}, zb, descs[i]);
-  }
-  {
-    const int nt = std::max(1, std::min(16, nbox / 16));
-    std::vector<std::thread> pool;
-    for (int t = 0; t < nt; t++)
-      pool.emplace_back([&, t] {
-        for (int i = t; i < nbox; i += nt) {
-          if (rc == WV_OK) params[i] = compressed_params_msg(&descs[i], books[i]);
-          wv_huff_free(books[i]);
-          books[i] = nullptr;
-        }
-      });
-    for (auto &th : pool) th.join();
-  }
-  if (rc != WV_OK) return rc;
-  StreamHandle sh;
-  sh.w = wv_cube_stream_open();
-  cube_stream_reserve(sh.w, words.size() * 4 + (size_t)nbox * (params.empty() ? 0 : params[0].size() + 256));
-  for (int i = 0; i < nbox; i++) {
-    rc = cube_stream_add_params(sh.w, &descs[i], params[i], boxes[i].offset,
-                                words.data() + woff[i], nwords[i]);
-    if (rc != WV_OK) return rc;
-  }
-  const int64_t need = wv_cube_stream_bytes(sh.w, opt->with_footer);
+  std::vector<int> offs((size_t)nbox * 3);
+  for (int i = 0; i < nbox; i++) std::memcpy(&offs[(size_t)i * 3], boxes[i].offset, 12);
+  int64_t need = 0;
+  const int64_t nb = wv_batch_write_stream(bh.b, offs.data(), opt->compress_zeros,
+                                           opt->with_footer, file_buf, cap, &need, stream);
   if (info) {
     info->cubelets = nbox;
     info->need = need;
@@ -262,11 +210,7 @@ extern "C" int64_t wv_split_compress(const void *vol_d, int dtype, const int siz
     info->ms_encode = ms_since(t2);
     info->ms_total = ms_since(t0);
   }
-  if (!file_buf || cap < need) {
-    set_error("wv_split_compress: file buffer %lld < %lld", (long long)cap, (long long)need);
-    return WV_ERR_NOMEM;
-  }
-  return wv_cube_stream_finish(sh.w, opt->with_footer, file_buf, cap);
+  return nb;
 }
 
 // A finished batch as one .cube stream: items grouped by (bin count, dup
@@ -381,23 +325,47 @@ extern "C" int64_t wv_batch_write_stream(wv_batch *b, const int *offsets, int co
     set_error("wv_batch_write_stream: file buffer %lld < %lld", (long long)cap, (long long)need);
     return WV_ERR_NOMEM;
   }
-  // the payloads go from the device straight into the file image
+  // large records: each payload goes from the device straight to its
+  // place in the file image; small ones (where a copy's fixed cost
+  // dominates): one copy of each group's payloads, then host copies
   hipStream_t s = (hipStream_t)stream;
   std::memset(file_buf, 0, 64);
   std::memcpy(file_buf, "SCU cubelets 1.0\n", 17);
+  int64_t payload_bytes = 0;
+  for (int i = 0; i < n; i++) payload_bytes += inw[i] * 4;
+  const bool direct = payload_bytes >= (int64_t)n * (128 << 10);   // >= 128 KB per record
+  std::vector<std::vector<uint32_t>> staged(direct ? 0 : gdev.size());
+  if (!direct) {
+    std::vector<int64_t> gwords(gdev.size(), 0);
+    for (int i = 0; i < n; i++) gwords[gof[i]] = std::max(gwords[gof[i]], iwoff[i] + inw[i]);
+    for (size_t g = 0; g < gdev.size(); g++) {
+      staged[g].resize((size_t)gwords[g]);
+      if (gwords[g] > 0 && hipMemcpyAsync(staged[g].data(), gdev[g].p, (size_t)gwords[g] * 4,
+                                          hipMemcpyDeviceToHost, s) != hipSuccess) {
+        set_error("wv_batch_write_stream: payload D2H");
+        return WV_ERR_HIP;
+      }
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) {
+      set_error("wv_batch_write_stream: sync");
+      return WV_ERR_HIP;
+    }
+  }
   for (int i = 0; i < n; i++) {
     const uint32_t mlen = (uint32_t)metas[i].size();
     uint8_t *r = file_buf + data_at[i] - 4 - mlen;
     std::memcpy(r, &mlen, 4);
     std::memcpy(r + 4, metas[i].data(), mlen);
-    if (inw[i] > 0 &&
-        hipMemcpyAsync(file_buf + data_at[i], (const uint32_t *)gdev[gof[i]].p + iwoff[i],
-                       (size_t)inw[i] * 4, hipMemcpyDeviceToHost, s) != hipSuccess) {
+    if (inw[i] <= 0) continue;
+    if (!direct) {
+      std::memcpy(file_buf + data_at[i], staged[gof[i]].data() + iwoff[i], (size_t)inw[i] * 4);
+    } else if (hipMemcpyAsync(file_buf + data_at[i], (const uint32_t *)gdev[gof[i]].p + iwoff[i],
+                              (size_t)inw[i] * 4, hipMemcpyDeviceToHost, s) != hipSuccess) {
       set_error("wv_batch_write_stream: payload D2H");
       return WV_ERR_HIP;
     }
   }
-  if (hipStreamSynchronize(s) != hipSuccess) {
+  if (direct && hipStreamSynchronize(s) != hipSuccess) {
     set_error("wv_batch_write_stream: sync");
     return WV_ERR_HIP;
   }
