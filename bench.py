@@ -12,13 +12,21 @@ stream's input bytes per second.  Beside it, on the same stream:
   plain     the single-cubelet drop-in (wv_compress_device) run cubelet by
             cubelet over the first cubelets of the stream, and whether its
             bins equal the batch's;
+  stages    one eager run of the batch serialised on one stream, HIP events
+            between its stages: each stage's algorithmic bytes (transform
+            s_in + 20, select 4, quantize + histogram 8 per sample) over its
+            time.  The transform (the fused DWT pass over every group) is
+            the dominant kernel: the line's roofline;
+  copy      the HBM rate of a 128-bit streaming copy kernel (the measured
+            ceiling beside the 8 TB/s peak);
   single    the 512^3 float32 cubelet (40 ellipsoids + N(0,20^2) noise),
-            CDF 9/7 3,3,3, keep 5%, uniform 1024 bins, whose fused-axis
-            DWT pass is the roofline kernel: its algorithmic bytes (read 4 B
-            + write 4 B per sample) over the average pass duration from HIP
-            events recorded on the pipeline's own stream;
-  cpu       the oracle (single-threaded C restatement of the reference CPU
-            path) on a bounded sample of the 512^3 cubelet.
+            CDF 9/7 3,3,3, keep 5%, uniform 1024 bins, with its fused DWT
+            pass's roofline: algorithmic bytes (read 4 B + write 4 B per
+            sample) over the average pass duration from HIP events recorded
+            on the pipeline's own stream, and measured HBM traffic;
+  cpu       the oracle (a C restatement of the reference CPU path) on a
+            bounded sample of the mixed stream, one thread and one worker
+            per host core, and single-threaded on a 512^3 sub-volume.
 
 Prints one JSON line.
 """
@@ -91,6 +99,108 @@ def cpu_baseline(vol_sample):
                       "= %.2f s single-threaded (Huffman %.2f s not counted)" % (
                           vol_sample.shape[2], vol_sample.shape[1], vol_sample.shape[0],
                           t, r.t_huff)}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def host_cores():
+    """The host cores this process may run on (the GPU box's share)."""
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline_stream(cubes, target_s=12.0):
+    """The oracle on a bounded sample of the mixed stream -- the first
+    cubelets, copied to the host, until about target_s of single-threaded
+    work -- timed two ways: one thread over the whole sample, and one worker
+    per host core over disjoint cubelets of it.  Stages timed: DWT +
+    threshold + quantize (the GPU step's work; the oracle's Huffman coding
+    is not counted)."""
+    import concurrent.futures as cf
+    tests = os.path.join(ROOT, "tests")
+    sys.path.insert(0, tests)
+    import _oracle as O
+    items, t_single, nbytes = [], 0.0, 0
+    for c in cubes:
+        v = c["vol"].cpu().numpy()
+        item = (v.astype(np.int32) if v.dtype == np.int16 else v, c)
+        r = O.compress(item[0], c["wavelet"], O.LOG, c["bins"], c["steps"], c["frac"],
+                       want_file=False)["res"]
+        t_single += r.t_dwt + r.t_thresh + r.t_quant
+        nbytes += v.nbytes
+        items.append(item)
+        if t_single >= target_s:
+            break
+    cores = host_cores()
+
+    def work(k):
+        t = 0.0
+        for v, c in items[k::cores]:
+            r = O.compress(v, c["wavelet"], O.LOG, c["bins"], c["steps"], c["frac"],
+                           want_file=False)["res"]
+            t += r.t_dwt + r.t_thresh + r.t_quant
+        return t
+
+    t0 = time.time()
+    with cf.ThreadPoolExecutor(max_workers=cores) as ex:
+        per_worker = list(ex.map(work, range(cores)))
+    wall = time.time() - t0
+    t_multi = max(per_worker)
+    model = cpu_model()
+    return {"value": round(nbytes / t_multi / 1e9, 4), "unit": "GB/s", "cores": cores,
+            "kind": "port",
+            "sample": "the first %d cubelets of the mixed stream (%.2f GB of input, copied to "
+                      "the host), DWT + threshold (sort) + log 256 quantize per cubelet; one "
+                      "worker per host core over disjoint cubelets: %.2f s of stage time in the "
+                      "busiest worker (%.2f s wall with Huffman)" % (
+                          len(items), nbytes / 1e9, t_multi, wall),
+            "single_thread": {"value": round(nbytes / t_single / 1e9, 4), "cores": 1,
+                              "seconds": round(t_single, 2)},
+            "cpu_model": model, "nproc": os.cpu_count(), "host_cores_used": cores}
+
+
+def copy_ceiling(wv, torch, dev, nbytes=1 << 30, reps=10):
+    """The HBM rate a streaming copy reaches on this GPU: wv_copy_stream
+    (128-bit accesses, 16 loads in flight per thread) over 1 GiB, the best
+    of its access hints x blocks per CU; bytes moved = read + write."""
+    src = torch.empty(nbytes // 4, dtype=torch.float32, device=dev).uniform_()
+    dst = torch.empty_like(src)
+    best = None
+    for nt in (0, 1):
+        for bpc in (2, 4, 8, 16):
+            for _ in range(3):
+                wv.copy_stream(src, dst, nt, bpc)
+            e0, e1 = wv.Event(), wv.Event()
+            e0.record()
+            for _ in range(reps):
+                wv.copy_stream(src, dst, nt, bpc)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_ms(e1) / reps
+            gbps = 2.0 * nbytes / (ms * 1e-3) / 1e9
+            if best is None or gbps > best[0]:
+                best = (gbps, nt, bpc, ms)
+    ok = bool(torch.equal(src, dst))
+    del src, dst
+    torch.cuda.empty_cache()
+    gbps, nt, bpc, ms = best
+    log("copy ceiling: %.1f GB/s (%s, %d blocks per CU)" % (gbps, "non-temporal" if nt else
+                                                            "plain", bpc))
+    return {"gbps": round(gbps, 1), "frac_of_peak": round(gbps / HBM_PEAK_GBS, 4),
+            "ms": round(ms, 4), "bytes_copied": nbytes, "nontemporal": bool(nt),
+            "blocks_per_cu": bpc, "copy_verified": ok,
+            "kernel": "wv_copy_stream (copy_kernel: 16 x 16-byte loads in flight per thread, "
+                      "64 KiB chunks per block)"}
 
 
 def measured_traffic():
@@ -379,6 +489,29 @@ def mixed_stream(args, wv, torch, dev, steps, warmup):
     ms_cube = (t2 - t1) * 1e3
     log("batched to .cube: submit %.1f ms + coding/stream %.1f ms, %d bytes"
         % ((t1 - t0) * 1e3, ms_cube, len(f)))
+    # per-stage times: eager runs serialised on one stream with events
+    # between the stages (median of 3), against each stage's algorithmic
+    # bytes: transform s_in + 4 + 8 (npass - 1), select 4, quantize 8 per sample
+    runs = [b.stage_times() for _ in range(3)]
+    st = {k: float(np.median([r[k] for r in runs])) for k in runs[0]}
+    samples = int(sum(c["vol"].numel() for c in cubes))
+    t_bytes = 0
+    for c in cubes:
+        npass = sum(1 for v in c["steps"] if v > 0)
+        t_bytes += c["vol"].numel() * (c["vol"].element_size() + 4 + 8 * (npass - 1))
+    stage_bytes = {"transform": t_bytes, "select": 4 * samples, "quantize": 8 * samples}
+    stages = {}
+    for k in ("transform", "select", "quantize"):
+        ms = st[k + "_ms"]
+        gbps = stage_bytes[k] / (ms * 1e-3) / 1e9
+        stages[k] = {"ms": round(ms, 3), "algorithmic_bytes": int(stage_bytes[k]),
+                     "gbps": round(gbps, 1), "frac": round(gbps / HBM_PEAK_GBS, 4)}
+    stages["transform"]["launches"] = int(st["transform_launches"])
+    stream_bytes = in_bytes + 32 * samples
+    log("stages (serialised): transform %.2f ms (%.2f), select %.2f ms (%.2f), quantize %.2f "
+        "ms (%.2f)" % (stages["transform"]["ms"], stages["transform"]["frac"],
+                       stages["select"]["ms"], stages["select"]["frac"],
+                       stages["quantize"]["ms"], stages["quantize"]["frac"]))
     # select statistics: candidates examined, exact fallbacks (top bit)
     nc = [b.result(i)[0].n_candidates for i in range(len(cubes))]
     fallbacks = sum(1 for v in nc if v < 0)
@@ -402,11 +535,19 @@ def mixed_stream(args, wv, torch, dev, steps, warmup):
                         "payload D2H and the one .cube stream on the host",
         "select_fallbacks": fallbacks,
         "select_max_candidates": max(v & 0x7fffffff for v in nc),
+        "stream_algorithmic_bytes": int(stream_bytes),
+        "stream_frac": round(stream_bytes / (ms_batch * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "stream_note": "(s_in + 32) B per sample over the whole submit: transform s_in + 20, "
+                       "select 4, quantize + histogram 8",
+        "stages": stages,
+        "stages_note": "one eager run serialised on one stream (no stream overlap, no graph) "
+                       "with HIP events between stages, median of 3; transform = sum of the "
+                       "groups' pass-launch spans",
     }
     b.close()
-    del cubes, work
+    del work
     torch.cuda.empty_cache()
-    return out
+    return out, cubes
 
 
 def split_leg(args, wv, torch, dev, vol):
@@ -485,17 +626,12 @@ def serving(args, wv, torch, dev):
     kept fraction uniform on [0.02, 0.2] (thresh = 1 - kept).  Latency is
     submit -> bins, counts and .cube image on the host."""
     from wavelets_amd import synth
-    rng = np.random.default_rng(5)
+    rng = np.random.default_rng(6)          # arrival times
     N = args.serve_count
     reqs, in_bytes = [], 0
     t0 = time.time()
-    for i in range(N):
-        ex = int(np.clip(np.round(np.exp(rng.normal(np.log(128), 0.6))), 32, 512))
-        ey = int(np.clip(np.round(np.exp(rng.normal(np.log(128), 0.6))), 32, 512))
-        ez = int(np.clip(np.round(np.exp(rng.normal(np.log(48), 1.0))), 1, 512))
-        dt = ("uint8", "int16", "float32")[int(rng.integers(3))]
-        keep = float(rng.uniform(0.02, 0.2))
-        vol = synth.cubelet_device((ex, ey, ez), dt, 5000 + i, device=dev).cpu().numpy()
+    for i, (size, dt, keep) in enumerate(synth.serving_requests(N, seed=5)):
+        vol = synth.cubelet_device(size, dt, 5000 + i, device=dev).cpu().numpy()
         in_bytes += vol.nbytes
         reqs.append((vol, 1.0 - keep))
     log("generated %d serving requests (%.2f GB) in %.1f s" % (N, in_bytes / 1e9, time.time() - t0))
@@ -555,9 +691,11 @@ def main():
     single, roof, vol = single_cubelet(args, wv, torch, dev,
                                        args.steps if args.workload == "single" else 20,
                                        args.warmup)
-    batched = None
+    copy = copy_ceiling(wv, torch, dev)
+    single["roofline"] = roof
+    batched = cubes = None
     if args.workload == "batched":
-        batched = mixed_stream(args, wv, torch, dev, args.steps, args.warmup)
+        batched, cubes = mixed_stream(args, wv, torch, dev, args.steps, args.warmup)
 
     serve = None
     if "serving" in args.extras and args.workload == "batched":
@@ -573,12 +711,28 @@ def main():
     if not args.no_cpu:
         q = max(32, args.size // 2)
         sample = np.ascontiguousarray(vol[:q, :q, :])
-        cpu = cpu_baseline(sample)
+        single["cpu_baseline"] = cpu_baseline(sample)
+        cpu = cpu_baseline_stream(cubes) if cubes is not None else single["cpu_baseline"]
+    del cubes
+    torch.cuda.empty_cache()
 
     if batched is not None:
         value, ms_step = batched["batched_gbps"], batched["batched_ms_per_stream"]
         workload = batched["workload"]
         dtype = "f32"
+        # the headline's dominant kernel: the fused DWT pass over every
+        # group of the stream (serialised launches, HIP events on their stream)
+        t = batched["stages"]["transform"]
+        roof = {"bound": "hbm", "achieved": t["gbps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": t["frac"], "traffic": None,
+                "kernel": "dwt_pass_kernel: every transform pass of the 2000-cubelet stream "
+                          "(%d launches over 30 groups, run serialised)" % t["launches"],
+                "algorithmic_bytes": t["algorithmic_bytes"], "ms": t["ms"],
+                "per_stage": {k: batched["stages"][k]["frac"]
+                              for k in ("transform", "select", "quantize")},
+                "stream_frac": batched["stream_frac"],
+                "copy_ceiling_gbps": copy["gbps"],
+                "frac_of_copy_ceiling": round(t["gbps"] / copy["gbps"], 4)}
     else:
         value, ms_step = single["gbps_input"], single["ms_per_step"]
         workload = single["workload"]
@@ -599,6 +753,7 @@ def main():
         "config": {"workload": workload},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "copy_ceiling": copy,
         "single_cubelet": single,
     }
     if batched is not None:

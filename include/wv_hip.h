@@ -278,6 +278,15 @@ int wv_encode_device(const int32_t *bins_d, int64_t n, const uint32_t *counts_d,
                      int64_t words_cap, int64_t *nwords_out, wv_huff **book_out,
                      void *stream);
 int64_t wv_huff_payload_bits(const wv_huff *book);
+/* One leaf of a code book: value 0..bin_count-1, or bin_count (the dup
+   key, when the book has one).  *count: the value's count in the tree (the
+   zero bin's singles left over from dup strings; the dup key's: the dup
+   strings); *bit_length: its code length (0 when it does not occur);
+   bits: the code in stream order, bit i of bits[i / 32] written i-th
+   (first = the root's branch; at most 64 bits).  What
+   Huffman::printEncoding prints (huffman.cc:100-107, the -enc option). */
+int wv_huff_code(const wv_huff *book, int value, int64_t *count, int *bit_length,
+                 uint32_t bits[2]);
 /* the .cube image around a device-coded payload (host copy of its words):
    the same bytes wv_write_cube writes for the same bins */
 int64_t wv_write_cube_payload(const wv_cube_desc *d, const wv_huff *book,
@@ -447,6 +456,12 @@ int wv_batch_groups(const wv_batch *b);
 int wv_batch_group_tuning(const wv_batch *b, int g, int *tile_cap, int *ept0, int *bpc,
                           int *ncub);
 int wv_batch_destroy(wv_batch *b);
+/* measurement: one eager run of the batch serialised on stream (no side
+   streams, no graphs) with events between stages; stage_ms[0..3] = the
+   transform passes (sum of each group's span), the selects, the quantize
+   passes, and the number of transform pass launches.  Same outputs as a
+   submit; synchronises.  No counterpart in the reference. */
+int wv_batch_stage_times(wv_batch *b, void *stream, double *stage_ms);
 /* after a submit: every item Huffman-coded on the device (items grouped by
    bin count, each group in one launch per phase), written as one .cube
    stream of compressed cubelets in item order, at offsets[3 * i] (NULL:
@@ -594,6 +609,14 @@ int64_t wv_opt_find_sorted(const wv_opt *o, float value);  /* findSorted */
 int wv_opt_test(wv_opt *o, float threshold, int bin_count, int quant_alg, wv_opt_result *r,
                 void *stream);
 int wv_opt_destroy(wv_opt *o);
+
+/* measurement: device-to-device streaming copy with 128-bit accesses (16
+   loads in flight per thread; optional non-temporal hints), blocks_per_cu
+   blocks per CU over 64 KiB chunks; bytes and both pointers 16-byte
+   aligned.  The measured HBM ceiling beside the published peak.  No
+   counterpart in the reference. */
+int wv_copy_stream(const void *src_d, void *dst_d, int64_t bytes, int nontemporal,
+                   int blocks_per_cu, void *stream);
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,10 @@ void or_transpose3d_fwd(const float *src, float *dst, const int size[3]);
 void or_transpose3d_back(const float *src, float *dst, const int size[3]);
 void or_dwt3d(float *data, float *tmp, int size[3], int wavelet,
               const int steps[3], int inverse);
+/* FP64 lifting filter bank over a padded [z][y][x] volume, no rotation
+   (or_dwt64.c): the yardstick for the float transforms' error */
+void or_dwt3d_f64(const float *in, const int size[3], int wavelet, const int steps[3],
+                  double *out);
 
 /* ---- a8: threshold (thresh_cpu.cc:80-109) ----
    sorted_abs (optional, count floats) receives the ascending |x|.

@@ -92,6 +92,7 @@ def _declare(L):
     for f in ("or_cdf97_fwd", "or_cdf97_inv", "or_haar_fwd", "or_haar_inv"):
         getattr(L, f).argtypes = [I, P, I, P]
     L.or_dwt3d.argtypes = [P, P, P, I, P, I]
+    L.or_dwt3d_f64.argtypes = [P, P, I, P, P]
     L.or_thresh.restype = F
     L.or_thresh.argtypes = [P, I, F, P, P, P, P, P, P]
     L.or_quant_init.argtypes = [C.POINTER(Quantizer), I, I, F, F, P, P]
@@ -177,6 +178,30 @@ def dwt3d(data, size, wavelet, steps, inverse=False):
     s = i3(size)
     lib().or_dwt3d(_p(d), _p(tmp), _p(s), wavelet, _p(i3(steps)), int(inverse))
     return d, [int(v) for v in s]
+
+
+def dwt3d_f64(padded_vol, wavelet, steps):
+    """FP64 lifting transform of a padded float32 volume [Z, Y, X]; the
+    result is float64 [Z, Y, X] in the input's layout (no rotation)."""
+    v = np.ascontiguousarray(padded_vol, np.float32)
+    out = np.empty(v.shape, np.float64)
+    size = i3([v.shape[2], v.shape[1], v.shape[0]])
+    lib().or_dwt3d_f64(_p(v), _p(size), wavelet, _p(i3(steps)), _p(out))
+    return out
+
+
+def rotated_rel_error(coef_flat, f64_zyx):
+    """max |coef - f64| / max |f64| for float coefficients in the rotated
+    z,x,y layout (flat, as the transform writes them) against the FP64
+    result in [z, y, x]: the layout maps [z, y, x] -> [y, x, z]."""
+    Z, Y, X = f64_zyx.shape
+    got = np.asarray(coef_flat, np.float32).reshape(Y, X, Z)
+    want = f64_zyx.transpose(1, 2, 0)
+    err = 0.0
+    for y in range(0, Y, 64):          # bounded temporaries
+        d = np.abs(got[y:y + 64].astype(np.float64) - want[y:y + 64])
+        err = max(err, float(d.max()))
+    return err / float(np.abs(f64_zyx).max())
 
 
 def thresh(coef, frac):

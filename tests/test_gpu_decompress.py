@@ -176,3 +176,19 @@ def test_device_decoder_short_stream_is_an_error(wv):
     words, _ = h.encode()
     with pytest.raises(wv.WvError):
         wv.huff_decode_device(h.decode_table(), 256, zb, words[: len(words) // 2], x.size)
+
+
+def test_device_decoder_rejects_leaf_outside_range(wv):
+    """The device decoder's lookup entries keep 27 bits of a leaf value:
+    a table whose leaf exceeds the largest bin count is refused on the host
+    before any launch."""
+    x = np.array([0, 1, 1, 2, 2, 2, 3, 3] * 64, np.int32)
+    h = O.Huffman(x, 4, -1)
+    words, _ = h.encode()
+    table = h.decode_table().copy()
+    got, _ = wv.huff_decode_device(table, -1, -1, words, x.size)
+    assert np.array_equal(got, x)
+    leaf = next(k for k in range(0, table.size, 2) if table[k] == 0)
+    table[leaf + 1] = 1 << 28
+    with pytest.raises(wv.WvError, match="malformed"):
+        wv.huff_decode_device(table, -1, -1, words, x.size)

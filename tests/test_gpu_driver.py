@@ -148,3 +148,73 @@ def test_driver_opt_table_matches_oracle(tmp_path):
         size, _, psnr, _ = O.opt_test(vol, [3, 3, 3], O.CDF97, 0, True, t, b, O.LOG)
         assert int(rows[k][2]) == size, k
         assert abs(float(rows[k][3]) - psnr) <= 1e-4 * abs(psnr), k
+
+
+def _codes_from_decode_table(table):
+    """Every leaf's code, root first (0 = left child), from a decode table
+    (pairs; a leaf is (0, value); internal pairs hold child ids)."""
+    codes, stack = {}, [(0, "")]
+    while stack:
+        node, path = stack.pop()
+        a, b = int(table[2 * node]), int(table[2 * node + 1])
+        if a == 0:
+            codes[b] = path
+        else:
+            stack.append((a, path + "0"))
+            stack.append((b, path + "1"))
+    return codes
+
+
+@pytest.mark.parametrize("noz", [False, True])
+def test_driver_enc_prints_the_encoding(tmp_path, noz):
+    """-enc prints Huffman::printEncoding's table (huffman.cc:100-107): the
+    values that occur, by decreasing count (ties by value), as
+    "value (count): length bits"; checked against the oracle's counts and
+    the codes read off the decode table of the file written."""
+    vol = _vol((16, 24, 32), np.uint8, 5)
+    src, dst = str(tmp_path / "in.cube"), str(tmp_path / "out.cube")
+    _write_input_cube(src, vol)
+    args = [DRIVER, "-q", "-enc", "-steps", "3", "-thresh", "0.8", "-qcount", "64"]
+    if noz:
+        args.append("-noz")
+    r = subprocess.run(args + [src, dst], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    want = O.compress(vol, O.CDF97, O.LOG, 64, [3, 3, 3], 0.8, compress_zeros=not noz,
+                      want_bins=True)
+    h = O.Huffman(want["bins"], 64, -1 if noz else want["res"].zero_bin)
+    counts = h.counts()
+    codes = _codes_from_decode_table(h.decode_table())
+    leaves = [(v, int(counts[v])) for v in range(65 if not noz else 64) if counts[v] > 0]
+    leaves.sort(key=lambda t: (-t[1], t[0]))
+    lines = ["%3d (%8d): %d %s" % (v, c, len(codes[v]), codes[v]) for v, c in leaves]
+    got = [l for l in r.stdout.splitlines() if re.match(r"^\s*\d+ \(\s*\d+\): \d+ [01]*$", l)]
+    assert got == lines
+    assert open(dst, "rb").read() == want["file"]
+
+
+@pytest.mark.parametrize("flags", [["-bq", "before.dat", "-aq", "after.dat", "-pb", "-v"],
+                                   ["-experiment"]])
+def test_driver_accepts_reference_flags(tmp_path, flags):
+    """Options the reference GPU driver parses without effect are accepted
+    and leave the file unchanged; -experiment implies -q."""
+    vol = _vol((8, 16, 16), np.uint8, 6)
+    src, dst = str(tmp_path / "in.cube"), str(tmp_path / "out.cube")
+    _write_input_cube(src, vol)
+    r = subprocess.run([DRIVER] + flags + ["-thresh", "0.9", src, dst], capture_output=True,
+                       text=True, timeout=120, cwd=str(tmp_path))
+    assert r.returncode == 0, r.stderr
+    assert open(dst, "rb").read() == O.compress(vol, O.CDF97, O.LOG, 256, [-1, -1, -1],
+                                                0.9)["file"]
+    assert not os.path.exists(tmp_path / "before.dat")
+    if flags == ["-experiment"]:
+        assert r.stdout.strip() == ""
+
+
+def test_driver_rejects_2d_like_the_reference(tmp_path):
+    vol = _vol((8, 16, 16), np.uint8, 7)
+    src, dst = str(tmp_path / "in.cube"), str(tmp_path / "out.cube")
+    _write_input_cube(src, vol)
+    r = subprocess.run([DRIVER, "-2d", src, dst], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "-2d option not implemented for test_compress_gpu yet." in r.stderr
+    assert not os.path.exists(dst)

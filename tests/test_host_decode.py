@@ -131,3 +131,56 @@ def test_write_cube_raw_matches_protobuf(wv, dtype, shape):
     ib = index.SerializeToString()
     want = head + vol.tobytes() + struct.pack("<i", -1) + ib + struct.pack("<i", len(ib)) + b"cube"
     assert got == want
+
+
+@pytest.mark.parametrize("bad", [-1, 65, 1 << 27, 1 << 30])
+def test_cube_parse_rejects_decode_leaf_outside_bins(wv, bad):
+    """A decode-table leaf holds a bin (0..bin_count-1) or the dup key
+    (bin_count); the reader rejects any other value, so the host and the
+    device decoders never see a table they would read differently."""
+    import struct
+    vol = _vol((8, 16, 16), np.int32, 3)
+    f = O.compress(vol, O.HAAR, O.LOG, 64, [2, 2, 2], 0.8)["file"]
+    wv.cube_parse(f)                                  # the oracle's file is fine
+    mlen = struct.unpack_from("<I", f, 64)[0]
+    cb = P.messages()["CubeletBuffer"].FromString(f[68:68 + mlen])
+    t = cb.compression_parameters.huffman_decode_table
+    leaf = next(h for h in range(0, len(t), 2) if t[h] == 0)
+    t[leaf + 1] = bad
+    meta = cb.SerializeToString()
+    g = f[:64] + struct.pack("<I", len(meta)) + meta + f[68 + mlen:]
+    with pytest.raises(wv.WvError, match="leaf"):
+        wv.cube_parse(g)
+
+
+def test_parse_and_host_decode_survive_corrupted_files(wv):
+    """Mutated and truncated .cube images: the reader and the host decoder
+    either reject them (WvError) or return a result -- never read or write
+    out of bounds (the host-sanitized build runs this too)."""
+    rng = np.random.default_rng(11)
+    vol = _vol((8, 16, 24), np.int32, 4)
+    base = O.compress(vol, O.CDF97, O.LOG, 64, [2, 2, 2], 0.7)["file"]
+    outcomes = {"rejected": 0, "parsed": 0}
+    for k in range(300):
+        f = bytearray(base)
+        if k % 3 == 0:
+            f = f[:int(rng.integers(70, len(f)))]
+        for _ in range(int(rng.integers(1, 6))):
+            pos = int(rng.integers(64, min(len(f), 64 + 400)))
+            f[pos] = int(rng.integers(0, 256))
+        try:
+            info, table, _, _ = wv.cube_parse(bytes(f))
+        except wv.WvError:
+            outcomes["rejected"] += 1
+            continue
+        outcomes["parsed"] += 1
+        end = min(len(f), info.payload_offset + info.payload_bytes)
+        words = np.frombuffer(bytes(f[info.payload_offset:end - (end - info.payload_offset) % 4]),
+                              np.uint32)
+        count = int(min(1 << 20, max(1, info.size[0]) * max(1, info.size[1]) *
+                        max(1, info.size[2])))
+        try:
+            wv.huff_decode_host(table, info.dup_key, info.dup_value, words, count)
+        except wv.WvError:
+            pass
+    assert outcomes["rejected"] > 0 and outcomes["parsed"] > 0

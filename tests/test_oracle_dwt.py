@@ -205,3 +205,45 @@ def test_repeat_padding():
     assert out.shape == (2, 4, 8)
     want = vol[np.minimum(np.arange(2), 1)][:, np.minimum(np.arange(4), 2)][:, :, np.minimum(np.arange(8), 4)]
     assert np.array_equal(out, want.astype(np.float32))
+
+
+@pytest.mark.parametrize("wavelet", [O.HAAR, O.CDF97])
+@pytest.mark.parametrize("shape,steps", [((8, 16, 32), [3, 2, 1]), ((4, 2, 64), [6, 1, 2]),
+                                         ((16, 8, 8), [0, 3, 4]), ((2, 4, 16), [4, 2, 1])])
+def test_fp64_3d_oracle_matches_numpy_lifting(wavelet, shape, steps):
+    """or_dwt3d_f64 (the yardstick of the float transforms' error at the
+    benchmark sizes) equals the numpy FP64 lifting above applied axis by
+    axis, and the float oracle to float precision."""
+    rng = np.random.default_rng(sum(shape) + steps[0])
+    vol = (rng.standard_normal(shape) * 100 + 20).astype(np.float32)
+    got = O.dwt3d_f64(vol, wavelet, steps)
+
+    def line(r, s):
+        if wavelet == O.CDF97:
+            return _lifting_ref(r, s)
+        x = np.asarray(r, np.float64).copy()
+        m = len(x)
+        for _ in range(s):
+            a, b = x[0:m:2].copy(), x[1:m:2].copy()
+            x[:m // 2], x[m // 2:m] = (a + b) / np.sqrt(2.0), (a - b) / np.sqrt(2.0)
+            m //= 2
+        return x
+
+    a = vol.astype(np.float64)
+    for axis, s in ((2, steps[0]), (1, steps[1]), (0, steps[2])):
+        if s > 0:
+            a = np.apply_along_axis(lambda r: line(r, s), axis, a)
+    assert np.abs(got - a).max() <= 1e-12 * np.abs(a).max()
+    d, _ = O.dwt3d(vol.ravel(), [shape[2], shape[1], shape[0]], wavelet, steps)
+    assert O.rotated_rel_error(d, got) < 1e-6
+
+
+def test_fp64_error_measure_catches_a_transposed_layout():
+    """rotated_rel_error compares in the z,x,y layout: the float result
+    read in the input's own layout (a transposed operand) fails it."""
+    rng = np.random.default_rng(9)
+    vol = (rng.standard_normal((8, 16, 32)) * 50).astype(np.float32)
+    want = O.dwt3d_f64(vol, O.CDF97, [2, 2, 2])
+    d, _ = O.dwt3d(vol.ravel(), [32, 16, 8], O.CDF97, [2, 2, 2])
+    assert O.rotated_rel_error(d, want) < 1e-6
+    assert O.rotated_rel_error(want.astype(np.float32).ravel(), want) > 1e-2

@@ -14,7 +14,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libwv_hip.so")
+# WV_LIB names another build of the same library (the host-sanitized one)
+LIB_PATH = os.environ.get("WV_LIB") or os.path.join(HERE, "libwv_hip.so")
 
 HAAR, CDF97 = 1, 2
 UNIFORM, LOG, LLOYD = 1, 2, 3
@@ -249,6 +250,10 @@ def _declare(L):
                                             C.POINTER(C.c_int), C.POINTER(C.c_int),
                                             C.POINTER(C.c_int)]
         L.wv_batch_destroy.argtypes = [P]
+        if hasattr(L, "wv_batch_stage_times"):
+            L.wv_batch_stage_times.argtypes = [P, P, C.POINTER(C.c_double)]
+    if hasattr(L, "wv_copy_stream"):
+        L.wv_copy_stream.argtypes = [P, P, I64, C.c_int, C.c_int, P]
     if hasattr(L, "wv_serve_create"):
         L.wv_serve_create.argtypes = [C.POINTER(ServeConfig), C.POINTER(C.c_void_p)]
         L.wv_serve_submit.argtypes = [P, C.POINTER(ServeRequest), C.POINTER(C.c_int64)]
@@ -388,6 +393,16 @@ def threshold(coef, frac, stats=None, stream=None, sync=True):
         return res, None
     host = ThreshResult.from_buffer_copy(res.cpu().numpy().tobytes())
     return res, host
+
+
+def copy_stream(src, dst, nontemporal=True, blocks_per_cu=8, stream=None):
+    """wv_copy_stream: device-to-device copy of src into dst (same byte
+    size, 16-byte multiples) with 128-bit accesses (measurement helper)."""
+    nbytes = src.numel() * src.element_size()
+    if dst.numel() * dst.element_size() != nbytes:
+        raise ValueError("copy_stream: sizes differ")
+    check(lib().wv_copy_stream(_ptr(src), _ptr(dst), nbytes, int(nontemporal),
+                               int(blocks_per_cu), _stream(stream)))
 
 
 class Quantizer:
@@ -816,6 +831,15 @@ class Batch:
             cap = need.value
         check_size(n)
         return buf[:n].tobytes()
+
+    def stage_times(self, stream=None):
+        """One eager run serialised on one stream with events between the
+        stages (wv_batch_stage_times): dict of transform / select / quantize
+        milliseconds and the number of transform pass launches."""
+        ms = (C.c_double * 4)()
+        check(lib().wv_batch_stage_times(self.h, _stream(stream), ms))
+        return dict(transform_ms=ms[0], select_ms=ms[1], quantize_ms=ms[2],
+                    transform_launches=int(ms[3]))
 
     def groups(self):
         out = []

@@ -36,6 +36,7 @@
 // has the same size as the coefficients), so a batch needs no per-cubelet
 // scratch beyond the select state and candidate list.
 #include <hip/hip_runtime.h>
+#include <array>
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -702,6 +703,104 @@ int wv_batch_submit(wv_batch *b, void *stream) {
     (void)hipMemcpyAsync(b->res_h2, b->res_d, sizeof(wv_thresh_result) * n,
                          hipMemcpyDeviceToHost, s);
   return WV_OK;
+}
+
+// One eager run of the whole batch serialised on stream (no side streams,
+// no graphs) with events between its stages: stage_ms[0] the transform
+// passes of every group (the sum of each group's span), [1] the selects,
+// [2] the quantize passes; [3] the number of transform pass launches.
+// Synchronises.  The outputs are those of a submit.
+int wv_batch_stage_times(wv_batch *b, void *stream, double *stage_ms) {
+  if (!b || !stage_ms) { set_error("wv_batch_stage_times: bad arguments"); return WV_ERR_ARG; }
+  hipStream_t s = (hipStream_t)stream;
+  const int n = (int)b->items.size();
+  std::vector<hipEvent_t> ev;
+  auto mark = [&]() -> int {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess || hipEventRecord(e, s) != hipSuccess) {
+      set_error("wv_batch_stage_times: event");
+      return WV_ERR_HIP;
+    }
+    ev.push_back(e);
+    return WV_OK;
+  };
+  auto cleanup = [&] {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  };
+  // segments: (first event, second event, stage)
+  std::vector<std::array<int, 3>> seg;
+  int launches = 0, rc = WV_OK;
+  for (int pt = 0; pt < b->nparts && rc == WV_OK; pt++) {
+    for (size_t g = 0; g < b->groups.size() && rc == WV_OK; g++) {
+      if (b->group_part[g] != pt) continue;
+      const int a = (int)ev.size();
+      if ((rc = mark()) != WV_OK) break;
+      if ((rc = enqueue_group(b, b->groups[g], s, 0)) != WV_OK) break;
+      if ((rc = mark()) != WV_OK) break;
+      seg.push_back({a, a + 1, 0});
+      launches += b->groups[g].npass;
+    }
+    if (rc != WV_OK) break;
+    const int a = (int)ev.size();
+    const int pb = b->part_begin[pt], pe = b->part_begin[pt + 1];
+    if ((rc = mark()) != WV_OK) break;
+    if (threshold_select_batch(b->sel_d + pb, b->selblk_d + pb, pe - pb, b->sel_total_p[pt], s) !=
+        hipSuccess) {
+      set_error("wv_batch_stage_times: select");
+      rc = WV_ERR_HIP;
+      break;
+    }
+    if ((rc = mark()) != WV_OK) break;
+    seg.push_back({a, a + 1, 1});
+  }
+  if (rc == WV_OK && b->host_params) {
+    if (hipMemcpyAsync(b->res_h2, b->res_d, sizeof(wv_thresh_result) * n, hipMemcpyDeviceToHost,
+                       s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess) {
+      set_error("wv_batch_stage_times: threshold results");
+      rc = WV_ERR_HIP;
+    }
+    for (int pt = 0; pt < b->nparts && rc == WV_OK; pt++) {
+      rc = host_params(b, pt, b->res_h2);
+      const int pb = b->part_begin[pt], pe = b->part_begin[pt + 1];
+      if (rc == WV_OK &&
+          (hipMemcpyAsync(b->qtab_d + (size_t)pb * b->qtab_stride,
+                          b->qtab_h + (size_t)pb * b->qtab_stride,
+                          sizeof(float) * (size_t)b->qtab_stride * (pe - pb),
+                          hipMemcpyHostToDevice, s) != hipSuccess ||
+           hipMemcpyAsync(b->qdev_d + pb, b->qdev_h + pb, sizeof(QuantDev) * (pe - pb),
+                          hipMemcpyHostToDevice, s) != hipSuccess)) {
+        set_error("wv_batch_stage_times: parameter upload");
+        rc = WV_ERR_HIP;
+      }
+    }
+  }
+  for (int pt = 0; pt < b->nparts && rc == WV_OK; pt++) {
+    const int a = (int)ev.size();
+    if ((rc = mark()) != WV_OK) break;
+    if ((rc = enqueue_B(b, s, pt)) != WV_OK) break;
+    if ((rc = mark()) != WV_OK) break;
+    seg.push_back({a, a + 1, 2});
+  }
+  if (rc == WV_OK && hipStreamSynchronize(s) != hipSuccess) {
+    set_error("wv_batch_stage_times: sync");
+    rc = WV_ERR_HIP;
+  }
+  if (rc == WV_OK) {
+    stage_ms[0] = stage_ms[1] = stage_ms[2] = 0;
+    for (auto &g : seg) {
+      float ms = 0;
+      (void)hipEventElapsedTime(&ms, ev[g[0]], ev[g[1]]);
+      stage_ms[g[2]] += ms;
+    }
+    stage_ms[3] = launches;
+    if (!b->host_params)
+      (void)hipMemcpy(b->res_h2, b->res_d, sizeof(wv_thresh_result) * n, hipMemcpyDeviceToHost);
+    b->last_stream = s;
+    b->submitted = true;
+  }
+  cleanup();
+  return rc;
 }
 
 int wv_batch_poll(wv_batch *b) {

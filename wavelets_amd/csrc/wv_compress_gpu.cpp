@@ -12,6 +12,15 @@
 //     -err                      round trip on the GPU (dequantize, inverse
 //                               transform) and print MSE / PSNR
 //     -lloyd-iters <n>          Lloyd iteration cap (default: none)
+//     -enc                      print the Huffman encoding of every value
+//                               (Huffman::printEncoding, huffman.cc:100-107)
+//     -v                        verbose: a line per stage after it ran
+//     -bq <f>, -aq <f>, -pb, -experiment
+//                               accepted as the reference GPU driver does
+//                               (parsed, no effect there; -experiment
+//                               implies -q)
+//     -2d                       rejected as the reference GPU driver rejects
+//                               it (test_compress_gpu.cu:165-168)
 //     -q                        quiet
 //     -d                        decompress (test_compress_common.cc:80):
 //                               the first compressed cubelet of the input
@@ -24,6 +33,7 @@
 // cubelet with the index footer, byte for byte what the reference GPU
 // driver writes for the same input.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdint>
@@ -197,7 +207,8 @@ void usage() {
   fprintf(stderr,
           "\n  wv_compress_gpu [opts] <input.cube> <output.cube>\n"
           "    -steps <n|x,y|x,y,z> -thresh <f> -qcount <n> -wave <haar|cdf97>\n"
-          "    -qalg <uniform|log|lloyd> -noz -err -lloyd-iters <n> -q\n"
+          "    -qalg <uniform|log|lloyd> -noz -err -lloyd-iters <n> -q -v -enc\n"
+          "    -bq <file> -aq <file> -pb -experiment (accepted, no effect), -2d (rejected)\n"
           "    -opt : parameter optimisation (the reference's default sweep,\n"
           "           threshold fraction x log bin count -> size, pSNR); no output file\n"
           "  wv_compress_gpu -d [-q] <compressed.cube> <output.cube>\n\n");
@@ -268,6 +279,27 @@ int optimize(const InputCube &cube, const wv_options &opt, bool quiet) {
   return 0;
 }
 
+// Huffman::printEncoding (huffman.cc:100-107): the values that occur
+// (leaves with a count, the dup key included), by decreasing count, ties
+// by increasing value, as "value (count): length bits"
+void print_book(const wv_huff *book, int bin_count, bool with_dup) {
+  struct Leaf { int v; int64_t count; int len; uint32_t bits[2]; };
+  std::vector<Leaf> leaves;
+  for (int v = 0; v < bin_count + (with_dup ? 1 : 0); v++) {
+    Leaf l{v, 0, 0, {0, 0}};
+    if (wv_huff_code(book, v, &l.count, &l.len, l.bits) != WV_OK || l.count <= 0) continue;
+    leaves.push_back(l);
+  }
+  std::stable_sort(leaves.begin(), leaves.end(),
+                   [](const Leaf &a, const Leaf &b) { return a.count > b.count; });
+  for (const Leaf &l : leaves) {
+    std::string s((size_t)l.len, '0');
+    for (int i = 0; i < l.len; i++)
+      if ((l.bits[i >> 5] >> (i & 31)) & 1u) s[(size_t)i] = '1';
+    printf("%3d (%8d): %d %s\n", l.v, (int)l.count, l.len, s.c_str());
+  }
+}
+
 }  // namespace
 
 int main(int argc, char **argv) {
@@ -282,6 +314,7 @@ int main(int argc, char **argv) {
   opt.lloyd_max_iters = -1;
   opt.with_footer = 1;                     // the GPU driver writes the index
   bool quiet = false, do_err = false, do_decompress = false, do_opt = false;
+  bool print_encoding = false, do_2d = false, verbose = false;
   int a = 1;
   for (; a < argc && argv[a][0] == '-'; a++) {
     const char *s = argv[a];
@@ -326,6 +359,18 @@ int main(int argc, char **argv) {
     } else if (!strcmp(s, "-lloyd-iters")) {
       if (++a >= argc) usage();
       opt.lloyd_max_iters = atoi(argv[a]);
+    } else if (!strcmp(s, "-bq") || !strcmp(s, "-aq")) {
+      if (++a >= argc) usage();   // file names the GPU driver never writes
+    } else if (!strcmp(s, "-pb")) {
+      // doPrintQuantizationBins: parsed, unused by the GPU driver
+    } else if (!strcmp(s, "-enc")) {
+      print_encoding = true;
+    } else if (!strcmp(s, "-2d")) {
+      do_2d = true;
+    } else if (!strcmp(s, "-v")) {
+      verbose = true;
+    } else if (!strcmp(s, "-experiment")) {
+      quiet = true;   // runQuantizationExperiments: sets quiet only here
     } else if (!strcmp(s, "-q")) {
       quiet = true;
     } else if (!strcmp(s, "-d")) {
@@ -338,6 +383,10 @@ int main(int argc, char **argv) {
   if (argc - a != 2) usage();
   const char *in_path = argv[a], *out_path = argv[a + 1];
   if (do_decompress) return decompress_file(in_path, out_path, quiet);
+  if (do_2d) {
+    fprintf(stderr, "-2d option not implemented for test_compress_gpu yet.\n");
+    return 1;
+  }
 
   const double t0 = now_ms();
   InputCube cube;
@@ -379,6 +428,13 @@ int main(int argc, char **argv) {
                        (uint32_t *)words_d, words_cap, &nwords, &book, nullptr));
   HOK(hipFree(enc_ws));
   const double t2e = now_ms();
+  if (verbose) {
+    printf("transform, threshold %g (index %d), quantize + histogram: %.2f ms\n",
+           info.thresh.threshold, info.thresh.thresh_idx, t2 - t1);
+    printf("Huffman coding on the device: %lld payload words, %.2f ms\n", (long long)nwords,
+           t2e - t2);
+  }
+  if (print_encoding) print_book(book, opt.bin_count, opt.compress_zeros != 0);
 
   wv_cube_desc d;
   std::memset(&d, 0, sizeof d);

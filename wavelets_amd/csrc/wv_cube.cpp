@@ -283,20 +283,29 @@ extern "C" int wv_cube_parse(const uint8_t *file, int64_t len, wv_cube_info *inf
       I.payload_offset = pos;
       I.payload_bytes = (int64_t)nbytes;
       I.n_decode = (int)P.decode.size();
+      // leaves (pairs (0, v)) hold bin values 0..bin_count-1 or the dup key
+      // bin_count; anything else cannot come from the coder
+      for (size_t h = 0; h + 1 < P.decode.size(); h += 2)
+        if (P.decode[h] == 0 && (P.decode[h + 1] < 0 || P.decode[h + 1] > I.bin_count)) {
+          set_error("decode table leaf %d outside [0, %d]", P.decode[h + 1], I.bin_count);
+          return WV_ERR_ARG;
+        }
       I.n_boundaries = (int)P.bounds.size();
       I.n_values = (int)P.values.size();
       *info = I;
       if (decode_table) {
         if (decode_cap < I.n_decode) { set_error("decode table buffer too small"); return WV_ERR_NOMEM; }
-        std::memcpy(decode_table, P.decode.data(), P.decode.size() * 4);
+        if (!P.decode.empty()) std::memcpy(decode_table, P.decode.data(), P.decode.size() * 4);
       }
       if (boundaries || values) {
         if (codebook_cap < I.n_boundaries || codebook_cap < I.n_values) {
           set_error("codebook buffers too small");
           return WV_ERR_NOMEM;
         }
-        if (boundaries) std::memcpy(boundaries, P.bounds.data(), P.bounds.size() * 4);
-        if (values) std::memcpy(values, P.values.data(), P.values.size() * 4);
+        if (boundaries && !P.bounds.empty())
+          std::memcpy(boundaries, P.bounds.data(), P.bounds.size() * 4);
+        if (values && !P.values.empty())
+          std::memcpy(values, P.values.data(), P.values.size() * 4);
       }
       return WV_OK;
     }

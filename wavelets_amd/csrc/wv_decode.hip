@@ -328,12 +328,17 @@ DecLayout dec_layout(int64_t nwords, int n_table) {
 }
 
 // the decode table must be a tree: every internal pair's child ids point
-// further into the table and inside it (BFS numbering), so walks end
+// further into the table and inside it (BFS numbering), so walks end; leaf
+// values are bins or the dup key, at most WV_MAX_BINS (the lookup entries
+// keep 27 bits of them)
 bool table_ok(const int32_t *t, int n) {
   if (n < 2 || (n & 1)) return false;
   const int pairs = n / 2;
   for (int h = 0; h < pairs; h++) {
-    if (t[2 * h] == 0) continue;   // leaf
+    if (t[2 * h] == 0) {   // leaf
+      if (t[2 * h + 1] < 0 || t[2 * h + 1] > WV_MAX_BINS) return false;
+      continue;
+    }
     for (int k = 0; k < 2; k++) {
       const int c = t[2 * h + k];
       if (c <= h || c >= pairs) return false;

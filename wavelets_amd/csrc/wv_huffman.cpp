@@ -348,6 +348,22 @@ struct wv_huff {
       : coder(counts, b, dup, singles, dups), bin_count(b), payload_bits(0) {}
 };
 
+extern "C" int wv_huff_code(const wv_huff *book, int value, int64_t *count, int *bit_length,
+                            uint32_t bits[2]) {
+  if (!book || value < 0 || value >= book->coder.leaves()) {
+    wv::set_error("wv_huff_code: value %d outside the code book", value);
+    return WV_ERR_ARG;
+  }
+  const wvh::Code &c = book->coder.code(value);
+  if (count) *count = book->coder.freq(value);
+  if (bit_length) *bit_length = c.len;
+  if (bits) {
+    bits[0] = c.words.size() > 0 ? c.words[0] : 0u;
+    bits[1] = c.words.size() > 1 ? c.words[1] : 0u;
+  }
+  return WV_OK;
+}
+
 namespace wv {
 // the CubeletBuffer message of a compressed cubelet (copyToCubeletBuffer,
 // wavelet.cc:283-343) at box offset `offset`, without and with

@@ -90,6 +90,11 @@ hipError_t launch_pass(const DwtPass &P, int ntiles, int wavelet, bool inverse,
 hipError_t stats_clear(void *stats, hipStream_t s);
 hipError_t stats_compute(const float *coef, int64_t n, void *stats,
                          hipStream_t s);
+// LSD radix sort of *count_d (at most max_n) uint32 keys, result in keys
+// (wv_lloyd.hip); tmp of max_n keys, hist of radix_sort_hist_bytes(max_n)
+int64_t radix_sort_hist_bytes(int64_t max_n);
+hipError_t radix_sort_u32(uint32_t *keys, uint32_t *tmp, uint32_t *hist,
+                          const uint32_t *count_d, int64_t max_n, hipStream_t s);
 hipError_t threshold_select(const float *coef, int64_t n, float frac,
                             const void *stats, void *ws,
                             wv_thresh_result *res, hipStream_t s);

@@ -15,12 +15,16 @@
 //    members are not contiguous in point order (only possible with an
 //    unsorted codebook) the chain scans all points and adds its own.
 //  * the loop, its stop rule (reldist > 1e-6f) and the codebook / partition
-//    updates run on the device; the host only enqueues iterations and polls
-//    a done flag every few iterations.
+//    updates run on the device.  The distortion only decides when the loop
+//    stops, so iterations run in speculative windows of eight: the
+//    trajectory runs ahead, the eight distortion chains of a window run
+//    side by side, and the first iteration that meets the stop rule is the
+//    result.  The host polls once per window.
 #include "wv_device.h"
 #include "wv_internal.h"
-#include <vector>
+#include <algorithm>
 #include <cmath>
+#include <vector>
 
 namespace wv {
 
@@ -35,7 +39,8 @@ struct LloydState {
   float stop;
   float dist, reldist, pmin, pmax;
   uint32_t n_gt;                    // keys > threshold gathered
-  int32_t pad[3];
+  int32_t final_slot;               // window slot of the last iteration run
+  int32_t pad[2];
 };
 
 // ---------------------------------------------------------------- gather
@@ -79,12 +84,12 @@ ll_gather(const float *coef, int64_t n, uint32_t tkey, uint32_t *out,
 // ---------------------------------------------------------------- sort
 // per-block digit histograms, block-major [block][256]
 __global__ void __launch_bounds__(WV_BLOCK)
-ll_sort_hist(const uint32_t *keys, const LloydState *S, int shift,
+ll_sort_hist(const uint32_t *keys, const uint32_t *count, int shift,
              uint32_t *hist) {
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0;
   __syncthreads();
-  const uint32_t n = S->n_gt;
+  const uint32_t n = *count;
   const uint32_t base = blockIdx.x * LL_SORT_CHUNK;
   for (int k = 0; k < LL_SORT_ITEMS; k++) {
     const uint32_t i = base + k * WV_BLOCK + threadIdx.x;
@@ -127,13 +132,13 @@ ll_sort_scan(uint32_t *hist, int nblocks) {
 
 // stable scatter: keys of a block are ranked in index order
 __global__ void __launch_bounds__(WV_BLOCK)
-ll_sort_scatter(const uint32_t *keys, uint32_t *out, const LloydState *S,
+ll_sort_scatter(const uint32_t *keys, uint32_t *out, const uint32_t *count,
                 int shift, const uint32_t *offs) {
   __shared__ uint32_t base[256];
   __shared__ uint32_t wave_cnt[WV_BLOCK / 64][256];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   base[tid] = offs[blockIdx.x * 256 + tid];
-  const uint32_t n = S->n_gt;
+  const uint32_t n = *count;
   const uint32_t start = blockIdx.x * LL_SORT_CHUNK;
   __syncthreads();
   for (int k = 0; k < LL_SORT_ITEMS; k++) {
@@ -169,6 +174,31 @@ ll_sort_scatter(const uint32_t *keys, uint32_t *out, const LloydState *S,
   }
 }
 
+int64_t radix_sort_hist_bytes(int64_t max_n) {
+  return ((max_n + LL_SORT_CHUNK - 1) / LL_SORT_CHUNK + 1) * 256 * 4;
+}
+
+// LSD radix sort of the first *count_d (at most max_n) 32-bit keys: four
+// stable passes of 8 bits (per-block digit histograms, one scan, a scatter
+// that ranks a block's keys in index order).  After the even number of
+// passes the sorted keys are back in keys; tmp holds max_n keys, hist
+// radix_sort_hist_bytes(max_n).
+hipError_t radix_sort_u32(uint32_t *keys, uint32_t *tmp, uint32_t *hist,
+                          const uint32_t *count_d, int64_t max_n, hipStream_t s) {
+  const int nblocks = (int)((max_n + LL_SORT_CHUNK - 1) / LL_SORT_CHUNK);
+  uint32_t *src = keys, *dst = tmp;
+  for (int pass = 0; pass < 4 && nblocks > 0; pass++) {
+    const int shift = pass * 8;
+    hipLaunchKernelGGL(ll_sort_hist, dim3(nblocks), dim3(WV_BLOCK), 0, s, src, count_d, shift,
+                       hist);
+    hipLaunchKernelGGL(ll_sort_scan, dim3(1), dim3(256), 0, s, hist, nblocks);
+    hipLaunchKernelGGL(ll_sort_scatter, dim3(nblocks), dim3(WV_BLOCK), 0, s, src, dst, count_d,
+                       shift, hist);
+    std::swap(src, dst);
+  }
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- Lloyd
 // nearest codebook entry: first minimum of fabsf(p - c[j]) (lloyds.cpp:50-60)
 __device__ inline int nearest(const float *cb, int cs, float p) {
@@ -190,18 +220,23 @@ __device__ inline int between(const float *cb, int cs, float p) {
 
 // assignment of every point; records the first index of every group run
 // and flags a group that appears in more than one run
+// (ring: a 16-bit copy of the nearest groups, kept for the distortion
+// chain of this iteration, which runs later)
 template <bool NEAREST>
 __global__ void __launch_bounds__(WV_BLOCK)
 ll_assign(const float *pts, const float *cb_g, LloydState *S, int32_t *grp,
-          int32_t *first, int32_t *last) {
+          uint16_t *ring) {
   extern __shared__ float cb[];
   if (S->done) return;
   const int cs = S->cs;
   for (int j = threadIdx.x; j < cs; j += WV_BLOCK) cb[j] = cb_g[j];
   __syncthreads();
   const int psize = S->psize;
-  for (int i = blockIdx.x * WV_BLOCK + threadIdx.x; i < psize; i += gridDim.x * WV_BLOCK)
-    grp[i] = NEAREST ? nearest(cb, cs, pts[i]) : between(cb, cs, pts[i]);
+  for (int i = blockIdx.x * WV_BLOCK + threadIdx.x; i < psize; i += gridDim.x * WV_BLOCK) {
+    const int g = NEAREST ? nearest(cb, cs, pts[i]) : between(cb, cs, pts[i]);
+    grp[i] = g;
+    if (NEAREST && ring) ring[i] = (uint16_t)g;
+  }
 }
 
 // run boundaries: a group whose members are contiguous has exactly one run
@@ -338,15 +373,55 @@ __global__ void ll_update_partition(LloydState *S, const float *cb, float *part,
   }
 }
 
-// loop tail: reldist = |dist_old - dist_new|, then the loop test
-__global__ void ll_finish_iter(LloydState *S, const float *dsum) {
+// ---- speculative windows ---------------------------------------------------
+// The codebook trajectory depends only on the cell sums and counts
+// (lloyds.cpp:73-111); the signed distortion decides only when the loop
+// stops.  So a window runs W iterations of the trajectory back to back,
+// each leaving its codebook and 16-bit nearest groups in a slot, then the W
+// distortion chains run side by side (one wavefront each, the same
+// sequential sum over the points in order as lloyds.cpp:124-133), and one
+// thread applies the loop test to them in order: the first iteration that
+// stops the loop is the result, the later ones of the window are dropped.
+__global__ void ll_snapshot(const LloydState *S, const float *cb, float *dst) {
   if (S->done) return;
-  const float prev = S->dist;
-  const float dist = *dsum / (float)(unsigned)S->psize;
-  S->dist = dist;
-  S->reldist = fabsf(prev - dist);
-  S->iters++;
-  S->done = !(S->reldist > S->stop) || (S->max_iters >= 0 && S->iters >= S->max_iters);
+  for (int i = threadIdx.x; i < S->cs; i += blockDim.x) dst[i] = cb[i];
+}
+
+__global__ void __launch_bounds__(64)
+ll_window_dist(const float *__restrict__ pts, const uint16_t *__restrict__ ring,
+               int64_t ring_stride, const float *__restrict__ cbs, int cb_stride,
+               const LloydState *S, float *dsum) {
+  if (S->done) return;
+  const int w = blockIdx.x;
+  const uint16_t *g = ring + (int64_t)w * ring_stride;
+  const float *cb = cbs + (int64_t)w * cb_stride;
+  const int psize = uni(S->psize);
+  float d = 0.f;
+  int i = 0;
+  for (; i + 16 <= psize; i += 16) {
+    float t[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) t[k] = cb[g[i + k]] - pts[i + k];
+#pragma unroll
+    for (int k = 0; k < 16; k++) d += t[k];
+  }
+  for (; i < psize; i++) d += cb[g[i]] - pts[i];
+  if (threadIdx.x == 0) dsum[w] = d;
+}
+
+// the loop tail of each iteration of the window, in order: reldist =
+// |dist_old - dist_new|, then the loop test (lloyds.cpp:122-135, plus the
+// iteration cap)
+__global__ void ll_window_eval(LloydState *S, const float *dsum, int nw) {
+  for (int w = 0; w < nw && !S->done; w++) {
+    const float prev = S->dist;
+    const float dist = dsum[w] / (float)(unsigned)S->psize;
+    S->dist = dist;
+    S->reldist = fabsf(prev - dist);
+    S->iters++;
+    S->final_slot = w;
+    S->done = !(S->reldist > S->stop) || (S->max_iters >= 0 && S->iters >= S->max_iters);
+  }
 }
 
 __global__ void ll_fill_dups(uint32_t *tail, uint32_t ndup, uint32_t tkey) {
@@ -363,6 +438,7 @@ __global__ void ll_state_init(LloydState *S, int psize, int cs, int max_iters) {
   S->max_iters = max_iters;
   S->stop = (float)10e-7;    // DEFAULT_LLOYD_STOP_CRITERIA, lloyds.h:1
   S->n_gt = 0;
+  S->final_slot = -1;
 }
 
 }  // namespace wv
@@ -371,8 +447,10 @@ using namespace wv;
 
 namespace {
 inline int64_t a256(int64_t v) { return (v + 255) & ~(int64_t)255; }
+constexpr int LL_WINDOW = 8;   // iterations per speculative window
 struct LWs {
-  int64_t state, keys, keys2, hist, grp, cb, part, sum, count, first, last, nruns, dsum, total;
+  int64_t state, keys, keys2, hist, grp, cb, part, sum, count, first, last, nruns, dsum;
+  int64_t ring, ring_stride, cbring, wdsum, total;
 };
 LWs lloyd_layout(int64_t n, int bins) {
   LWs L{};
@@ -392,6 +470,10 @@ LWs lloyd_layout(int64_t n, int bins) {
   L.last = o; o += a256((int64_t)cs * 4 + 4);
   L.nruns = o; o += a256((int64_t)cs * 4 + 4);
   L.dsum = o; o += a256(16);
+  L.ring_stride = a256(n * 2) / 2;   // uint16 elements per slot
+  L.ring = o; o += L.ring_stride * 2 * LL_WINDOW;
+  L.cbring = o; o += a256((int64_t)(cs + 1) * 4 * LL_WINDOW);
+  L.wdsum = o; o += a256(4 * LL_WINDOW);
   L.total = o;
   return L;
 }
@@ -471,18 +553,10 @@ extern "C" int wv_lloyd(const float *coef_d, int64_t n, const wv_thresh_result *
   const int64_t ngt = hs.n_gt;
   const int64_t ndup = nz - ngt;
   if (ndup < 0 || ndup > n) { set_error("lloyd: inconsistent threshold result"); return WV_ERR_STATE; }
-  const int nblocks = (int)((ngt + LL_SORT_CHUNK - 1) / LL_SORT_CHUNK);
-  uint32_t *src = keys, *dst = keys2;
-  for (int pass = 0; pass < 4 && nblocks > 0; pass++) {
-    const int shift = pass * 8;
-    hipLaunchKernelGGL(ll_sort_hist, dim3(nblocks), dim3(WV_BLOCK), 0, s, src, S, shift, hist);
-    hipLaunchKernelGGL(ll_sort_scan, dim3(1), dim3(256), 0, s, hist, nblocks);
-    hipLaunchKernelGGL(ll_sort_scatter, dim3(nblocks), dim3(WV_BLOCK), 0, s, src, dst, S,
-                       shift, hist);
-    std::swap(src, dst);
-  }
-  // after 4 passes the sorted keys are in 'src' (== keys); lay out the tail
-  // [dups..., sorted...] in keys2
+  LCHK(radix_sort_u32(keys, keys2, hist, &S->n_gt, ngt, s), "sort");
+  const uint32_t *src = keys;
+  // the sorted keys are in keys; lay out the tail [dups..., sorted...] in
+  // keys2
   uint32_t *tail = keys2;
   if (ndup > 0)
     hipLaunchKernelGGL(ll_fill_dups, dim3(64), dim3(256), 0, s, tail, (uint32_t)ndup,
@@ -501,16 +575,20 @@ extern "C" int wv_lloyd(const float *coef_d, int64_t n, const wv_thresh_result *
 
   const unsigned ag = (unsigned)std::min<int64_t>((psize + WV_BLOCK - 1) / WV_BLOCK, 4096);
   const size_t cb_lds = (size_t)cs * 4;
-  auto assign_and_chains = [&](bool nearest_pass) {
+  uint16_t *ring = (uint16_t *)(w + L.ring);
+  float *cbring = (float *)(w + L.cbring), *wdsum = (float *)(w + L.wdsum);
+  // one assignment pass and its cell chains (+ the distortion chain with
+  // dist); ring: the slot for the nearest groups
+  auto assign_and_chains = [&](bool nearest_pass, bool dist, uint16_t *slot) {
     if (nearest_pass)
       hipLaunchKernelGGL(ll_assign<true>, dim3(ag), dim3(WV_BLOCK), cb_lds, s, pts, cb, S, grp,
-                         first, last);
+                         slot);
     else
       hipLaunchKernelGGL(ll_assign<false>, dim3(ag), dim3(WV_BLOCK), cb_lds, s, pts, cb, S, grp,
-                         first, last);
+                         (uint16_t *)nullptr);
     hipLaunchKernelGGL(ll_clear_runs, dim3(1), dim3(256), 0, s, S, nruns, first, last);
     hipLaunchKernelGGL(ll_runs, dim3(ag), dim3(WV_BLOCK), 0, s, S, grp, first, last, nruns);
-    if (nearest_pass)
+    if (dist)
       hipLaunchKernelGGL(ll_chains<true>, dim3(cs + 1), dim3(64), 0, s, pts, grp, cb, S, first,
                          last, nruns, sum, count, dsum);
     else
@@ -519,33 +597,40 @@ extern "C" int wv_lloyd(const float *coef_d, int64_t n, const wv_thresh_result *
   };
 
   // 3. initial assignment (lloyds.cpp:35-67)
-  assign_and_chains(true);
+  assign_and_chains(true, true, nullptr);
   hipLaunchKernelGGL(ll_init_finish, dim3(1), dim3(256), 0, s, S, cb, part, dsum, pts);
+  LCHK(hipMemcpyAsync(&hs, S, sizeof hs, hipMemcpyDeviceToHost, s), "D2H state");
+  LCHK(hipStreamSynchronize(s), "sync");
 
-  // 4. iterations, polled every few
-  const int poll = 4;
+  // 4. iterations in speculative windows, one host poll per window
   int enqueued = 0;
-  for (;;) {
-    for (int k = 0; k < poll; k++) {
+  while (!hs.done) {
+    int nw = LL_WINDOW;
+    if (max_iters >= 0 && max_iters - enqueued < nw) nw = max_iters - enqueued;
+    for (int k = 0; k < nw; k++) {
       hipLaunchKernelGGL(ll_update_codebook, dim3(1), dim3(256), 0, s, S, cb, part, sum, count);
-      assign_and_chains(false);
+      assign_and_chains(false, false, nullptr);
       hipLaunchKernelGGL(ll_update_partition, dim3(1), dim3(256), 0, s, S, cb, part, sum, count);
-      assign_and_chains(true);
-      hipLaunchKernelGGL(ll_finish_iter, dim3(1), dim3(1), 0, s, S, dsum);
-      enqueued++;
+      assign_and_chains(true, false, ring + (int64_t)k * L.ring_stride);
+      hipLaunchKernelGGL(ll_snapshot, dim3(1), dim3(256), 0, s, S, cb, cbring + (int64_t)k * (cs + 1));
     }
+    hipLaunchKernelGGL(ll_window_dist, dim3(nw), dim3(64), 0, s, pts, ring, L.ring_stride, cbring,
+                       cs + 1, S, wdsum);
+    hipLaunchKernelGGL(ll_window_eval, dim3(1), dim3(1), 0, s, S, wdsum, nw);
+    enqueued += nw;
     LCHK(hipGetLastError(), "launch");
     LCHK(hipMemcpyAsync(&hs, S, sizeof hs, hipMemcpyDeviceToHost, s), "D2H state");
     LCHK(hipStreamSynchronize(s), "sync");
-    if (hs.done) break;
-    if (max_iters >= 0 && enqueued >= max_iters) break;
-    if (enqueued >= LL_HARD_CAP) {
+    if (!hs.done && enqueued >= LL_HARD_CAP) {
       set_error("lloyd: no convergence after %d iterations", enqueued);
       return WV_ERR_STATE;
     }
   }
+  // the codebook of the iteration that stopped the loop (the initial one
+  // when the loop never ran)
+  const float *final_cb = hs.iters > 0 ? cbring + (int64_t)hs.final_slot * (cs + 1) : cb;
   std::vector<float> fin(cs);
-  LCHK(hipMemcpyAsync(fin.data(), cb, cs * 4, hipMemcpyDeviceToHost, s), "D2H codebook");
+  LCHK(hipMemcpyAsync(fin.data(), final_cb, cs * 4, hipMemcpyDeviceToHost, s), "D2H codebook");
   LCHK(hipStreamSynchronize(s), "sync");
   if (iters_out) *iters_out = hs.iters;
   return bins_from_codebook(bin_values, bin_boundaries, bin_count, fin.data(), cs,

@@ -15,7 +15,6 @@
 // as for a compression.  The sums are deterministic: fixed per-block
 // ranges reduced in a fixed order, in double like ErrorAccumulator
 // (test_compress_common.h:90-160).
-#include <hipcub/hipcub.hpp>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -210,14 +209,14 @@ extern "C" int wv_opt_create(const void *in_d, int dtype, const int orig[3], con
   o->in_d = in_d;
   const int64_t n = o->g.count;
   hipStream_t s = (hipStream_t)stream;
-  size_t sort_bytes = 0;
-  hipError_t e = hipcub::DeviceRadixSort::SortKeys(nullptr, sort_bytes, (const uint32_t *)nullptr,
-                                                   (uint32_t *)nullptr, (int)n, 0, 31, s);
-  if (e != hipSuccess || n > 0x7fffffffll) {
+  if (n > 0x7fffffffll) {
     delete o;
     set_error("wv_opt_create: sort of %lld keys", (long long)n);
     return WV_ERR_ARG;
   }
+  // the sort's key count (device) and its per-block digit histograms
+  const size_t sort_bytes = 256 + (size_t)radix_sort_hist_bytes(n);
+  hipError_t e = hipSuccess;
   o->sort_bytes = sort_bytes;
   o->table_cap = al256(((int64_t)WV_MAX_BINS * 3 + 16) * 4);
   const int64_t enc = wv_encode_ws_bytes(n, 256);
@@ -268,9 +267,13 @@ extern "C" int wv_opt_create(const void *in_d, int dtype, const int orig[3], con
   }
   if (rc == WV_OK) {
     hipLaunchKernelGGL(abs_keys_kernel, dim3(grid_of(n)), dim3(256), 0, s, o->coef, n,
-                       (uint32_t *)o->work);
-    e = hipcub::DeviceRadixSort::SortKeys(o->sort_tmp, sort_bytes, (const uint32_t *)o->work,
-                                          o->sorted, (int)n, 0, 31, s);
+                       o->sorted);
+    const uint32_t nkeys = (uint32_t)n;
+    uint32_t *count_d = (uint32_t *)o->sort_tmp;
+    e = hipMemcpyAsync(count_d, &nkeys, 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess)
+      e = radix_sort_u32(o->sorted, (uint32_t *)o->work, (uint32_t *)(o->sort_tmp + 256), count_d,
+                         n, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) {
       set_error("wv_opt_create: sort: %s", hipGetErrorString(e));

@@ -15,25 +15,26 @@
 //            rank's error is about 0.3% rms on cubelets of 4 M keys);
 //   collect  one streaming read of the coefficients by SEL_GRID blocks:
 //            counts the keys below the window, appends the keys inside it
-//            to a candidate list (buffered in LDS, one global atomic per
-//            block flush) and writes per-block partials (count below, min,
-//            max, max |x|, zeros) that a single block then reduces -- no
-//            same-address global atomics per element.  Zero keys are only
-//            counted, never listed: exact zeros are the one tie of any size
-//            in wavelet coefficients (flat regions of integer data), and a
-//            window that starts at 0 would otherwise list all of them;
-//   verify   the rank falls inside the window when below <= idx <
-//            below + candidates; otherwise an exact fallback (full 2048-bin
-//            histogram of bits 30..20, then the keys of the rank's bucket)
-//            rebuilds the candidate list -- its kernels return at once when
-//            the window held the rank;
-//   radix    histogram/scan rounds over the candidates, relative to the
-//            window: each round splits [base, base + 2^bits) into up to
-//            2048 buckets and keeps the rank's bucket, so the window's
-//            ~20 bits take two rounds and the fallback's bucket at most
-//            two (a third launch returns at once when bits is 0).
-// Everything stays on the device; the result struct is written by the last
-// single-block kernel.
+//            to a candidate list (buffered per wave in LDS, one global
+//            atomic per flush; no block barrier in the streaming loop, the
+//            next sub-chunk's loads in flight while one is examined) and
+//            writes per-block partials (count below, min, max, max |x|,
+//            zeros).  Zero keys are only counted, never listed: exact zeros
+//            are the one tie of any size in wavelet coefficients (flat
+//            regions of integer data), and a window that starts at 0 would
+//            otherwise list all of them.  The last block to finish reduces
+//            the partials and verifies the window: the rank falls inside it
+//            when below <= idx < below + candidates; otherwise the exact
+//            fallback takes every key, from the full 31-bit range;
+//   radix    two rounds over the candidates, relative to the window: each
+//            splits [base, base + 2^bits) into up to 2048 buckets (LDS
+//            histograms merged with one atomic per bin) and the last block
+//            to finish keeps the rank's bucket, so the window's ~20 bits
+//            take the two rounds; the second round's last block finishes
+//            any bits left (the fallback's) on its own and writes the
+//            result record.
+// Four launches (sample, collect, two rounds); everything stays on the
+// device.
 #include "wv_device.h"
 #include "wv_internal.h"
 
@@ -43,9 +44,9 @@ namespace wv {
 #define SEL_MARGIN 0.0125f
 #define SEL_ITEMS 16
 #define SEL_CHUNK (WV_BLOCK * SEL_ITEMS)
-#define SEL_FB_GRID 2048
-#define SEL_GRID 1024       // collect blocks (each streams n / SEL_GRID keys)
-#define SEL_CAND_CAP 8192   // LDS candidate buffer per collect block
+#define SEL_GRID 2048       // collect blocks (each streams n / SEL_GRID keys)
+#define SEL_WAVE_CAP 512    // LDS candidate buffer per collect wave
+#define SEL_RGRID 256       // blocks of a radix round
 
 struct SelState {
   int32_t idx;          // target rank
@@ -59,7 +60,7 @@ struct SelState {
   uint32_t zwin;       // zero keys inside the window (counted, not listed)
   uint32_t min_ord, max_ord, max_key;
   uint32_t overflow;    // the candidate list outgrew its capacity
-  uint32_t pad;
+  uint32_t ctr;         // blocks done (the last one of a launch finishes it)
   uint32_t hist1[WV_H1_BINS];
   uint32_t hist2[WV_H1_BINS];   // radix-round histogram
   uint32_t part[SEL_GRID][5];   // per collect block: below, min, max, max key, zeros
@@ -182,6 +183,39 @@ __device__ void find_bucket(const uint32_t *cnt, uint32_t rank, uint32_t *out_b,
   __syncthreads();
 }
 
+// The same search for any block size (a multiple of 64, at most 1024).
+template <int NB>
+__device__ void find_bucket_blk(const uint32_t *cnt, uint32_t rank, uint32_t *out_b,
+                                uint32_t *out_rank) {
+  __shared__ uint32_t part[1024];
+  const int t = threadIdx.x, nt = blockDim.x;
+  const int per = (NB + nt - 1) / nt;
+  uint32_t sum = 0;
+  for (int k = 0; k < per; k++) {
+    const int b = t * per + k;
+    sum += b < NB ? cnt[b] : 0u;
+  }
+  part[t] = sum;
+  __syncthreads();
+  for (int off = 1; off < nt; off <<= 1) {   // inclusive scan
+    const uint32_t v = t >= off ? part[t - off] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  uint32_t before = t ? part[t - 1] : 0;
+  for (int k = 0; k < per; k++) {
+    const int b = t * per + k;
+    const uint32_t c = b < NB ? cnt[b] : 0u;
+    if (rank >= before && rank < before + c) {
+      *out_b = (uint32_t)b;
+      *out_rank = rank - before;
+    }
+    before += c;
+  }
+  __syncthreads();
+}
+
 // ---- 1. sample -------------------------------------------------------------
 // 32-bit integer mixer (xor-shift / multiply rounds with the "lowbias32"
 // constants published by C. Wellons, hash-prospector)
@@ -279,6 +313,7 @@ __device__ void sel_sample_dev(const float *__restrict__ coef, int64_t n, float 
     S->max_ord = 0u;
     S->max_key = 0u;
     S->overflow = 0;
+    S->ctr = 0;
   }
 }
 
@@ -288,40 +323,69 @@ sel_sample(const float *__restrict__ coef, int64_t n, float frac, SelState *S) {
 }
 
 // ---- 2. collect ------------------------------------------------------------
-// SEL_GRID blocks, each streaming a contiguous range in sub-chunks of
-// SEL_CHUNK keys (16-byte loads, 16 keys per thread).  Per-block results go
-// to S->part (no same-address global atomics); the candidate list is
-// reserved with one atomic per LDS-buffer flush.
+// nblk blocks, each streaming a contiguous range in sub-chunks of SEL_CHUNK
+// keys (16-byte loads, 16 keys per thread; the next sub-chunk's loads are
+// issued before the current one is examined).  Each wave appends its
+// in-window keys to its own LDS buffer and flushes it to the candidate
+// list (one global atomic per flush) when it is nearly full, so the
+// streaming loop has no block barrier.  Per-block results go to S->part
+// (no same-address global atomics per element).
+__device__ inline void sel_load_chunk(const float *__restrict__ coef, int64_t n, int64_t c,
+                                      int tid, float *v) {
+  const int64_t start = c * SEL_CHUNK;
+  if (start + SEL_CHUNK <= n) {
+    const float4 *c4 = (const float4 *)(coef + start);
+#pragma unroll
+    for (int u = 0; u < SEL_ITEMS / 4; u++) {
+      const float4 q = c4[tid + u * WV_BLOCK];
+      v[4 * u] = q.x; v[4 * u + 1] = q.y; v[4 * u + 2] = q.z; v[4 * u + 3] = q.w;
+    }
+  } else {
+#pragma unroll
+    for (int u = 0; u < SEL_ITEMS; u++) {
+      const int64_t i = start + (int64_t)(u >> 2) * 4 * WV_BLOCK + 4 * tid + (u & 3);
+      v[u] = i < n ? coef[i] : __builtin_bit_cast(float, 0xffffffffu);
+    }
+  }
+}
+
+// a wave's buffered candidates to the global list (wave-uniform call)
+__device__ inline void sel_flush(uint32_t *buf, uint32_t &wcnt, SelState *S, uint32_t *cand,
+                                 uint32_t cap, int lane) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  uint32_t base = 0;
+  if (lane == 0) {
+    base = atomicAdd(&S->ncand, wcnt);
+    if (base + wcnt > cap) S->overflow = 1;
+  }
+  base = __shfl(base, 0);
+  if (base + wcnt <= cap)
+    for (uint32_t i = lane; i < wcnt; i += 64) cand[base + i] = buf[i];
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  wcnt = 0;
+}
+
 __device__ void sel_collect_dev(const float *__restrict__ coef, int64_t n, SelState *S,
                                 uint32_t *cand, uint32_t cap, int bidx, int nblk) {
-  __shared__ uint32_t buf[SEL_CAND_CAP];
-  __shared__ uint32_t cnt, base;
+  __shared__ uint32_t wbuf[WV_BLOCK / 64][SEL_WAVE_CAP];
   __shared__ uint32_t red[5][WV_BLOCK / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  if (tid == 0) cnt = 0;
-  __syncthreads();
+  uint32_t *buf = wbuf[wave];
+  uint32_t wcnt = 0;   // wave-uniform
   const uint32_t klo = S->key_lo, khi = S->key_hi;
   const int64_t nchunk = (n + SEL_CHUNK - 1) / SEL_CHUNK;
   const int64_t c0 = nchunk * bidx / nblk;
   const int64_t c1 = nchunk * (bidx + 1) / nblk;
   uint32_t nb = 0, mn = 0xffffffffu, mx = 0, mk = 0, nz = 0;
+  float v[SEL_ITEMS];
+  if (c0 < c1) sel_load_chunk(coef, n, c0, tid, v);
   for (int64_t c = c0; c < c1; c++) {
-    const int64_t start = c * SEL_CHUNK;
-    float v[SEL_ITEMS];
-    const bool full = start + SEL_CHUNK <= n;
-    if (full) {
-      const float4 *c4 = (const float4 *)(coef + start);
-#pragma unroll
-      for (int u = 0; u < SEL_ITEMS / 4; u++) {
-        const float4 q = c4[tid + u * WV_BLOCK];
-        v[4 * u] = q.x; v[4 * u + 1] = q.y; v[4 * u + 2] = q.z; v[4 * u + 3] = q.w;
-      }
-    } else {
-      for (int u = 0; u < SEL_ITEMS; u++) {
-        const int64_t i = start + (int64_t)(u >> 2) * 4 * WV_BLOCK + 4 * tid + (u & 3);
-        v[u] = i < n ? coef[i] : __builtin_bit_cast(float, 0xffffffffu);
-      }
-    }
+    float nv[SEL_ITEMS];
+    if (c + 1 < c1) sel_load_chunk(coef, n, c + 1, tid, nv);
 #pragma unroll
     for (int u = 0; u < SEL_ITEMS; u++) {
       const uint32_t bits = __builtin_bit_cast(uint32_t, v[u]);
@@ -338,28 +402,17 @@ __device__ void sel_collect_dev(const float *__restrict__ coef, int64_t n, SelSt
       const bool hit = ok && key >= klo && key <= khi && key != 0;
       const uint64_t m = __ballot(hit);
       if (m) {
-        uint32_t off = 0;
-        if (lane == 0) off = atomicAdd(&cnt, (uint32_t)__popcll(m));
-        off = __shfl(off, 0);
-        if (hit) buf[off + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = key;
+        if (hit) buf[wcnt + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = key;
+        wcnt += (uint32_t)__popcll(m);
+        if (wcnt > SEL_WAVE_CAP - 64) sel_flush(buf, wcnt, S, cand, cap, lane);
       }
     }
-    __syncthreads();
-    // flush when the next sub-chunk might not fit
-    if (cnt > SEL_CAND_CAP - SEL_CHUNK || c == c1 - 1) {
-      const uint32_t k = cnt;
-      if (tid == 0 && k) {
-        base = atomicAdd(&S->ncand, k);
-        if (base + k > cap) S->overflow = 1;
-      }
-      __syncthreads();
-      if (base + k <= cap)
-        for (uint32_t i = tid; i < k; i += WV_BLOCK) cand[base + i] = buf[i];
-      __syncthreads();
-      if (tid == 0) cnt = 0;
-      __syncthreads();
+    if (c + 1 < c1) {
+#pragma unroll
+      for (int u = 0; u < SEL_ITEMS; u++) v[u] = nv[u];
     }
   }
+  if (wcnt) sel_flush(buf, wcnt, S, cand, cap, lane);
   for (int off = 32; off > 0; off >>= 1) {
     nb += (uint32_t)__shfl_xor((int)nb, off);
     mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
@@ -387,17 +440,30 @@ __device__ void sel_collect_dev(const float *__restrict__ coef, int64_t n, SelSt
   }
 }
 
-__global__ void __launch_bounds__(WV_BLOCK)
-sel_collect(const float *__restrict__ coef, int64_t n, SelState *S, uint32_t *cand) {
-  sel_collect_dev(coef, n, S, cand, 0xffffffffu, blockIdx.x, gridDim.x);
+// Is this block the last of its launch to finish?  Every block's global
+// writes are fenced before it counts itself; the last one resets the
+// counter and fences again before it reads what the others wrote.
+__device__ inline bool last_block_done(uint32_t *ctr) {
+  __shared__ uint32_t last;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __threadfence();
+    last = atomicAdd(ctr, 1u) == gridDim.x - 1;
+    if (last) {
+      *ctr = 0;
+      __threadfence();
+    }
+  }
+  __syncthreads();
+  return last != 0;
 }
 
 // ---- 3. verify -------------------------------------------------------------
 __device__ void sel_verify_dev(SelState *S, int nparts) {
   __shared__ uint32_t r[5][16];
-  const int t = threadIdx.x;
+  const int t = threadIdx.x, nw = blockDim.x >> 6;
   uint32_t nb = 0, mn = 0xffffffffu, mx = 0, mk = 0, nz = 0;
-  for (int i = t; i < nparts; i += 1024) {
+  for (int i = t; i < nparts; i += blockDim.x) {
     nb += S->part[i][0];
     mn = min(mn, S->part[i][1]);
     mx = max(mx, S->part[i][2]);
@@ -416,7 +482,7 @@ __device__ void sel_verify_dev(SelState *S, int nparts) {
   }
   __syncthreads();
   if (t != 0) return;
-  for (int w = 1; w < 16; w++) {
+  for (int w = 1; w < nw; w++) {
     nb += r[0][w];
     mn = min(mn, r[1][w]);
     mx = max(mx, r[2][w]);
@@ -434,65 +500,30 @@ __device__ void sel_verify_dev(SelState *S, int nparts) {
   if (!S->overflow && S->below <= idx && idx < S->below + S->ncand + S->zwin) {
     S->crank = idx - S->below;
   } else {
+    // exact fallback: the rank among all keys, from the full key range
     S->fallback = 1;
     S->ncand = 0;
-    S->zwin = 0;   // the fallback lists every key of the rank's bucket
+    S->zwin = 0;
+    S->crank = idx;
+    S->base = 0;
+    S->bits = 31;
   }
 }
 
-__global__ void __launch_bounds__(1024) sel_verify(SelState *S, int nparts) {
-  sel_verify_dev(S, nparts);
-}
-
-// ---- fallback: exact bucket of the rank from a full histogram --------------
-__global__ void __launch_bounds__(1024) sel_fb_scan(SelState *S) {
-  if (!S->fallback) return;
-  __shared__ uint32_t b, r;
-  if (threadIdx.x == 0) { b = 0; r = 0; }
-  __syncthreads();
-  find_bucket<WV_H1_BINS>(S->hist1, (uint32_t)S->idx, &b, &r);
-  if (threadIdx.x == 0) {
-    S->key_lo = b << WV_H1_SHIFT;
-    S->key_hi = (b << WV_H1_SHIFT) | ((1u << WV_H1_SHIFT) - 1);
-    S->crank = r;
-    S->base = b << WV_H1_SHIFT;
-    S->bits = WV_H1_SHIFT;
-  }
-  for (int i = threadIdx.x; i < WV_H1_BINS; i += 1024) S->hist1[i] = 0;
-}
-
+// the plain path's collect: the last block to finish reduces the partials
+// and checks the window (the verify)
 __global__ void __launch_bounds__(WV_BLOCK)
-sel_fb_collect(const float *__restrict__ coef, int64_t n, SelState *S,
-               uint32_t *cand) {
-  if (!S->fallback) return;
-  const uint32_t klo = S->key_lo, khi = S->key_hi;
-  const int lane = threadIdx.x & 63;
-  const int64_t stride = (int64_t)gridDim.x * WV_BLOCK;
-  const int64_t nround = (n + stride - 1) / stride;
-  for (int64_t r = 0; r < nround; r++) {
-    const int64_t i = r * stride + (int64_t)blockIdx.x * WV_BLOCK + threadIdx.x;
-    uint32_t key = 0;
-    bool hit = false;
-    if (i < n) {
-      key = __builtin_bit_cast(uint32_t, coef[i]) & 0x7fffffffu;
-      hit = key >= klo && key <= khi;
-    }
-    const uint64_t m = __ballot(hit);
-    if (m) {
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(&S->ncand, (uint32_t)__popcll(m));
-      base = __shfl(base, 0);
-      if (hit) cand[base + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = key;
-    }
-  }
+sel_collect(const float *__restrict__ coef, int64_t n, SelState *S, uint32_t *cand) {
+  sel_collect_dev(coef, n, S, cand, 0xffffffffu, blockIdx.x, gridDim.x);
+  if (last_block_done(&S->ctr)) sel_verify_dev(S, gridDim.x);
 }
 
-// ---- 4. radix select over the candidates -----------------------------------
-// The key of rank crank among the candidates lies in [base, base + 2^bits).
+// ---- 4. radix select -----------------------------------------------------------
+// The key of rank crank lies in [base, base + 2^bits) among the candidates
+// (or, after an exact fallback, among all keys, from base 0 and 31 bits).
 // A round histograms the next d = min(bits, 11) bits of (key - base) over
-// the candidates in that range, then narrows base / bits / crank to the
-// rank's bucket.  Starting from the sampled window (bits ~ 20) two rounds
-// finish; the exact fallback starts from one top-digit bucket (bits = 20).
+// the keys in that range and narrows base / bits / crank to the rank's
+// bucket.  The sampled window spans ~20 bits, so two rounds finish.
 __device__ inline bool in_range(uint32_t key, uint32_t base, uint32_t bits) {
   return key >= base && (uint64_t)(key - base) < (1ull << bits);
 }
@@ -500,58 +531,6 @@ __device__ inline bool in_range(uint32_t key, uint32_t base, uint32_t bits) {
 __device__ inline uint32_t round_digit(uint32_t key, uint32_t base, uint32_t bits) {
   const uint32_t d = bits < 11u ? bits : 11u;
   return ((key - base) >> (bits - d)) & ((1u << d) - 1u);
-}
-
-// few blocks, so the global histogram takes few atomics; the digits of a
-// narrowed range spread over the bins, so plain LDS atomics suffice
-__global__ void __launch_bounds__(WV_BLOCK)
-sel_rhist(const uint32_t *cand, SelState *S) {
-  __shared__ uint32_t h[WV_H1_BINS];
-  if (S->special || S->bits == 0) return;
-  const uint32_t nc = S->ncand;
-  const uint32_t per = (nc + gridDim.x - 1) / gridDim.x;
-  const uint32_t a = blockIdx.x * per, e = min(nc, a + per);
-  if (a >= e && !(blockIdx.x == 0 && S->zwin)) return;
-  for (int i = threadIdx.x; i < WV_H1_BINS; i += WV_BLOCK) h[i] = 0;
-  __syncthreads();
-  const uint32_t base = S->base, bits = S->bits;
-  for (uint32_t i0 = a; i0 < e; i0 += WV_BLOCK * 4) {
-    uint32_t kk[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const uint32_t i = i0 + u * WV_BLOCK + threadIdx.x;
-      kk[u] = i < e ? cand[i] : 0xffffffffu;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const uint32_t key = kk[u];
-      if (key != 0xffffffffu && in_range(key, base, bits))
-        atomicAdd(&h[round_digit(key, base, bits)], 1u);
-    }
-  }
-  // the listed keys exclude the zeros of the window: block 0 adds them
-  if (threadIdx.x == 0 && blockIdx.x == 0 && S->zwin && in_range(0u, base, bits))
-    atomicAdd(&h[round_digit(0u, base, bits)], S->zwin);
-  __syncthreads();
-  for (int i = threadIdx.x; i < WV_H1_BINS; i += WV_BLOCK)
-    if (h[i]) atomicAdd(&S->hist2[i], h[i]);
-}
-
-__global__ void __launch_bounds__(1024) sel_rscan(SelState *S) {
-  if (S->special || S->bits == 0) return;
-  __shared__ uint32_t b, r;
-  if (threadIdx.x == 0) { b = 0; r = 0; }
-  __syncthreads();
-  find_bucket<WV_H1_BINS>(S->hist2, S->crank, &b, &r);
-  __syncthreads();
-  for (int i = threadIdx.x; i < WV_H1_BINS; i += 1024) S->hist2[i] = 0;
-  if (threadIdx.x == 0) {
-    const uint32_t bits = S->bits;
-    const uint32_t d = bits < 11u ? bits : 11u;
-    S->base += b << (bits - d);
-    S->bits = bits - d;
-    S->crank = r;
-  }
 }
 
 __device__ void sel_result_dev(int64_t n, const SelState *S, wv_thresh_result *res) {
@@ -584,7 +563,90 @@ __device__ void sel_result_dev(int64_t n, const SelState *S, wv_thresh_result *r
   *res = o;
 }
 
-__global__ void sel_result(int64_t n, const SelState *S, wv_thresh_result *res) {
+// One round over SEL_RGRID blocks: LDS histograms of the digit, merged
+// into S->hist2 with one atomic per non-empty bin; the last block picks the
+// rank's bucket.  In the last round that block then finishes whatever bits
+// are left on its own (none for the sampled window) and writes the result.
+// (With the exact fallback the keys are all n coefficients and the window's
+// zeros are not counted apart.)
+__global__ void __launch_bounds__(WV_BLOCK)
+sel_round(const uint32_t *cand, const float *coef, int64_t n, SelState *S,
+          wv_thresh_result *res, int last_round) {
+  __shared__ uint32_t h[WV_H1_BINS];
+  __shared__ uint32_t rb, rr;
+  const bool fb = S->fallback != 0;
+  const uint32_t *src = fb ? (const uint32_t *)coef : cand;
+  const int64_t cnt = fb ? n : (int64_t)S->ncand;
+  const uint32_t zwin = fb ? 0u : S->zwin;
+  if (!S->special && S->bits > 0) {
+    for (int i = threadIdx.x; i < WV_H1_BINS; i += WV_BLOCK) h[i] = 0;
+    __syncthreads();
+    const uint32_t base = S->base, bits = S->bits;
+    const int64_t per = (cnt + gridDim.x - 1) / gridDim.x;
+    const int64_t a = (int64_t)blockIdx.x * per, e = min(cnt, a + per);
+    for (int64_t i0 = a; i0 < e; i0 += WV_BLOCK * 4) {
+      uint32_t kk[4];
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t i = i0 + u * WV_BLOCK + threadIdx.x;
+        kk[u] = i < e ? src[i] & 0x7fffffffu : 0xffffffffu;
+      }
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const uint32_t key = kk[u];
+        if (key != 0xffffffffu && in_range(key, base, bits))
+          atomicAdd(&h[round_digit(key, base, bits)], 1u);
+      }
+    }
+    // the listed keys exclude the window's zeros: block 0 adds them
+    if (threadIdx.x == 0 && blockIdx.x == 0 && zwin && in_range(0u, base, bits))
+      atomicAdd(&h[round_digit(0u, base, bits)], zwin);
+    __syncthreads();
+    for (int i = threadIdx.x; i < WV_H1_BINS; i += WV_BLOCK)
+      if (h[i]) atomicAdd(&S->hist2[i], h[i]);
+  }
+  if (!last_block_done(&S->ctr)) return;
+  if (!S->special && S->bits > 0) {
+    if (threadIdx.x == 0) { rb = 0; rr = 0; }
+    __syncthreads();
+    find_bucket_blk<WV_H1_BINS>(S->hist2, S->crank, &rb, &rr);
+    for (int i = threadIdx.x; i < WV_H1_BINS; i += WV_BLOCK) S->hist2[i] = 0;
+    if (threadIdx.x == 0) {
+      const uint32_t bits = S->bits;
+      const uint32_t d = bits < 11u ? bits : 11u;
+      S->base += rb << (bits - d);
+      S->bits = bits - d;
+      S->crank = rr;
+    }
+    __syncthreads();
+  }
+  if (!last_round) return;
+  if (!S->special) {
+    uint32_t rank = S->crank, base = S->base, bits = S->bits;
+    while (bits > 0) {
+      for (int i = threadIdx.x; i < WV_H1_BINS; i += WV_BLOCK) h[i] = 0;
+      if (threadIdx.x == 0) { rb = 0; rr = 0; }
+      __syncthreads();
+      if (threadIdx.x == 0 && zwin && in_range(0u, base, bits))
+        atomicAdd(&h[round_digit(0u, base, bits)], zwin);
+      for (int64_t i = threadIdx.x; i < cnt; i += WV_BLOCK) {
+        const uint32_t key = src[i] & 0x7fffffffu;
+        if (in_range(key, base, bits)) atomicAdd(&h[round_digit(key, base, bits)], 1u);
+      }
+      __syncthreads();
+      find_bucket_blk<WV_H1_BINS>(h, rank, &rb, &rr);
+      const uint32_t d = bits < 11u ? bits : 11u;
+      base += rb << (bits - d);
+      bits -= d;
+      rank = rr;
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      S->base = base;
+      S->bits = 0;
+    }
+    __syncthreads();
+  }
   if (threadIdx.x == 0) sel_result_dev(n, S, res);
 }
 
@@ -598,23 +660,8 @@ hipError_t threshold_select(const float *coef, int64_t n, float frac,
   const int64_t nchunk = (n + SEL_CHUNK - 1) / SEL_CHUNK;
   const int grid = (int)(nchunk < SEL_GRID ? nchunk : SEL_GRID);
   hipLaunchKernelGGL(sel_collect, dim3(grid), dim3(WV_BLOCK), 0, s, coef, n, S, cand);
-  hipLaunchKernelGGL(sel_verify, dim3(1), dim3(1024), 0, s, S, grid);
-  // exact fallback (returns at once unless the window missed the rank)
-  int64_t fb = (n + WV_BLOCK * 8 - 1) / (WV_BLOCK * 8);
-  if (fb > SEL_FB_GRID) fb = SEL_FB_GRID;
-  hipLaunchKernelGGL(hist1_kernel, dim3((unsigned)fb), dim3(WV_BLOCK), 0, s, coef, n,
-                     S->hist1, (uint32_t *)nullptr, (uint32_t *)nullptr,
-                     (uint32_t *)nullptr, (const int32_t *)&S->fallback);
-  hipLaunchKernelGGL(sel_fb_scan, dim3(1), dim3(1024), 0, s, S);
-  hipLaunchKernelGGL(sel_fb_collect, dim3((unsigned)fb), dim3(WV_BLOCK), 0, s, coef, n, S,
-                     cand);
-  // radix rounds over the candidates (at most three: 11 + 11 + 9 bits;
-  // the sampled window usually needs two, and a spent round returns at once)
-  for (int round = 0; round < 3; round++) {
-    hipLaunchKernelGGL(sel_rhist, dim3(256), dim3(WV_BLOCK), 0, s, cand, S);
-    hipLaunchKernelGGL(sel_rscan, dim3(1), dim3(1024), 0, s, S);
-  }
-  hipLaunchKernelGGL(sel_result, dim3(1), dim3(64), 0, s, n, S, res);
+  hipLaunchKernelGGL(sel_round, dim3(SEL_RGRID), dim3(WV_BLOCK), 0, s, cand, coef, n, S, res, 0);
+  hipLaunchKernelGGL(sel_round, dim3(SEL_RGRID), dim3(WV_BLOCK), 0, s, cand, coef, n, S, res, 1);
   return hipGetLastError();
 }
 

@@ -19,6 +19,7 @@
 #include <map>
 #include <atomic>
 #include <chrono>
+#include <cstdint>
 #include <cstring>
 #include <string>
 #include <thread>
@@ -244,6 +245,20 @@ extern "C" int64_t wv_batch_write_stream(wv_batch *b, const int *offsets, int co
   auto free_books = [&] {
     for (auto *bk : books) wv_huff_free(bk);
   };
+  // one coding workspace, sized for the largest group (encode_batch
+  // synchronises its stream before it returns, so the groups can share it)
+  int64_t wsb = 0;
+  for (auto &kv : groups) {
+    std::vector<int64_t> cnt;
+    for (int i : kv.second) cnt.push_back(geoms[i].count);
+    wsb = std::max(wsb, encode_batch_ws_bytes(cnt.data(), (int)cnt.size(), kv.first.first,
+                                              kv.first.second));
+  }
+  DevBuf ws;
+  if (wsb > 0 && hipMalloc(&ws.p, (size_t)wsb) != hipSuccess) {
+    set_error("wv_batch_write_stream: coding workspace of %lld bytes", (long long)wsb);
+    return WV_ERR_NOMEM;
+  }
   int gi = 0;
   for (auto &kv : groups) {
     const std::vector<int> &ix = kv.second;
@@ -255,13 +270,6 @@ extern "C" int64_t wv_batch_write_stream(wv_batch *b, const int *offsets, int co
       bp[k] = items[ix[k]].bins_d;
       cp[k] = items[ix[k]].counts_d;
       cnt[k] = geoms[ix[k]].count;
-    }
-    const int64_t wsb = encode_batch_ws_bytes(cnt.data(), nj, bins, dup);
-    DevBuf ws;
-    if (hipMalloc(&ws.p, (size_t)wsb) != hipSuccess) {
-      free_books();
-      set_error("wv_batch_write_stream: coding workspace");
-      return WV_ERR_NOMEM;
     }
     std::vector<uint32_t> none;
     std::vector<int64_t> gwoff, gnw;
@@ -282,6 +290,13 @@ extern "C" int64_t wv_batch_write_stream(wv_batch *b, const int *offsets, int co
     }
     gi++;
   }
+  for (int i = 0; i < n; i++)
+    if (inw[i] > 0x3fffffff) {   // the record's byte_count is a uint32
+      free_books();
+      set_error("wv_batch_write_stream: item %d: payload of %lld words exceeds a record's 4 GiB",
+                i, (long long)inw[i]);
+      return WV_ERR_ARG;
+    }
   std::vector<wv_cube_desc> descs((size_t)n);
   std::vector<std::string> params((size_t)n);
   {
@@ -325,49 +340,51 @@ extern "C" int64_t wv_batch_write_stream(wv_batch *b, const int *offsets, int co
     set_error("wv_batch_write_stream: file buffer %lld < %lld", (long long)cap, (long long)need);
     return WV_ERR_NOMEM;
   }
-  // large records: each payload goes from the device straight to its
-  // place in the file image; small ones (where a copy's fixed cost
-  // dominates): one copy of each group's payloads, then host copies
+  // per record: a payload of 128 KB or more goes from the device straight
+  // to its place in the file image; smaller ones (where a copy's fixed cost
+  // dominates) come from one copy of their group's span of small payloads,
+  // then host copies
   hipStream_t s = (hipStream_t)stream;
   std::memset(file_buf, 0, 64);
   std::memcpy(file_buf, "SCU cubelets 1.0\n", 17);
-  int64_t payload_bytes = 0;
-  for (int i = 0; i < n; i++) payload_bytes += inw[i] * 4;
-  const bool direct = payload_bytes >= (int64_t)n * (128 << 10);   // >= 128 KB per record
-  std::vector<std::vector<uint32_t>> staged(direct ? 0 : gdev.size());
-  if (!direct) {
-    std::vector<int64_t> gwords(gdev.size(), 0);
-    for (int i = 0; i < n; i++) gwords[gof[i]] = std::max(gwords[gof[i]], iwoff[i] + inw[i]);
-    for (size_t g = 0; g < gdev.size(); g++) {
-      staged[g].resize((size_t)gwords[g]);
-      if (gwords[g] > 0 && hipMemcpyAsync(staged[g].data(), gdev[g].p, (size_t)gwords[g] * 4,
-                                          hipMemcpyDeviceToHost, s) != hipSuccess) {
-        set_error("wv_batch_write_stream: payload D2H");
-        return WV_ERR_HIP;
-      }
+  auto is_direct = [&](int i) { return inw[i] * 4 >= (int64_t)(128 << 10); };
+  const size_t ng = gdev.size();
+  std::vector<int64_t> glo(ng, INT64_MAX), ghi(ng, 0);
+  for (int i = 0; i < n; i++)
+    if (inw[i] > 0 && !is_direct(i)) {
+      glo[gof[i]] = std::min(glo[gof[i]], iwoff[i]);
+      ghi[gof[i]] = std::max(ghi[gof[i]], iwoff[i] + inw[i]);
     }
-    if (hipStreamSynchronize(s) != hipSuccess) {
-      set_error("wv_batch_write_stream: sync");
+  std::vector<std::vector<uint32_t>> staged(ng);
+  for (size_t g = 0; g < ng; g++) {
+    if (ghi[g] <= 0) continue;
+    staged[g].resize((size_t)(ghi[g] - glo[g]));
+    if (hipMemcpyAsync(staged[g].data(), (const uint32_t *)gdev[g].p + glo[g],
+                       (size_t)(ghi[g] - glo[g]) * 4, hipMemcpyDeviceToHost, s) != hipSuccess) {
+      set_error("wv_batch_write_stream: payload D2H");
       return WV_ERR_HIP;
     }
+  }
+  for (int i = 0; i < n; i++) {
+    if (inw[i] <= 0 || !is_direct(i)) continue;
+    if (hipMemcpyAsync(file_buf + data_at[i], (const uint32_t *)gdev[gof[i]].p + iwoff[i],
+                       (size_t)inw[i] * 4, hipMemcpyDeviceToHost, s) != hipSuccess) {
+      set_error("wv_batch_write_stream: payload D2H");
+      return WV_ERR_HIP;
+    }
+  }
+  if (hipStreamSynchronize(s) != hipSuccess) {
+    set_error("wv_batch_write_stream: sync");
+    return WV_ERR_HIP;
   }
   for (int i = 0; i < n; i++) {
     const uint32_t mlen = (uint32_t)metas[i].size();
     uint8_t *r = file_buf + data_at[i] - 4 - mlen;
     std::memcpy(r, &mlen, 4);
     std::memcpy(r + 4, metas[i].data(), mlen);
-    if (inw[i] <= 0) continue;
-    if (!direct) {
-      std::memcpy(file_buf + data_at[i], staged[gof[i]].data() + iwoff[i], (size_t)inw[i] * 4);
-    } else if (hipMemcpyAsync(file_buf + data_at[i], (const uint32_t *)gdev[gof[i]].p + iwoff[i],
-                              (size_t)inw[i] * 4, hipMemcpyDeviceToHost, s) != hipSuccess) {
-      set_error("wv_batch_write_stream: payload D2H");
-      return WV_ERR_HIP;
-    }
-  }
-  if (direct && hipStreamSynchronize(s) != hipSuccess) {
-    set_error("wv_batch_write_stream: sync");
-    return WV_ERR_HIP;
+    if (inw[i] > 0 && !is_direct(i))
+      std::memcpy(file_buf + data_at[i], staged[gof[i]].data() + (iwoff[i] - glo[gof[i]]),
+                  (size_t)inw[i] * 4);
   }
   if (!tail.empty()) std::memcpy(file_buf + pos, tail.data(), tail.size());
   return need;

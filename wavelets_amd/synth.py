@@ -10,6 +10,7 @@ same seed gives the same bytes on every machine).  Arrays are indexed
                     interpolation of a 17^3 U[500,3500] lattice plus
                     Laplace(b=30) noise, clipped to 0..4095
   mixed_stream      the sizes / dtypes / wavelets of a mixed cubelet stream
+  serving_requests  the shapes / dtypes / kept fractions of serving requests
 """
 import numpy as np
 
@@ -64,7 +65,9 @@ def ellipsoids_f32(n=512, seed=2, count=40, noise_sigma=20.0):
         ax = rng.uniform(16, 128, 3)
         R = _rotation(rng)
         val = np.float32(rng.uniform(0, 4000))
-        r = float(ax.max())
+        # half extent along world axis i of p = c + R diag(ax) u, |u| <= 1;
+        # grid points a unit or more beyond it are well outside
+        r = np.sqrt(((R * ax[None, :]) ** 2).sum(axis=1))
         lo = np.maximum(np.floor(c - r), 0).astype(int)
         hi = np.minimum(np.ceil(c + r) + 1, n).astype(int)
         if (hi <= lo).any():
@@ -112,6 +115,23 @@ def mixed_stream(count=2000, seed=4):
         dtype = ("uint8", "int16", "float32")[int(rng.integers(3))]
         wavelet = 2 if rng.random() < 0.5 else 1
         out.append((size, dtype, wavelet))
+    return out
+
+
+def serving_requests(count, seed=5):
+    """The serving workload's requests: (size_xyz, dtype, keep) with edges
+    lognormal (median 128, sigma 0.6) clipped to [32, 512], depth lognormal
+    (median 48, sigma 1.0) clipped to [1, 512], dtypes u8 / i16 / f32 and
+    the kept fraction uniform on [0.02, 0.2] (threshold = 1 - keep)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        ex = int(np.clip(np.round(np.exp(rng.normal(np.log(128), 0.6))), 32, 512))
+        ey = int(np.clip(np.round(np.exp(rng.normal(np.log(128), 0.6))), 32, 512))
+        ez = int(np.clip(np.round(np.exp(rng.normal(np.log(48), 1.0))), 1, 512))
+        dt = ("uint8", "int16", "float32")[int(rng.integers(3))]
+        keep = float(rng.uniform(0.02, 0.2))
+        out.append(((ex, ey, ez), dt, keep))
     return out
 
 
