@@ -177,30 +177,31 @@ def copy_ceiling(wv, torch, dev, nbytes=1 << 30, reps=10):
     dst = torch.empty_like(src)
     best = None
     for nt in (0, 1):
-        for bpc in (2, 4, 8, 16):
-            for _ in range(3):
-                wv.copy_stream(src, dst, nt, bpc)
-            e0, e1 = wv.Event(), wv.Event()
-            e0.record()
-            for _ in range(reps):
-                wv.copy_stream(src, dst, nt, bpc)
-            e1.record()
-            torch.cuda.synchronize()
-            ms = e0.elapsed_ms(e1) / reps
-            gbps = 2.0 * nbytes / (ms * 1e-3) / 1e9
-            if best is None or gbps > best[0]:
-                best = (gbps, nt, bpc, ms)
+        for vpt in (4, 8, 16):
+            for bpc in (1, 2, 4, 8, 16):
+                for _ in range(3):
+                    wv.copy_stream(src, dst, nt, bpc, vpt)
+                e0, e1 = wv.Event(), wv.Event()
+                e0.record()
+                for _ in range(reps):
+                    wv.copy_stream(src, dst, nt, bpc, vpt)
+                e1.record()
+                torch.cuda.synchronize()
+                ms = e0.elapsed_ms(e1) / reps
+                gbps = 2.0 * nbytes / (ms * 1e-3) / 1e9
+                if best is None or gbps > best[0]:
+                    best = (gbps, nt, bpc, vpt, ms)
     ok = bool(torch.equal(src, dst))
     del src, dst
     torch.cuda.empty_cache()
-    gbps, nt, bpc, ms = best
-    log("copy ceiling: %.1f GB/s (%s, %d blocks per CU)" % (gbps, "non-temporal" if nt else
-                                                            "plain", bpc))
+    gbps, nt, bpc, vpt, ms = best
+    log("copy ceiling: %.1f GB/s (%s, %d x 16 B per thread, %d blocks per CU)" % (
+        gbps, "non-temporal" if nt else "plain", vpt, bpc))
     return {"gbps": round(gbps, 1), "frac_of_peak": round(gbps / HBM_PEAK_GBS, 4),
             "ms": round(ms, 4), "bytes_copied": nbytes, "nontemporal": bool(nt),
-            "blocks_per_cu": bpc, "copy_verified": ok,
-            "kernel": "wv_copy_stream (copy_kernel: 16 x 16-byte loads in flight per thread, "
-                      "64 KiB chunks per block)"}
+            "blocks_per_cu": bpc, "vec_per_thread": vpt, "copy_verified": ok,
+            "kernel": "wv_copy_stream (copy_kernel: 4, 8 or 16 x 16-byte loads in flight per "
+                      "thread; the best of hints x depth x blocks per CU)"}
 
 
 def measured_traffic():

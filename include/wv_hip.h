@@ -610,13 +610,13 @@ int wv_opt_test(wv_opt *o, float threshold, int bin_count, int quant_alg, wv_opt
                 void *stream);
 int wv_opt_destroy(wv_opt *o);
 
-/* measurement: device-to-device streaming copy with 128-bit accesses (16
-   loads in flight per thread; optional non-temporal hints), blocks_per_cu
-   blocks per CU over 64 KiB chunks; bytes and both pointers 16-byte
-   aligned.  The measured HBM ceiling beside the published peak.  No
-   counterpart in the reference. */
+/* measurement: device-to-device streaming copy with 128-bit accesses
+   (vec_per_thread = 4, 8 or 16 loads in flight per thread; optional
+   non-temporal hints), blocks_per_cu blocks per CU; bytes and both
+   pointers 16-byte aligned.  The measured HBM ceiling beside the published
+   peak.  No counterpart in the reference. */
 int wv_copy_stream(const void *src_d, void *dst_d, int64_t bytes, int nontemporal,
-                   int blocks_per_cu, void *stream);
+                   int blocks_per_cu, int vec_per_thread, void *stream);
 
 #ifdef __cplusplus
 }

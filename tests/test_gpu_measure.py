@@ -19,11 +19,11 @@ def wv():
 
 
 @pytest.mark.parametrize("nfloat", [4, 16384, 16384 * 3 + 36, (1 << 22) + 4])
-@pytest.mark.parametrize("nt,bpc", [(0, 2), (1, 8)])
-def test_copy_stream_copies_exactly(wv, nfloat, nt, bpc):
+@pytest.mark.parametrize("nt,bpc,vpt", [(0, 2, 4), (1, 8, 16), (1, 1, 8)])
+def test_copy_stream_copies_exactly(wv, nfloat, nt, bpc, vpt):
     src = torch.arange(nfloat, dtype=torch.float32, device="cuda")
     dst = torch.full_like(src, -1.0)
-    wv.copy_stream(src, dst, nt, bpc)
+    wv.copy_stream(src, dst, nt, bpc, vpt)
     torch.cuda.synchronize()
     assert torch.equal(src, dst)
 

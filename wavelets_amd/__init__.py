@@ -253,7 +253,7 @@ def _declare(L):
         if hasattr(L, "wv_batch_stage_times"):
             L.wv_batch_stage_times.argtypes = [P, P, C.POINTER(C.c_double)]
     if hasattr(L, "wv_copy_stream"):
-        L.wv_copy_stream.argtypes = [P, P, I64, C.c_int, C.c_int, P]
+        L.wv_copy_stream.argtypes = [P, P, I64, C.c_int, C.c_int, C.c_int, P]
     if hasattr(L, "wv_serve_create"):
         L.wv_serve_create.argtypes = [C.POINTER(ServeConfig), C.POINTER(C.c_void_p)]
         L.wv_serve_submit.argtypes = [P, C.POINTER(ServeRequest), C.POINTER(C.c_int64)]
@@ -395,14 +395,15 @@ def threshold(coef, frac, stats=None, stream=None, sync=True):
     return res, host
 
 
-def copy_stream(src, dst, nontemporal=True, blocks_per_cu=8, stream=None):
+def copy_stream(src, dst, nontemporal=True, blocks_per_cu=8, vec_per_thread=16,
+                stream=None):
     """wv_copy_stream: device-to-device copy of src into dst (same byte
     size, 16-byte multiples) with 128-bit accesses (measurement helper)."""
     nbytes = src.numel() * src.element_size()
     if dst.numel() * dst.element_size() != nbytes:
         raise ValueError("copy_stream: sizes differ")
     check(lib().wv_copy_stream(_ptr(src), _ptr(dst), nbytes, int(nontemporal),
-                               int(blocks_per_cu), _stream(stream)))
+                               int(blocks_per_cu), int(vec_per_thread), _stream(stream)))
 
 
 class Quantizer:

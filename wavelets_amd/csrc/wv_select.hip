@@ -26,14 +26,14 @@
 //            the partials and verifies the window: the rank falls inside it
 //            when below <= idx < below + candidates; otherwise the exact
 //            fallback takes every key, from the full 31-bit range;
-//   radix    two rounds over the candidates, relative to the window: each
-//            splits [base, base + 2^bits) into up to 2048 buckets (LDS
+//   radix    three rounds over the candidates, relative to the window:
+//            each splits [base, base + 2^bits) into up to 2048 buckets (LDS
 //            histograms merged with one atomic per bin) and the last block
-//            to finish keeps the rank's bucket, so the window's ~20 bits
-//            take the two rounds; the second round's last block finishes
-//            any bits left (the fallback's) on its own and writes the
-//            result record.
-// Four launches (sample, collect, two rounds); everything stays on the
+//            to finish keeps the rank's bucket (a window of 20-24 bits
+//            takes two or three rounds, the fallback's 31 bits three; a
+//            round with no bits left returns at once); the third round's
+//            last block writes the result record.
+// Five launches (sample, collect, three rounds); everything stays on the
 // device.
 #include "wv_device.h"
 #include "wv_internal.h"
@@ -566,7 +566,7 @@ __device__ void sel_result_dev(int64_t n, const SelState *S, wv_thresh_result *r
 // One round over SEL_RGRID blocks: LDS histograms of the digit, merged
 // into S->hist2 with one atomic per non-empty bin; the last block picks the
 // rank's bucket.  In the last round that block then finishes whatever bits
-// are left on its own (none for the sampled window) and writes the result.
+// are left on its own (none after three rounds) and writes the result.
 // (With the exact fallback the keys are all n coefficients and the window's
 // zeros are not counted apart.)
 __global__ void __launch_bounds__(WV_BLOCK)
@@ -660,8 +660,11 @@ hipError_t threshold_select(const float *coef, int64_t n, float frac,
   const int64_t nchunk = (n + SEL_CHUNK - 1) / SEL_CHUNK;
   const int grid = (int)(nchunk < SEL_GRID ? nchunk : SEL_GRID);
   hipLaunchKernelGGL(sel_collect, dim3(grid), dim3(WV_BLOCK), 0, s, coef, n, S, cand);
-  hipLaunchKernelGGL(sel_round, dim3(SEL_RGRID), dim3(WV_BLOCK), 0, s, cand, coef, n, S, res, 0);
-  hipLaunchKernelGGL(sel_round, dim3(SEL_RGRID), dim3(WV_BLOCK), 0, s, cand, coef, n, S, res, 1);
+  // three rounds cover any window (11 + 11 + 11 >= 31 bits); a round with
+  // no bits left returns at once
+  for (int r = 0; r < 3; r++)
+    hipLaunchKernelGGL(sel_round, dim3(SEL_RGRID), dim3(WV_BLOCK), 0, s, cand, coef, n, S, res,
+                       r == 2 ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -340,14 +340,19 @@ extern "C" int64_t wv_batch_write_stream(wv_batch *b, const int *offsets, int co
     set_error("wv_batch_write_stream: file buffer %lld < %lld", (long long)cap, (long long)need);
     return WV_ERR_NOMEM;
   }
-  // per record: a payload of 128 KB or more goes from the device straight
-  // to its place in the file image; smaller ones (where a copy's fixed cost
-  // dominates) come from one copy of their group's span of small payloads,
-  // then host copies
+  // records averaging 128 KB or more: each payload goes from the device
+  // straight to its place in the file image; smaller ones (where a copy's
+  // fixed cost dominates): one copy of each group's payloads, then host
+  // copies.  The choice is made for the whole stream from the average: the
+  // items of a group are interleaved, so staging only the small records of
+  // a group would copy the large ones between them twice.
   hipStream_t s = (hipStream_t)stream;
   std::memset(file_buf, 0, 64);
   std::memcpy(file_buf, "SCU cubelets 1.0\n", 17);
-  auto is_direct = [&](int i) { return inw[i] * 4 >= (int64_t)(128 << 10); };
+  int64_t payload_words = 0;
+  for (int i = 0; i < n; i++) payload_words += inw[i];
+  const bool direct_all = payload_words * 4 >= (int64_t)n * (128 << 10);
+  auto is_direct = [&](int) { return direct_all; };
   const size_t ng = gdev.size();
   std::vector<int64_t> glo(ng, INT64_MAX), ghi(ng, 0);
   for (int i = 0; i < n; i++)
