@@ -145,6 +145,19 @@ int wv_threshold(const float *coef_d, int64_t n, float frac,
                  const void *stats_d, void *ws_d, wv_thresh_result *result_d,
                  void *stream);
 
+/* wv_threshold without a workspace argument and without allocating: no
+   counterpart in the reference.  For callers that cannot allocate device
+   memory (code under HIP graph capture, tight serving loops).  The select's
+   state and candidate list live in a static device array of the library
+   (one per GPU, about 16 MiB, held until the process ends), so n is at most
+   WV_NOALLOC_MAX_N (WV_ERR_ARG beyond).  Every call on a GPU shares that
+   GPU's array: calls that overlap on one GPU (other streams, other host
+   threads) race and must be serialised by the caller.  Same results as
+   wv_threshold; does not synchronise. */
+#define WV_NOALLOC_MAX_N 4194304
+int wv_threshold_noalloc(const float *coef_d, int64_t n, float frac,
+                         wv_thresh_result *result_d, void *stream);
+
 /* ------------------------------------------------------------------ *
  * Quantizer parameters, computed on the host with the host libm exactly
  * as QuantUniform::init / QuantLog::init / QuantCodebook::init

@@ -153,6 +153,8 @@ def _declare(L):
     L.wv_threshold_ws_bytes.restype = I64
     L.wv_threshold_ws_bytes.argtypes = [I64]
     L.wv_threshold.argtypes = [P, I64, C.c_float, P, P, P, P]
+    if hasattr(L, "wv_threshold_noalloc"):
+        L.wv_threshold_noalloc.argtypes = [P, I64, C.c_float, P, P]
     L.wv_quant_params_init.argtypes = [C.POINTER(QuantParams), C.c_int, C.c_int,
                                        C.c_float, C.c_float, P, P]
     L.wv_quant_params_free.argtypes = [C.POINTER(QuantParams)]
@@ -393,6 +395,22 @@ def threshold(coef, frac, stats=None, stream=None, sync=True):
         return res, None
     host = ThreshResult.from_buffer_copy(res.cpu().numpy().tobytes())
     return res, host
+
+
+NOALLOC_MAX_N = 4194304     # WV_NOALLOC_MAX_N
+
+
+def threshold_noalloc(coef, frac, stream=None, sync=True):
+    """wv_threshold_noalloc: the select on the library's static scratch (no
+    workspace, no allocation inside; at most NOALLOC_MAX_N coefficients).
+    Returns the device result record and, with sync, the host ThreshResult."""
+    import torch
+    res = torch.zeros(C.sizeof(ThreshResult), dtype=torch.uint8, device=coef.device)
+    check(lib().wv_threshold_noalloc(_ptr(coef), coef.numel(), C.c_float(frac), _ptr(res),
+                                     _stream(stream)))
+    if not sync:
+        return res, None
+    return res, ThreshResult.from_buffer_copy(res.cpu().numpy().tobytes())
 
 
 def copy_stream(src, dst, nontemporal=True, blocks_per_cu=8, vec_per_thread=16,

@@ -35,6 +35,7 @@
 //            last block writes the result record.
 // Five launches (sample, collect, three rounds); everything stays on the
 // device.
+#include <mutex>
 #include "wv_device.h"
 #include "wv_internal.h"
 
@@ -737,6 +738,23 @@ __global__ void __launch_bounds__(1024) bsel_finish(const BSelJob *J) {
   if (threadIdx.x == 0) sel_result_dev(j.n, S, j.res);
 }
 
+// ---- the allocation-free threshold entry point ------------------------------
+// Its workspace (the select state and a candidate list of up to
+// WV_NOALLOC_MAX_N keys: the layout threshold_ws_bytes describes) is this
+// file-scope device array, one copy per GPU, held until the process ends.
+// Host code never passes the array's name to a kernel: on the host it is a
+// stand-in, not a device address.  Each device's address comes from
+// hipGetSymbolAddress, looked up once per device with that device current.
+constexpr size_t kNoallocScratchBytes =
+    ((sizeof(SelState) + 255) & ~(size_t)255) + (size_t)WV_NOALLOC_MAX_N * 4 + 256;
+static __device__ __align__(256) unsigned char scratch[kNoallocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices];   // per device; null until looked up
+}  // namespace
+
 int64_t sel_state_bytes() { return (int64_t)((sizeof(SelState) + 255) & ~(size_t)255); }
 int sel_max_collect_blocks() { return SEL_GRID; }
 int64_t sel_collect_chunk() { return SEL_CHUNK; }
@@ -751,4 +769,57 @@ hipError_t threshold_select_batch(const BSelJob *jobs_d, const int *blk0_d, int 
   return hipGetLastError();
 }
 
+// The current device's copy of the scratch array (looked up on the first
+// call on each device).
+static int noalloc_scratch(void **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) {
+    set_error("wv_threshold_noalloc: hipGetDevice: %s", hipGetErrorString(e));
+    return WV_ERR_HIP;
+  }
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("wv_threshold_noalloc: device %d beyond %d", dev, kMaxDevices);
+    return WV_ERR_UNSUPPORTED;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error("wv_threshold_noalloc: scratch address: %s", hipGetErrorString(e));
+      return WV_ERR_HIP;
+    }
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return WV_OK;
+}
+
 }  // namespace wv
+
+extern "C" int wv_threshold_noalloc(const float *coef_d, int64_t n, float frac,
+                                    wv_thresh_result *result_d, void *stream) {
+  using namespace wv;
+  if (!coef_d || !result_d || n <= 0) {
+    set_error("wv_threshold_noalloc: bad arguments");
+    return WV_ERR_ARG;
+  }
+  if (n > WV_NOALLOC_MAX_N) {
+    set_error("wv_threshold_noalloc: %lld coefficients exceed the static scratch's %d "
+              "(use wv_threshold with a workspace)", (long long)n, (int)WV_NOALLOC_MAX_N);
+    return WV_ERR_ARG;
+  }
+  void *ws = nullptr;
+  const int rc = noalloc_scratch(&ws);
+  if (rc != WV_OK) return rc;
+  // the sample kernel initialises every state field the later kernels read,
+  // so nothing is cleared from the host
+  const hipError_t e = threshold_select(coef_d, n, frac, nullptr, ws, result_d,
+                                        (hipStream_t)stream);
+  if (e != hipSuccess) {
+    set_error("wv_threshold_noalloc: %s", hipGetErrorString(e));
+    return WV_ERR_HIP;
+  }
+  return WV_OK;
+}
