@@ -66,6 +66,7 @@ struct Group {
   PassSpec spec[3];
   int npass;
   int dst_out[3];              // per pass: 1 writes the coefficients, 0 the bins buffer
+  int dst_aligned = 1;         // every output pointer of the group 16-byte aligned
   const void **src_tab_d[3];   // per pass: device pointer tables
   float **dst_tab_d[3];
   Tuning tune;
@@ -209,6 +210,7 @@ static int enqueue_group(wv_batch *b, Group &G, hipStream_t s, int ncub_limit) {
     P.tcap = G.tune.tcap;
     P.ept0 = G.tune.ept0;
     P.bpc = G.tune.bpc;
+    P.dst_tab_aligned = G.dst_aligned;
     const int ntiles = plan_pass(P, G.spec[i], i == 0);
     if (ntiles < 0) { set_error("batch: unsupported pass geometry"); return WV_ERR_UNSUPPORTED; }
     const bool clamp = i == 0;
@@ -607,6 +609,7 @@ int wv_batch_plan(const wv_batch_item *items, int n, int flags, wv_batch **out) 
         src[c] = p == 0 ? it.in_d
                         : (const void *)(G.dst_out[p - 1] ? it.coef_d : inter);
         dst[c] = G.dst_out[p] ? it.coef_d : inter;
+        if (((uintptr_t)dst[c] & 15) != 0) G.dst_aligned = 0;
       }
       G.src_tab_d[p] = (const void **)tp;
       BCHK(hipMemcpy(tp, src.data(), sizeof(void *) * nc, hipMemcpyHostToDevice), "H2D");

@@ -40,6 +40,20 @@
 #define WV_NT_ST 1
 #endif
 
+// Global (address space 1) views of generic pointers.  Pointers that reach a
+// kernel through a struct or a table are generic to the compiler, which
+// then emits flat instructions; flat accesses also count on LGKM_CNT, so
+// every wait for an LDS result would wait for the loads in flight as well.
+#define WV_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ inline const WV_GLOBAL T *gptr(const T *p) { return (const WV_GLOBAL T *)p; }
+template <typename T>
+__device__ inline WV_GLOBAL T *gptr(T *p) { return (WV_GLOBAL T *)p; }
+// 16-byte vectors for those views (HIP's float4 / int4 classes cannot be
+// copied out of address space 1)
+typedef float wv_f4 __attribute__((ext_vector_type(4)));
+typedef int wv_i4 __attribute__((ext_vector_type(4)));
+
 // Fused statistics block (device).  Layout is part of the ABI size
 // WV_STATS_BYTES.
 struct WvStats {

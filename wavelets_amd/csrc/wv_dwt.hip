@@ -183,14 +183,31 @@ __device__ inline bool exact_tile(const DwtPass &P, const Tile &t) {
 // wv_device.h): a pass reads and writes every sample once, so neither side
 // is reused from the caches (measured on the 512^3 float cubelet: 58.6% ->
 // 60.0% of 8 TB/s over the three forward passes).
+//
+// The cubelet pointers reach the kernel through the pass struct and its
+// tables, so the compiler cannot tell they address global memory and would
+// emit flat instructions.  Flat accesses also count on LGKM_CNT, so the
+// lgkmcnt(0) wait in front of every LDS barrier of the levels would wait
+// for the next tile's loads as well and undo the software pipelining.
+// Every access therefore goes through a global (address space 1) pointer
+// (gptr, wv_device.h).
+
 template <typename T>
 __device__ inline T ld_tile(const T *p) {
-  if (WV_NT_LD) return __builtin_nontemporal_load(p);
-  return *p;
+  if (WV_NT_LD) return __builtin_nontemporal_load(gptr(p));
+  return *gptr(p);
 }
+template <typename T>
+__device__ inline T ld_g(const T *p) { return *gptr(p); }
 __device__ inline void st_tile(float *p, float v) {
-  if (WV_NT_ST) __builtin_nontemporal_store(v, p);
-  else *p = v;
+  if (WV_NT_ST) __builtin_nontemporal_store(v, gptr(p));
+  else *gptr(p) = v;
+}
+__device__ inline void st_g(float *p, float v) { *gptr(p) = v; }
+typedef float wv_vec4 __attribute__((ext_vector_type(4)));
+__device__ inline void st_tile4(wv_vec4 *p, wv_vec4 v) {
+  if (WV_NT_ST) __builtin_nontemporal_store(v, gptr(p));
+  else *gptr(p) = v;
 }
 
 // 64-bit value that is the same in every lane, moved to scalar registers
@@ -238,7 +255,7 @@ __device__ inline void load_regs(const DwtPass &P, const Tile &t, const size_t *
     for (int u = 0; u < EPT; u++) {
       int l, p;
       lm_index<TT, EPT, NT>(P, tid, u, l, p);
-      v[u] = to_f(src[lb[l] + min(p, pmax)]);
+      v[u] = to_f(ld_g(src + lb[l] + min(p, pmax)));
     }
   } else {
     const PosMap m = pos_map(P.ld, t);
@@ -246,7 +263,8 @@ __device__ inline void load_regs(const DwtPass &P, const Tile &t, const size_t *
     for (int u = 0; u < EPT; u++) {
       int l, p;
       pm_index<TT, EPT, NT>(P, tid, u, l, p);
-      v[u] = to_f(src[m.rowbase + (size_t)min(p, m.pcap) * m.pstride + min(t.a + l, m.xcap)]);
+      v[u] = to_f(ld_g(src + m.rowbase + (size_t)min(p, m.pcap) * m.pstride +
+                       min(t.a + l, m.xcap)));
     }
   }
 }
@@ -312,6 +330,22 @@ __device__ inline void store_tile(const DwtPass &P, const Tile &t, const float *
 #pragma unroll
         for (int j = 0; j < JN; j++) st_tile(base + j * 64 + lane, b[j * 64 * LS + li]);
       }
+    } else if (TT >= 32 && TT % 32 == 0 && P.vec_st) {
+      // 16-byte vectors of 4 neighbouring lines: a half-wave covers 4 rows
+      // of 32 lines (8 lanes per 128-byte row piece), so its LDS reads
+      // (row stride TT + 1) fall on 32 distinct banks
+      constexpr int CB = TT > 0 ? TT / 32 : 1;   // 32-line blocks per row
+      const PosMap m = pos_map(P.st, t);
+      float *base = t.dst + m.rowbase + t.a;
+      const size_t ps = (size_t)m.pstride;
+      const int cq = tid & 7, rr = (tid >> 3) & 3;
+#pragma unroll
+      for (int j = 0; j < EPT / 4; j++) {
+        const int rest = j * (NT / 32) + (tid >> 5);
+        const int p = (rest / CB) * 4 + rr, l = (rest % CB) * 32 + cq * 4;
+        const float *r = tout + p * LS + l;
+        st_tile4((wv_vec4 *)(base + (size_t)p * ps + l), wv_vec4{r[0], r[1], r[2], r[3]});
+      }
     } else {
       constexpr int PSTEP = NT / (TT > 0 ? TT : 1);
       const PosMap m = pos_map(P.st, t);
@@ -333,7 +367,7 @@ __device__ inline void store_tile(const DwtPass &P, const Tile &t, const float *
       int l, p;
       lm_index<TT, EPT, NT>(P, tid, u, l, p);
       p = min(p, plast);
-      t.dst[lb[l] + p] = tout[p * LS + l];
+      st_g(t.dst + lb[l] + p, tout[p * LS + l]);
     }
   } else {
     const PosMap m = pos_map(P.st, t);
@@ -342,7 +376,7 @@ __device__ inline void store_tile(const DwtPass &P, const Tile &t, const float *
       int l, p;
       pm_index<TT, EPT, NT>(P, tid, u, l, p);
       p = min(p, plast);
-      t.dst[m.rowbase + (size_t)p * m.pstride + min(t.a + l, m.xcap)] = tout[p * LS + l];
+      st_g(t.dst + m.rowbase + (size_t)p * m.pstride + min(t.a + l, m.xcap), tout[p * LS + l]);
     }
   }
 }
@@ -638,6 +672,15 @@ dwt_pass_kernel(DwtPass P, int ntiles) {
 }
 
 // ---- host side ------------------------------------------------------------
+// WV_DWT_VEC_ST=0 keeps the segment stores scalar (for comparison)
+static bool vec_store_off() {
+  static const bool v = [] {
+    const char *e = getenv("WV_DWT_VEC_ST");
+    return e && e[0] == '0';
+  }();
+  return v;
+}
+
 // resident blocks per CU override (autotuning hook): WV_DWT_BPC
 static int bpc_override() {
   static const int v = [] {
@@ -907,6 +950,12 @@ hipError_t launch_pass(const DwtPass &P, int ntiles, int wavelet, bool inverse,
   Q.exact = (long long)P.L * P.T == (long long)P.nt * P.ept && P.T >= 8 &&
             P.L % 64 == 0 && unpadded &&
             ((long long)(P.nt / P.T) * stride + P.T) * 4 < (1ll << 31);
+  // segments of 32+ lines as 16-byte vector stores: rows, positions and the
+  // output base 16-byte aligned
+  const bool dst_aligned = P.dst_tab ? P.dst_tab_aligned != 0
+                                     : ((uintptr_t)P.dst & 15) == 0;
+  Q.vec_st = Q.exact && !P.st_line && P.T % 32 == 0 && P.st.sp % 4 == 0 &&
+             P.st.sv % 4 == 0 && dst_aligned && !vec_store_off();
   if (wavelet == WV_WAVELET_CDF97)
     return inverse ? launch_wave<WV_WAVELET_CDF97, true>(Q, src_dtype, clamp, ntiles, s)
                    : launch_wave<WV_WAVELET_CDF97, false>(Q, src_dtype, clamp, ntiles, s);

@@ -205,10 +205,10 @@ __device__ inline void load_bins(const EncArgs &A, int64_t c, int32_t e[ENC_EPT]
   const int t = threadIdx.x;
   const int64_t i0 = c * ENC_C + (int64_t)ENC_EPT * t;
   if (A.vec && i0 + ENC_EPT <= A.n) {
-    const int4 *p = (const int4 *)(A.bins + i0);
+    const WV_GLOBAL wv_i4 *p = (const WV_GLOBAL wv_i4 *)gptr(A.bins + i0);
 #pragma unroll
     for (int j = 0; j < ENC_EPT / 4; j++) {
-      const int4 v = p[j];
+      const wv_i4 v = p[j];
       e[4 * j] = v.x;
       e[4 * j + 1] = v.y;
       e[4 * j + 2] = v.z;
@@ -216,7 +216,7 @@ __device__ inline void load_bins(const EncArgs &A, int64_t c, int32_t e[ENC_EPT]
     }
   } else {
 #pragma unroll
-    for (int k = 0; k < ENC_EPT; k++) e[k] = i0 + k < A.n ? A.bins[i0 + k] : ENC_SENT;
+    for (int k = 0; k < ENC_EPT; k++) e[k] = i0 + k < A.n ? gptr(A.bins)[i0 + k] : ENC_SENT;
   }
   first[t] = e[0];
   __syncthreads();
@@ -225,7 +225,7 @@ __device__ inline void load_bins(const EncArgs &A, int64_t c, int32_t e[ENC_EPT]
     nx = first[t + 1];
   } else {
     const int64_t j = (c + 1) * ENC_C;
-    nx = j < A.n ? A.bins[j] : ENC_SENT;
+    nx = j < A.n ? gptr(A.bins)[j] : ENC_SENT;
   }
   next_dup = nx == A.zb;
 }
@@ -240,26 +240,27 @@ struct Chunk {
 
 // full chunk, 16-byte aligned bins: exactly five loads per thread
 __device__ inline void fetch_fast(const EncArgs &A, int64_t c, Chunk &k) {
-  const int4 *p = (const int4 *)(A.bins + c * ENC_C + (int64_t)ENC_EPT * threadIdx.x);
+  const WV_GLOBAL wv_i4 *p =
+      (const WV_GLOBAL wv_i4 *)gptr(A.bins + c * ENC_C + (int64_t)ENC_EPT * threadIdx.x);
 #pragma unroll
   for (int j = 0; j < ENC_EPT / 4; j++) {
-    const int4 v = p[j];
+    const wv_i4 v = p[j];
     k.e[4 * j] = v.x;
     k.e[4 * j + 1] = v.y;
     k.e[4 * j + 2] = v.z;
     k.e[4 * j + 3] = v.w;
   }
   const int64_t j = (c + 1) * ENC_C;
-  const int32_t a = A.bins[j < A.n ? j : A.n - 1];
+  const int32_t a = gptr(A.bins)[j < A.n ? j : A.n - 1];
   k.after = j < A.n ? a : ENC_SENT;
 }
 
 __device__ inline void fetch_plain(const EncArgs &A, int64_t c, Chunk &k) {
   const int64_t i0 = c * ENC_C + (int64_t)ENC_EPT * threadIdx.x;
 #pragma unroll
-  for (int i = 0; i < ENC_EPT; i++) k.e[i] = i0 + i < A.n ? A.bins[i0 + i] : ENC_SENT;
+  for (int i = 0; i < ENC_EPT; i++) k.e[i] = i0 + i < A.n ? gptr(A.bins)[i0 + i] : ENC_SENT;
   const int64_t j = (c + 1) * ENC_C;
-  k.after = j < A.n ? A.bins[j] : ENC_SENT;
+  k.after = j < A.n ? gptr(A.bins)[j] : ENC_SENT;
 }
 
 // Runs body(c, chunk, parity) over chunks [cb, ce) in order.  Full chunks
@@ -778,7 +779,7 @@ __global__ void __launch_bounds__(ENC_NT) enc_write(EncArgs A0, const EncArgs *J
         if (i == 0 || i == (int)nw - 1) {
           if (v) atomicOr(&A.words[wfirst + i], v);
         } else {
-          A.words[wfirst + i] = v;
+          gptr(A.words)[wfirst + i] = v;
         }
       }
     }
@@ -933,7 +934,7 @@ __global__ void __launch_bounds__(ENC_NT) enc_write_fast(EncArgs A0, const EncAr
         if (i == 0 || i == (int)nw - 1) {
           if (v) atomicOr(&A.words[wfirst + i], v);
         } else {
-          A.words[wfirst + i] = v;
+          gptr(A.words)[wfirst + i] = v;
         }
       }
     }

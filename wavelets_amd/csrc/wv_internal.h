@@ -53,6 +53,9 @@ struct DwtPass {
   // first forward pass of a box inside a larger volume: source samples
   // between rows and slices (0: the dense unpadded source)
   int64_t src_sy, src_sz;
+  // every dst_tab pointer is 16-byte aligned (batches; P.dst is checked
+  // directly); vec_st: exact tiles store their segments as 16-byte vectors
+  int dst_tab_aligned, vec_st;
 };
 
 // Memory layouts of a cubelet, fastest axis first: x,y,z (the input and

@@ -170,7 +170,7 @@ __device__ __forceinline__ void quant_hist_body(const float *coef, int64_t n, co
   const int ntab = q.n_pos + (ALG == WV_QUANT_LOG ? q.n_neg : 0);
   const bool tab_in_lds = ntab <= tab_cap;
   if (ALG != WV_QUANT_UNIFORM && tab_in_lds)
-    for (int i = threadIdx.x; i < ntab; i += WV_BLOCK) tab[i] = q.table[i];
+    for (int i = threadIdx.x; i < ntab; i += WV_BLOCK) tab[i] = gptr(q.table)[i];
   const bool hist_in_lds = q.bin_count <= hist_cap;
   if (hist_in_lds)
     for (int i = threadIdx.x; i < q.bin_count; i += WV_BLOCK) h[i] = 0;
@@ -206,10 +206,13 @@ __device__ __forceinline__ void quant_hist_body(const float *coef, int64_t n, co
     constexpr bool ROLL = decltype(roll)::value;
     const bool aligned = (((uintptr_t)coef | (uintptr_t)bins) & 15) == 0;
     const int64_t nvec = aligned ? (n >> 2) : 0;
-    const float4 *c4 = (const float4 *)coef;
-    int4 *b4 = (int4 *)bins;
-    auto one4 = [&](float4 v, int64_t i) {
-      int4 r;
+    // global views: the batch passes these pointers through its job table
+    const WV_GLOBAL wv_f4 *c4 = gptr((const wv_f4 *)coef);
+    WV_GLOBAL wv_i4 *b4 = gptr((wv_i4 *)bins);
+    const WV_GLOBAL float *cg = gptr(coef);
+    WV_GLOBAL int32_t *bg = gptr(bins);
+    auto one4 = [&](wv_f4 v, int64_t i) {
+      wv_i4 r;
       r.x = qf(v.x);
       r.y = qf(v.y);
       r.z = qf(v.z);
@@ -221,8 +224,8 @@ __device__ __forceinline__ void quant_hist_body(const float *coef, int64_t n, co
     const int64_t stride = (int64_t)nblk * WV_BLOCK;
     int64_t i0 = (int64_t)bidx * WV_BLOCK + threadIdx.x;
     for (; i0 + 3 * stride < nvec; i0 += 4 * stride) {
-      float4 v0 = c4[i0], v1 = c4[i0 + stride], v2 = c4[i0 + 2 * stride],
-             v3 = c4[i0 + 3 * stride];
+      wv_f4 v0 = c4[i0], v1 = c4[i0 + stride], v2 = c4[i0 + 2 * stride],
+            v3 = c4[i0 + 3 * stride];
       if constexpr (ROLL) {
 #pragma unroll 1
         for (int u = 0; u < 4; u++) {
@@ -240,8 +243,8 @@ __device__ __forceinline__ void quant_hist_body(const float *coef, int64_t n, co
     // tail
     for (int64_t i = nvec * 4 + (int64_t)bidx * WV_BLOCK + threadIdx.x; i < n;
          i += (int64_t)nblk * WV_BLOCK) {
-      const int r = qf(coef[i]);
-      bins[i] = r;
+      const int r = qf(cg[i]);
+      bg[i] = r;
       count(r);
     }
   };

@@ -5,36 +5,39 @@
 //
 // The bit patterns of |x| order like the values, so the select works on
 // 31-bit keys:
-//   sample   one block reads 16K keys spread over the array (jittered
-//            strata: key i from a hashed position inside the i-th 1/16K of
-//            the array -- the 32-bit hash of i times the stratum's width,
-//            shifted down 32 bits -- so the sample cannot alias with the sub-band
-//            layout of the coefficients), and from their two-level
-//            histogram picks a key window [lo, hi] around the target rank
-//            with a margin of 1.25% of the ranks on each side (the sampled
-//            rank's error is about 0.3% rms on cubelets of 4 M keys);
+//   sample   16K keys spread over the array (jittered strata: key i from a
+//            hashed position inside the i-th 1/16K of the array -- the
+//            32-bit hash of i times the stratum's width, shifted down 32
+//            bits -- so the sample cannot alias with the sub-band layout of
+//            the coefficients; gathered by 64 blocks so the scattered loads
+//            are in flight from many CUs), and from their two-level
+//            histogram one block picks a key window [lo, hi] around the
+//            target rank with a margin of 1.25% of the ranks on each side
+//            (the sampled rank's error is about 0.3% rms on cubelets of 4 M
+//            keys);
 //   collect  one streaming read of the coefficients by SEL_GRID blocks:
 //            counts the keys below the window, appends the keys inside it
 //            to a candidate list (buffered per wave in LDS, one global
-//            atomic per flush; no block barrier in the streaming loop, the
-//            next sub-chunk's loads in flight while one is examined) and
+//            atomic per flush; no block barrier in the streaming loop) and
 //            writes per-block partials (count below, min, max, max |x|,
 //            zeros).  Zero keys are only counted, never listed: exact zeros
 //            are the one tie of any size in wavelet coefficients (flat
 //            regions of integer data), and a window that starts at 0 would
-//            otherwise list all of them.  The last block to finish reduces
-//            the partials and verifies the window: the rank falls inside it
-//            when below <= idx < below + candidates; otherwise the exact
-//            fallback takes every key, from the full 31-bit range;
+//            otherwise list all of them;
 //   radix    three rounds over the candidates, relative to the window:
 //            each splits [base, base + 2^bits) into up to 2048 buckets (LDS
-//            histograms merged with one atomic per bin) and the last block
-//            to finish keeps the rank's bucket (a window of 20-24 bits
-//            takes two or three rounds, the fallback's 31 bits three; a
-//            round with no bits left returns at once); the third round's
-//            last block writes the result record.
-// Five launches (sample, collect, three rounds); everything stays on the
-// device.
+//            histograms merged with one atomic per bin).  Every block of a
+//            round first derives the plan itself from the state -- the
+//            partials reduced and the window verified (the rank falls inside
+//            it when below <= idx < below + candidates, else the exact
+//            fallback takes every key from the full 31-bit range), then the
+//            buckets of the earlier rounds -- so no launch needs a grid-wide
+//            synchronisation.  A window of 20-24 bits takes two or three
+//            rounds, the fallback's 31 bits three; a round with no bits
+//            left returns at once;
+//   finish   one block derives the final plan and writes the result record.
+// Seven launches (gather, sample, collect, three rounds, finish);
+// everything stays on the device.
 #include <mutex>
 #include "wv_device.h"
 #include "wv_internal.h"
@@ -61,9 +64,9 @@ struct SelState {
   uint32_t zwin;       // zero keys inside the window (counted, not listed)
   uint32_t min_ord, max_ord, max_key;
   uint32_t overflow;    // the candidate list outgrew its capacity
-  uint32_t ctr;         // blocks done (the last one of a launch finishes it)
-  uint32_t hist1[WV_H1_BINS];
-  uint32_t hist2[WV_H1_BINS];   // radix-round histogram
+  uint32_t pad;
+  uint32_t rhist[3][WV_H1_BINS];  // the radix rounds' digit histograms
+  uint32_t skeys[SEL_SAMPLE];     // the sampled keys (plain path: gathered by many blocks)
   uint32_t part[SEL_GRID][5];   // per collect block: below, min, max, max key, zeros
 };
 
@@ -229,8 +232,26 @@ __device__ inline uint32_t sample_hash(uint32_t x) {
   return x;
 }
 
+// position of sampled key i of n: one per stratum [i n / m, (i + 1) n / m)
+// at a hashed offset, the hash scaled to the stratum's width (multiply-shift)
+__device__ inline int64_t sample_pos(int i, int64_t n) {
+  const int64_t lo = (int64_t)i * n / SEL_SAMPLE, hi = (int64_t)(i + 1) * n / SEL_SAMPLE;
+  const uint64_t w = (uint64_t)(hi - lo);
+  return lo + (int64_t)(((uint64_t)sample_hash((uint32_t)i) * w) >> 32);
+}
+
+// The plain path's gather of the sample (n > SEL_SAMPLE): one key per
+// thread over SEL_SAMPLE / 256 blocks, so the scattered loads are in flight
+// from many CUs at once rather than queued behind one.
+__global__ void __launch_bounds__(WV_BLOCK)
+sel_gather(const float *__restrict__ coef, int64_t n, SelState *S) {
+  const int i = blockIdx.x * WV_BLOCK + threadIdx.x;
+  if (i < SEL_SAMPLE) S->skeys[i] = __builtin_bit_cast(uint32_t, coef[sample_pos(i, n)]) & 0x7fffffffu;
+}
+
+// pre: the gathered sample (sel_gather), or null to gather it here
 __device__ void sel_sample_dev(const float *__restrict__ coef, int64_t n, float frac,
-                               SelState *S) {
+                               SelState *S, const uint32_t *pre) {
   __shared__ uint32_t keys[SEL_SAMPLE];
   __shared__ uint32_t h[WV_H1_BINS];
   __shared__ uint32_t res_b[2], res_r[2];
@@ -246,16 +267,14 @@ __device__ void sel_sample_dev(const float *__restrict__ coef, int64_t n, float 
   if (n <= SEL_SAMPLE) {
     for (int i = t; i < m; i += 1024)
       keys[i] = __builtin_bit_cast(uint32_t, coef[i]) & 0x7fffffffu;
+  } else if (pre) {
+    for (int i = t; i < SEL_SAMPLE; i += 1024) keys[i] = pre[i];
   } else {
-    // all 16 loads of a thread issued before any is used; the offset in
-    // the stratum is the hash scaled to its width (multiply-shift)
+    // all 16 loads of a thread issued before any is used
 #pragma unroll
     for (int k = 0; k < SEL_SAMPLE / 1024; k++) {
       const int i = t + k * 1024;
-      const int64_t lo = (int64_t)i * n / SEL_SAMPLE, hi = (int64_t)(i + 1) * n / SEL_SAMPLE;
-      const uint64_t w = (uint64_t)(hi - lo);
-      const int64_t pos = lo + (int64_t)(((uint64_t)sample_hash((uint32_t)i) * w) >> 32);
-      keys[i] = __builtin_bit_cast(uint32_t, coef[pos]) & 0x7fffffffu;
+      keys[i] = __builtin_bit_cast(uint32_t, coef[sample_pos(i, n)]) & 0x7fffffffu;
     }
   }
   for (int i = t; i < WV_H1_BINS; i += 1024) h[i] = 0;
@@ -295,8 +314,7 @@ __device__ void sel_sample_dev(const float *__restrict__ coef, int64_t n, float 
     if (end == 0) key_lo = (b << WV_H1_SHIFT) | (sub << WV_H2_SHIFT);
     else key_hi = (b << WV_H1_SHIFT) | (sub << WV_H2_SHIFT) | ((1u << WV_H2_SHIFT) - 1);
   }
-  for (int i = t; i < WV_H1_BINS; i += 1024) S->hist1[i] = 0;
-  for (int i = t; i < WV_H1_BINS; i += 1024) S->hist2[i] = 0;
+  for (int i = t; i < 3 * WV_H1_BINS; i += 1024) (&S->rhist[0][0])[i] = 0;
   if (t == 0) {
     S->idx = (int32_t)(special ? n : idx);
     S->special = special;
@@ -314,19 +332,17 @@ __device__ void sel_sample_dev(const float *__restrict__ coef, int64_t n, float 
     S->max_ord = 0u;
     S->max_key = 0u;
     S->overflow = 0;
-    S->ctr = 0;
   }
 }
 
 __global__ void __launch_bounds__(1024)
 sel_sample(const float *__restrict__ coef, int64_t n, float frac, SelState *S) {
-  sel_sample_dev(coef, n, frac, S);
+  sel_sample_dev(coef, n, frac, S, n > SEL_SAMPLE ? S->skeys : nullptr);
 }
 
 // ---- 2. collect ------------------------------------------------------------
 // nblk blocks, each streaming a contiguous range in sub-chunks of SEL_CHUNK
-// keys (16-byte loads, 16 keys per thread; the next sub-chunk's loads are
-// issued before the current one is examined).  Each wave appends its
+// keys (16-byte loads, 16 keys per thread).  Each wave appends its
 // in-window keys to its own LDS buffer and flushes it to the candidate
 // list (one global atomic per flush) when it is nearly full, so the
 // streaming loop has no block barrier.  Per-block results go to S->part
@@ -334,18 +350,19 @@ sel_sample(const float *__restrict__ coef, int64_t n, float frac, SelState *S) {
 __device__ inline void sel_load_chunk(const float *__restrict__ coef, int64_t n, int64_t c,
                                       int tid, float *v) {
   const int64_t start = c * SEL_CHUNK;
+  const WV_GLOBAL float *cg = gptr(coef);
   if (start + SEL_CHUNK <= n) {
-    const float4 *c4 = (const float4 *)(coef + start);
+    const WV_GLOBAL wv_f4 *c4 = (const WV_GLOBAL wv_f4 *)(cg + start);
 #pragma unroll
     for (int u = 0; u < SEL_ITEMS / 4; u++) {
-      const float4 q = c4[tid + u * WV_BLOCK];
+      const wv_f4 q = c4[tid + u * WV_BLOCK];
       v[4 * u] = q.x; v[4 * u + 1] = q.y; v[4 * u + 2] = q.z; v[4 * u + 3] = q.w;
     }
   } else {
 #pragma unroll
     for (int u = 0; u < SEL_ITEMS; u++) {
       const int64_t i = start + (int64_t)(u >> 2) * 4 * WV_BLOCK + 4 * tid + (u & 3);
-      v[u] = i < n ? coef[i] : __builtin_bit_cast(float, 0xffffffffu);
+      v[u] = i < n ? cg[i] : __builtin_bit_cast(float, 0xffffffffu);
     }
   }
 }
@@ -363,15 +380,16 @@ __device__ inline void sel_flush(uint32_t *buf, uint32_t &wcnt, SelState *S, uin
   }
   base = __shfl(base, 0);
   if (base + wcnt <= cap)
-    for (uint32_t i = lane; i < wcnt; i += 64) cand[base + i] = buf[i];
+    for (uint32_t i = lane; i < wcnt; i += 64) gptr(cand)[base + i] = buf[i];
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   wcnt = 0;
 }
 
-__device__ void sel_collect_dev(const float *__restrict__ coef, int64_t n, SelState *S,
-                                uint32_t *cand, uint32_t cap, int bidx, int nblk) {
+__device__ __attribute__((always_inline)) void
+sel_collect_dev(const float *__restrict__ coef, int64_t n, SelState *S, uint32_t *cand,
+                uint32_t cap, int bidx, int nblk) {
   __shared__ uint32_t wbuf[WV_BLOCK / 64][SEL_WAVE_CAP];
   __shared__ uint32_t red[5][WV_BLOCK / 64];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -382,11 +400,9 @@ __device__ void sel_collect_dev(const float *__restrict__ coef, int64_t n, SelSt
   const int64_t c0 = nchunk * bidx / nblk;
   const int64_t c1 = nchunk * (bidx + 1) / nblk;
   uint32_t nb = 0, mn = 0xffffffffu, mx = 0, mk = 0, nz = 0;
-  float v[SEL_ITEMS];
-  if (c0 < c1) sel_load_chunk(coef, n, c0, tid, v);
   for (int64_t c = c0; c < c1; c++) {
-    float nv[SEL_ITEMS];
-    if (c + 1 < c1) sel_load_chunk(coef, n, c + 1, tid, nv);
+    float v[SEL_ITEMS];
+    sel_load_chunk(coef, n, c, tid, v);
 #pragma unroll
     for (int u = 0; u < SEL_ITEMS; u++) {
       const uint32_t bits = __builtin_bit_cast(uint32_t, v[u]);
@@ -407,10 +423,6 @@ __device__ void sel_collect_dev(const float *__restrict__ coef, int64_t n, SelSt
         wcnt += (uint32_t)__popcll(m);
         if (wcnt > SEL_WAVE_CAP - 64) sel_flush(buf, wcnt, S, cand, cap, lane);
       }
-    }
-    if (c + 1 < c1) {
-#pragma unroll
-      for (int u = 0; u < SEL_ITEMS; u++) v[u] = nv[u];
     }
   }
   if (wcnt) sel_flush(buf, wcnt, S, cand, cap, lane);
@@ -439,24 +451,6 @@ __device__ void sel_collect_dev(const float *__restrict__ coef, int64_t n, SelSt
     S->part[bidx][3] = mk;
     S->part[bidx][4] = nz;
   }
-}
-
-// Is this block the last of its launch to finish?  Every block's global
-// writes are fenced before it counts itself; the last one resets the
-// counter and fences again before it reads what the others wrote.
-__device__ inline bool last_block_done(uint32_t *ctr) {
-  __shared__ uint32_t last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    last = atomicAdd(ctr, 1u) == gridDim.x - 1;
-    if (last) {
-      *ctr = 0;
-      __threadfence();
-    }
-  }
-  __syncthreads();
-  return last != 0;
 }
 
 // ---- 3. verify -------------------------------------------------------------
@@ -511,12 +505,9 @@ __device__ void sel_verify_dev(SelState *S, int nparts) {
   }
 }
 
-// the plain path's collect: the last block to finish reduces the partials
-// and checks the window (the verify)
 __global__ void __launch_bounds__(WV_BLOCK)
 sel_collect(const float *__restrict__ coef, int64_t n, SelState *S, uint32_t *cand) {
   sel_collect_dev(coef, n, S, cand, 0xffffffffu, blockIdx.x, gridDim.x);
-  if (last_block_done(&S->ctr)) sel_verify_dev(S, gridDim.x);
 }
 
 // ---- 4. radix select -----------------------------------------------------------
@@ -564,91 +555,149 @@ __device__ void sel_result_dev(int64_t n, const SelState *S, wv_thresh_result *r
   *res = o;
 }
 
-// One round over SEL_RGRID blocks: LDS histograms of the digit, merged
-// into S->hist2 with one atomic per non-empty bin; the last block picks the
-// rank's bucket.  In the last round that block then finishes whatever bits
-// are left on its own (none after three rounds) and writes the result.
-// (With the exact fallback the keys are all n coefficients and the window's
-// zeros are not counted apart.)
-__global__ void __launch_bounds__(WV_BLOCK)
-sel_round(const uint32_t *cand, const float *coef, int64_t n, SelState *S,
-          wv_thresh_result *res, int last_round) {
-  __shared__ uint32_t h[WV_H1_BINS];
+// The plan every block of a round derives on its own, from what earlier
+// launches left in the state: the verify (the partials reduced, the window
+// checked) and the buckets of the rounds already run.  It is deterministic,
+// so every block agrees, and no launch needs a grid-wide synchronisation
+// (a device-scope release fence in every block costs an L2 write-back).
+struct SelPlan {
+  uint32_t below, min_ord, max_ord, max_key, zwin, crank, base, bits;
+  int fallback;
+};
+
+__device__ void sel_plan(const SelState *S, int nparts, int rounds_done, SelPlan *pl) {
+  __shared__ uint32_t r[5][16];
   __shared__ uint32_t rb, rr;
-  const bool fb = S->fallback != 0;
-  const uint32_t *src = fb ? (const uint32_t *)coef : cand;
-  const int64_t cnt = fb ? n : (int64_t)S->ncand;
-  const uint32_t zwin = fb ? 0u : S->zwin;
-  if (!S->special && S->bits > 0) {
-    for (int i = threadIdx.x; i < WV_H1_BINS; i += WV_BLOCK) h[i] = 0;
-    __syncthreads();
-    const uint32_t base = S->base, bits = S->bits;
-    const int64_t per = (cnt + gridDim.x - 1) / gridDim.x;
-    const int64_t a = (int64_t)blockIdx.x * per, e = min(cnt, a + per);
-    for (int64_t i0 = a; i0 < e; i0 += WV_BLOCK * 4) {
-      uint32_t kk[4];
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int64_t i = i0 + u * WV_BLOCK + threadIdx.x;
-        kk[u] = i < e ? src[i] & 0x7fffffffu : 0xffffffffu;
-      }
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const uint32_t key = kk[u];
-        if (key != 0xffffffffu && in_range(key, base, bits))
-          atomicAdd(&h[round_digit(key, base, bits)], 1u);
-      }
-    }
-    // the listed keys exclude the window's zeros: block 0 adds them
-    if (threadIdx.x == 0 && blockIdx.x == 0 && zwin && in_range(0u, base, bits))
-      atomicAdd(&h[round_digit(0u, base, bits)], zwin);
-    __syncthreads();
-    for (int i = threadIdx.x; i < WV_H1_BINS; i += WV_BLOCK)
-      if (h[i]) atomicAdd(&S->hist2[i], h[i]);
+  const int t = threadIdx.x, nw = blockDim.x >> 6;
+  uint32_t nb = 0, mn = 0xffffffffu, mx = 0, mk = 0, nz = 0;
+  for (int i = t; i < nparts; i += blockDim.x) {
+    nb += S->part[i][0];
+    mn = min(mn, S->part[i][1]);
+    mx = max(mx, S->part[i][2]);
+    mk = max(mk, S->part[i][3]);
+    nz += S->part[i][4];
   }
-  if (!last_block_done(&S->ctr)) return;
-  if (!S->special && S->bits > 0) {
-    if (threadIdx.x == 0) { rb = 0; rr = 0; }
+  for (int off = 32; off > 0; off >>= 1) {
+    nb += (uint32_t)__shfl_xor((int)nb, off);
+    mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
+    mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
+    mk = max(mk, (uint32_t)__shfl_xor((int)mk, off));
+    nz += (uint32_t)__shfl_xor((int)nz, off);
+  }
+  if ((t & 63) == 0) {
+    r[0][t >> 6] = nb; r[1][t >> 6] = mn; r[2][t >> 6] = mx; r[3][t >> 6] = mk; r[4][t >> 6] = nz;
+  }
+  __syncthreads();
+  if (t == 0) {
+    for (int w = 1; w < nw; w++) {
+      nb += r[0][w];
+      mn = min(mn, r[1][w]);
+      mx = max(mx, r[2][w]);
+      mk = max(mk, r[3][w]);
+      nz += r[4][w];
+    }
+    pl->below = nb;
+    pl->min_ord = mn;
+    pl->max_ord = mx;
+    pl->max_key = mk;
+    // the zeros are in the window when it starts at 0 (else they are below)
+    pl->zwin = S->key_lo == 0 ? nz : 0u;
+    const uint32_t idx = (uint32_t)S->idx;
+    if (S->special) {
+      pl->fallback = 0;
+      pl->crank = 0;
+      pl->base = S->base;
+      pl->bits = 0;
+    } else if (!S->overflow && nb <= idx && idx < nb + S->ncand + pl->zwin) {
+      pl->fallback = 0;
+      pl->crank = idx - nb;
+      pl->base = S->base;
+      pl->bits = S->bits;
+    } else {
+      // exact fallback: the rank among all keys, from the full key range
+      pl->fallback = 1;
+      pl->zwin = 0;
+      pl->crank = idx;
+      pl->base = 0;
+      pl->bits = 31;
+    }
+  }
+  __syncthreads();
+  for (int k = 0; k < rounds_done; k++) {
+    if (pl->bits == 0) break;
+    if (t == 0) { rb = 0; rr = 0; }
     __syncthreads();
-    find_bucket_blk<WV_H1_BINS>(S->hist2, S->crank, &rb, &rr);
-    for (int i = threadIdx.x; i < WV_H1_BINS; i += WV_BLOCK) S->hist2[i] = 0;
-    if (threadIdx.x == 0) {
-      const uint32_t bits = S->bits;
+    find_bucket_blk<WV_H1_BINS>(S->rhist[k], pl->crank, &rb, &rr);
+    if (t == 0) {
+      const uint32_t bits = pl->bits;
       const uint32_t d = bits < 11u ? bits : 11u;
-      S->base += rb << (bits - d);
-      S->bits = bits - d;
-      S->crank = rr;
+      pl->base += rb << (bits - d);
+      pl->bits = bits - d;
+      pl->crank = rr;
     }
     __syncthreads();
   }
-  if (!last_round) return;
-  if (!S->special) {
-    uint32_t rank = S->crank, base = S->base, bits = S->bits;
-    while (bits > 0) {
-      for (int i = threadIdx.x; i < WV_H1_BINS; i += WV_BLOCK) h[i] = 0;
-      if (threadIdx.x == 0) { rb = 0; rr = 0; }
-      __syncthreads();
-      if (threadIdx.x == 0 && zwin && in_range(0u, base, bits))
-        atomicAdd(&h[round_digit(0u, base, bits)], zwin);
-      for (int64_t i = threadIdx.x; i < cnt; i += WV_BLOCK) {
-        const uint32_t key = src[i] & 0x7fffffffu;
-        if (in_range(key, base, bits)) atomicAdd(&h[round_digit(key, base, bits)], 1u);
-      }
-      __syncthreads();
-      find_bucket_blk<WV_H1_BINS>(h, rank, &rb, &rr);
-      const uint32_t d = bits < 11u ? bits : 11u;
-      base += rb << (bits - d);
-      bits -= d;
-      rank = rr;
-      __syncthreads();
+}
+
+// Round k over SEL_RGRID blocks: the next min(bits, 11) bits of (key - base)
+// of the keys in [base, base + 2^bits) -- the candidates, or after an exact
+// fallback all n coefficients -- histogrammed in LDS and merged into
+// S->rhist[k] with one atomic per non-empty bin.  A round with no bits left
+// returns at once.
+__global__ void __launch_bounds__(WV_BLOCK)
+sel_round(const uint32_t *cand, const float *coef, int64_t n, SelState *S, int nparts,
+          int round) {
+  __shared__ uint32_t h[WV_H1_BINS];
+  __shared__ SelPlan pl;
+  sel_plan(S, nparts, round, &pl);
+  if (S->special || pl.bits == 0) return;
+  const uint32_t *src = pl.fallback ? (const uint32_t *)coef : cand;
+  const int64_t cnt = pl.fallback ? n : (int64_t)S->ncand;
+  for (int i = threadIdx.x; i < WV_H1_BINS; i += WV_BLOCK) h[i] = 0;
+  __syncthreads();
+  const uint32_t base = pl.base, bits = pl.bits;
+  const int64_t per = (cnt + gridDim.x - 1) / gridDim.x;
+  const int64_t a = (int64_t)blockIdx.x * per, e = min(cnt, a + per);
+  for (int64_t i0 = a; i0 < e; i0 += WV_BLOCK * 4) {
+    uint32_t kk[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t i = i0 + u * WV_BLOCK + threadIdx.x;
+      kk[u] = i < e ? src[i] & 0x7fffffffu : 0xffffffffu;
     }
-    if (threadIdx.x == 0) {
-      S->base = base;
-      S->bits = 0;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const uint32_t key = kk[u];
+      if (key != 0xffffffffu && in_range(key, base, bits))
+        atomicAdd(&h[round_digit(key, base, bits)], 1u);
     }
-    __syncthreads();
   }
-  if (threadIdx.x == 0) sel_result_dev(n, S, res);
+  // the listed keys exclude the window's zeros: block 0 adds them
+  if (threadIdx.x == 0 && blockIdx.x == 0 && pl.zwin && in_range(0u, base, bits))
+    atomicAdd(&h[round_digit(0u, base, bits)], pl.zwin);
+  __syncthreads();
+  for (int i = threadIdx.x; i < WV_H1_BINS; i += WV_BLOCK)
+    if (h[i]) atomicAdd(&S->rhist[round][i], h[i]);
+}
+
+// one block: the plan after the three rounds -> the state and the result
+__global__ void __launch_bounds__(WV_BLOCK)
+sel_finish(int64_t n, SelState *S, int nparts, wv_thresh_result *res) {
+  __shared__ SelPlan pl;
+  sel_plan(S, nparts, 3, &pl);
+  if (threadIdx.x == 0) {
+    S->below = pl.below;
+    S->min_ord = pl.min_ord;
+    S->max_ord = pl.max_ord;
+    S->max_key = pl.max_key;
+    S->zwin = pl.zwin;
+    S->fallback = pl.fallback;
+    if (pl.fallback) S->ncand = 0;
+    S->crank = pl.crank;
+    S->base = pl.base;
+    S->bits = pl.bits;
+    sel_result_dev(n, S, res);
+  }
 }
 
 hipError_t threshold_select(const float *coef, int64_t n, float frac,
@@ -657,15 +706,17 @@ hipError_t threshold_select(const float *coef, int64_t n, float frac,
   (void)stats;
   SelState *S = (SelState *)ws;
   uint32_t *cand = (uint32_t *)((char *)ws + ((sizeof(SelState) + 255) & ~(size_t)255));
+  if (n > SEL_SAMPLE)
+    hipLaunchKernelGGL(sel_gather, dim3(SEL_SAMPLE / WV_BLOCK), dim3(WV_BLOCK), 0, s, coef, n, S);
   hipLaunchKernelGGL(sel_sample, dim3(1), dim3(1024), 0, s, coef, n, frac, S);
   const int64_t nchunk = (n + SEL_CHUNK - 1) / SEL_CHUNK;
   const int grid = (int)(nchunk < SEL_GRID ? nchunk : SEL_GRID);
   hipLaunchKernelGGL(sel_collect, dim3(grid), dim3(WV_BLOCK), 0, s, coef, n, S, cand);
-  // three rounds cover any window (11 + 11 + 11 >= 31 bits); a round with
-  // no bits left returns at once
+  // three rounds cover any window (11 + 11 + 11 >= 31 bits)
   for (int r = 0; r < 3; r++)
-    hipLaunchKernelGGL(sel_round, dim3(SEL_RGRID), dim3(WV_BLOCK), 0, s, cand, coef, n, S, res,
-                       r == 2 ? 1 : 0);
+    hipLaunchKernelGGL(sel_round, dim3(SEL_RGRID), dim3(WV_BLOCK), 0, s, cand, coef, n, S, grid,
+                       r);
+  hipLaunchKernelGGL(sel_finish, dim3(1), dim3(WV_BLOCK), 0, s, n, S, grid, res);
   return hipGetLastError();
 }
 
@@ -683,7 +734,7 @@ __device__ inline int find_job(const int *blk0, int njobs, int b) {
 
 __global__ void __launch_bounds__(1024) bsel_sample(const BSelJob *J) {
   const BSelJob &j = J[blockIdx.x];
-  sel_sample_dev(j.coef, j.n, j.frac, (SelState *)j.state);
+  sel_sample_dev(j.coef, j.n, j.frac, (SelState *)j.state, nullptr);
 }
 
 __global__ void __launch_bounds__(WV_BLOCK)
