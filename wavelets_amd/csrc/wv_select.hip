@@ -67,7 +67,8 @@ struct SelState {
   uint32_t pad;
   uint32_t rhist[3][WV_H1_BINS];  // the radix rounds' digit histograms
   uint32_t skeys[SEL_SAMPLE];     // the sampled keys (plain path: gathered by many blocks)
-  uint32_t part[SEL_GRID][5];   // per collect block: below, min, max, max key, zeros
+  // per collect block: below, min, max, max key, zeros, candidates
+  uint32_t part[SEL_GRID][6];
 };
 
 int64_t threshold_ws_bytes(int64_t n) {
@@ -367,19 +368,32 @@ __device__ inline void sel_load_chunk(const float *__restrict__ coef, int64_t n,
   }
 }
 
-// a wave's buffered candidates to the global list (wave-uniform call)
+// A wave's buffered candidates to the candidate list (wave-uniform call).
+// SEG (one select per launch): the block's own segment of the list, which
+// starts at the block's first sample and cannot overflow (a block has no
+// more candidates than samples); the offset comes from a block counter in
+// LDS, so the collect takes no global atomic.  Otherwise (batches: many
+// selects share the launch, each list holding cap keys) one global atomic
+// per flush reserves the space.
+template <bool SEG>
 __device__ inline void sel_flush(uint32_t *buf, uint32_t &wcnt, SelState *S, uint32_t *cand,
-                                 uint32_t cap, int lane) {
+                                 uint32_t cap, int lane, uint32_t *seg_cnt, int64_t seg0) {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   uint32_t base = 0;
   if (lane == 0) {
-    base = atomicAdd(&S->ncand, wcnt);
-    if (base + wcnt > cap) S->overflow = 1;
+    if (SEG) {
+      base = atomicAdd(seg_cnt, wcnt);
+    } else {
+      base = atomicAdd(&S->ncand, wcnt);
+      if (base + wcnt > cap) S->overflow = 1;
+    }
   }
   base = __shfl(base, 0);
-  if (base + wcnt <= cap)
+  if (SEG)
+    for (uint32_t i = lane; i < wcnt; i += 64) gptr(cand)[seg0 + base + i] = buf[i];
+  else if (base + wcnt <= cap)
     for (uint32_t i = lane; i < wcnt; i += 64) gptr(cand)[base + i] = buf[i];
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -387,11 +401,13 @@ __device__ inline void sel_flush(uint32_t *buf, uint32_t &wcnt, SelState *S, uin
   wcnt = 0;
 }
 
+template <bool SEG>
 __device__ __attribute__((always_inline)) void
 sel_collect_dev(const float *__restrict__ coef, int64_t n, SelState *S, uint32_t *cand,
                 uint32_t cap, int bidx, int nblk) {
   __shared__ uint32_t wbuf[WV_BLOCK / 64][SEL_WAVE_CAP];
   __shared__ uint32_t red[5][WV_BLOCK / 64];
+  __shared__ uint32_t seg_cnt;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   uint32_t *buf = wbuf[wave];
   uint32_t wcnt = 0;   // wave-uniform
@@ -399,6 +415,11 @@ sel_collect_dev(const float *__restrict__ coef, int64_t n, SelState *S, uint32_t
   const int64_t nchunk = (n + SEL_CHUNK - 1) / SEL_CHUNK;
   const int64_t c0 = nchunk * bidx / nblk;
   const int64_t c1 = nchunk * (bidx + 1) / nblk;
+  const int64_t seg0 = c0 * SEL_CHUNK;
+  if (SEG) {
+    if (tid == 0) seg_cnt = 0;
+    __syncthreads();
+  }
   uint32_t nb = 0, mn = 0xffffffffu, mx = 0, mk = 0, nz = 0;
   for (int64_t c = c0; c < c1; c++) {
     float v[SEL_ITEMS];
@@ -421,11 +442,12 @@ sel_collect_dev(const float *__restrict__ coef, int64_t n, SelState *S, uint32_t
       if (m) {
         if (hit) buf[wcnt + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = key;
         wcnt += (uint32_t)__popcll(m);
-        if (wcnt > SEL_WAVE_CAP - 64) sel_flush(buf, wcnt, S, cand, cap, lane);
+        if (wcnt > SEL_WAVE_CAP - 64)
+          sel_flush<SEG>(buf, wcnt, S, cand, cap, lane, &seg_cnt, seg0);
       }
     }
   }
-  if (wcnt) sel_flush(buf, wcnt, S, cand, cap, lane);
+  if (wcnt) sel_flush<SEG>(buf, wcnt, S, cand, cap, lane, &seg_cnt, seg0);
   for (int off = 32; off > 0; off >>= 1) {
     nb += (uint32_t)__shfl_xor((int)nb, off);
     mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
@@ -450,6 +472,7 @@ sel_collect_dev(const float *__restrict__ coef, int64_t n, SelState *S, uint32_t
     S->part[bidx][2] = mx;
     S->part[bidx][3] = mk;
     S->part[bidx][4] = nz;
+    S->part[bidx][5] = SEG ? seg_cnt : 0u;
   }
 }
 
@@ -507,7 +530,7 @@ __device__ void sel_verify_dev(SelState *S, int nparts) {
 
 __global__ void __launch_bounds__(WV_BLOCK)
 sel_collect(const float *__restrict__ coef, int64_t n, SelState *S, uint32_t *cand) {
-  sel_collect_dev(coef, n, S, cand, 0xffffffffu, blockIdx.x, gridDim.x);
+  sel_collect_dev<true>(coef, n, S, cand, 0xffffffffu, blockIdx.x, gridDim.x);
 }
 
 // ---- 4. radix select -----------------------------------------------------------
@@ -561,21 +584,22 @@ __device__ void sel_result_dev(int64_t n, const SelState *S, wv_thresh_result *r
 // so every block agrees, and no launch needs a grid-wide synchronisation
 // (a device-scope release fence in every block costs an L2 write-back).
 struct SelPlan {
-  uint32_t below, min_ord, max_ord, max_key, zwin, crank, base, bits;
+  uint32_t below, min_ord, max_ord, max_key, zwin, crank, base, bits, ncand;
   int fallback;
 };
 
 __device__ void sel_plan(const SelState *S, int nparts, int rounds_done, SelPlan *pl) {
-  __shared__ uint32_t r[5][16];
+  __shared__ uint32_t r[6][16];
   __shared__ uint32_t rb, rr;
   const int t = threadIdx.x, nw = blockDim.x >> 6;
-  uint32_t nb = 0, mn = 0xffffffffu, mx = 0, mk = 0, nz = 0;
+  uint32_t nb = 0, mn = 0xffffffffu, mx = 0, mk = 0, nz = 0, nc = 0;
   for (int i = t; i < nparts; i += blockDim.x) {
     nb += S->part[i][0];
     mn = min(mn, S->part[i][1]);
     mx = max(mx, S->part[i][2]);
     mk = max(mk, S->part[i][3]);
     nz += S->part[i][4];
+    nc += S->part[i][5];
   }
   for (int off = 32; off > 0; off >>= 1) {
     nb += (uint32_t)__shfl_xor((int)nb, off);
@@ -583,9 +607,11 @@ __device__ void sel_plan(const SelState *S, int nparts, int rounds_done, SelPlan
     mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
     mk = max(mk, (uint32_t)__shfl_xor((int)mk, off));
     nz += (uint32_t)__shfl_xor((int)nz, off);
+    nc += (uint32_t)__shfl_xor((int)nc, off);
   }
   if ((t & 63) == 0) {
     r[0][t >> 6] = nb; r[1][t >> 6] = mn; r[2][t >> 6] = mx; r[3][t >> 6] = mk; r[4][t >> 6] = nz;
+    r[5][t >> 6] = nc;
   }
   __syncthreads();
   if (t == 0) {
@@ -595,7 +621,9 @@ __device__ void sel_plan(const SelState *S, int nparts, int rounds_done, SelPlan
       mx = max(mx, r[2][w]);
       mk = max(mk, r[3][w]);
       nz += r[4][w];
+      nc += r[5][w];
     }
+    pl->ncand = nc;
     pl->below = nb;
     pl->min_ord = mn;
     pl->max_ord = mx;
@@ -608,7 +636,7 @@ __device__ void sel_plan(const SelState *S, int nparts, int rounds_done, SelPlan
       pl->crank = 0;
       pl->base = S->base;
       pl->bits = 0;
-    } else if (!S->overflow && nb <= idx && idx < nb + S->ncand + pl->zwin) {
+    } else if (nb <= idx && idx < nb + nc + pl->zwin) {
       pl->fallback = 0;
       pl->crank = idx - nb;
       pl->base = S->base;
@@ -644,6 +672,26 @@ __device__ void sel_plan(const SelState *S, int nparts, int rounds_done, SelPlan
 // fallback all n coefficients -- histogrammed in LDS and merged into
 // S->rhist[k] with one atomic per non-empty bin.  A round with no bits left
 // returns at once.
+// histogram of the keys src[a, e) in [base, base + 2^bits) into LDS h
+__device__ inline void sel_hist_range(const uint32_t *src, int64_t a, int64_t e, uint32_t base,
+                                      uint32_t bits, uint32_t *h) {
+  const WV_GLOBAL uint32_t *g = gptr(src);
+  for (int64_t i0 = a; i0 < e; i0 += WV_BLOCK * 4) {
+    uint32_t kk[4];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int64_t i = i0 + u * WV_BLOCK + threadIdx.x;
+      kk[u] = i < e ? g[i] & 0x7fffffffu : 0xffffffffu;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const uint32_t key = kk[u];
+      if (key != 0xffffffffu && in_range(key, base, bits))
+        atomicAdd(&h[round_digit(key, base, bits)], 1u);
+    }
+  }
+}
+
 __global__ void __launch_bounds__(WV_BLOCK)
 sel_round(const uint32_t *cand, const float *coef, int64_t n, SelState *S, int nparts,
           int round) {
@@ -651,25 +699,20 @@ sel_round(const uint32_t *cand, const float *coef, int64_t n, SelState *S, int n
   __shared__ SelPlan pl;
   sel_plan(S, nparts, round, &pl);
   if (S->special || pl.bits == 0) return;
-  const uint32_t *src = pl.fallback ? (const uint32_t *)coef : cand;
-  const int64_t cnt = pl.fallback ? n : (int64_t)S->ncand;
   for (int i = threadIdx.x; i < WV_H1_BINS; i += WV_BLOCK) h[i] = 0;
   __syncthreads();
   const uint32_t base = pl.base, bits = pl.bits;
-  const int64_t per = (cnt + gridDim.x - 1) / gridDim.x;
-  const int64_t a = (int64_t)blockIdx.x * per, e = min(cnt, a + per);
-  for (int64_t i0 = a; i0 < e; i0 += WV_BLOCK * 4) {
-    uint32_t kk[4];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const int64_t i = i0 + u * WV_BLOCK + threadIdx.x;
-      kk[u] = i < e ? src[i] & 0x7fffffffu : 0xffffffffu;
-    }
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const uint32_t key = kk[u];
-      if (key != 0xffffffffu && in_range(key, base, bits))
-        atomicAdd(&h[round_digit(key, base, bits)], 1u);
+  if (pl.fallback) {
+    // every coefficient, split evenly over the blocks
+    const int64_t per = (n + gridDim.x - 1) / gridDim.x;
+    const int64_t a = (int64_t)blockIdx.x * per;
+    sel_hist_range((const uint32_t *)coef, a, min(n, a + per), base, bits, h);
+  } else {
+    // the collect blocks' segments, a share of them per block
+    const int64_t nchunk = (n + SEL_CHUNK - 1) / SEL_CHUNK;
+    for (int sg = blockIdx.x; sg < nparts; sg += gridDim.x) {
+      const int64_t a = nchunk * sg / nparts * SEL_CHUNK;
+      sel_hist_range(cand, a, a + S->part[sg][5], base, bits, h);
     }
   }
   // the listed keys exclude the window's zeros: block 0 adds them
@@ -692,7 +735,7 @@ sel_finish(int64_t n, SelState *S, int nparts, wv_thresh_result *res) {
     S->max_key = pl.max_key;
     S->zwin = pl.zwin;
     S->fallback = pl.fallback;
-    if (pl.fallback) S->ncand = 0;
+    S->ncand = pl.fallback ? 0u : pl.ncand;
     S->crank = pl.crank;
     S->base = pl.base;
     S->bits = pl.bits;
@@ -741,7 +784,7 @@ __global__ void __launch_bounds__(WV_BLOCK)
 bsel_collect(const BSelJob *J, const int *blk0, int njobs) {
   const int jb = find_job(blk0, njobs, blockIdx.x);
   const BSelJob &j = J[jb];
-  sel_collect_dev(j.coef, j.n, (SelState *)j.state, j.cand, j.cap,
+  sel_collect_dev<false>(j.coef, j.n, (SelState *)j.state, j.cand, j.cap,
                   (int)blockIdx.x - j.blk0, j.nblk);
 }
 
