@@ -266,11 +266,119 @@ __global__ void ll_clear_runs(LloydState *S, int32_t *nruns, int32_t *first,
 // one wavefront per chain: chains 0..cs-1 are the cells, chain cs the
 // signed distortion (only when DIST).  The float sums must follow the
 // points' order exactly (lloyds.cpp:56-64,124-132), so each chain is one
-// dependent add per point.  The loop is wave-uniform: every lane walks the
-// same indices, so the loads are scalar (read through the scalar cache, 16
-// points per batch, issued ahead of the adds) and a step costs one
-// dependent VALU add instead of a cross-lane shuffle.
+// dependent add per point.
+//
+// The chain is carried across the lanes: a batch of 64 consecutive points
+// sits one per lane (a coalesced vector load), and each step is one
+// v_add_f32_dpp with wave_ror:1, which hands lane k-1's running sum to lane
+// k and adds lane k's point.  After step k of a batch lane k holds the sum
+// through the batch's point k; step 0 of the next batch takes lane 63's.
+// Every lane executes every step, but only lane k's result of step k is
+// ever read again, so the sum is exactly the sequential one.  Lanes outside
+// the chain's range add -0.0f, the identity of IEEE addition.  Loads run
+// LL_CU batches ahead of the adds.
+#define LL_CU 8
+
 __device__ inline int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ float chain64(float acc, float t) {
+#pragma unroll
+  for (int k = 0; k < 64; k++) {
+    const int r = __builtin_amdgcn_update_dpp(0, __float_as_int(acc), 0x13C /* wave_ror:1 */,
+                                              0xF, 0xF, false);
+    acc = __int_as_float(r) + t;
+  }
+  return acc;
+}
+
+__device__ __forceinline__ float chain_result(float acc) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(acc), 63));
+}
+
+// s = 0; s += pts[i] for i in [a, b), a < b.  Two register sets take
+// turns: the loads of the next LL_CU batches are in flight while the
+// current ones are added.  Load indices are clamped into [a, b), so every
+// load is in bounds (also the one issued past the end, whose values are
+// never used) and the prefetch needs no branch; the lanes outside the
+// range add -0.0f.
+__device__ __forceinline__ float run_chain(const float *__restrict__ pts, int a, int b) {
+  const int lane = threadIdx.x;
+  constexpr int G = 64 * LL_CU;
+  float acc = 0.f;
+  float pA[LL_CU], pB[LL_CU];
+  auto load = [&](float *pp, int j) {
+#pragma unroll
+    for (int u = 0; u < LL_CU; u++) pp[u] = pts[max(a, min(j + u * 64 + lane, b - 1))];
+  };
+  auto run = [&](const float *pp, int j) {
+#pragma unroll
+    for (int u = 0; u < LL_CU; u++) {
+      const int i = j + u * 64 + lane;
+      if (j + u * 64 < b) acc = chain64(acc, (i >= a && i < b) ? pp[u] : -0.f);
+    }
+  };
+  int j = a & ~63;
+  load(pA, j);
+  while (true) {
+    load(pB, j + G);
+    run(pA, j);
+    j += G;
+    if (j >= b) break;
+    load(pA, j + G);
+    run(pB, j);
+    j += G;
+    if (j >= b) break;
+  }
+  return chain_result(acc);
+}
+
+// d = 0; d += cb[g[i]] - pts[i] for i in [0, psize), psize >= 1, with the
+// codebook in LDS (lcb): its reads count on their own counter, so they do
+// not wait for the next batches' loads in flight.
+template <typename G>
+__device__ __forceinline__ float dist_chain(const float *__restrict__ pts, const G *__restrict__ g,
+                                            const float *lcb, int psize) {
+  const int lane = threadIdx.x;
+  constexpr int GR = 64 * LL_CU;
+  float acc = 0.f;
+  float pA[LL_CU], pB[LL_CU];
+  int qA[LL_CU], qB[LL_CU];
+  auto load = [&](float *pp, int *qq, int j) {
+#pragma unroll
+    for (int u = 0; u < LL_CU; u++) {
+      const int i = min(j + u * 64 + lane, psize - 1);
+      pp[u] = pts[i];
+      qq[u] = (int)g[i];
+    }
+  };
+  auto run = [&](const float *pp, const int *qq, int j) {
+    float t[LL_CU];
+#pragma unroll
+    for (int u = 0; u < LL_CU; u++) t[u] = lcb[qq[u]] - pp[u];
+#pragma unroll
+    for (int u = 0; u < LL_CU; u++)
+      if (j + u * 64 < psize) acc = chain64(acc, j + u * 64 + lane < psize ? t[u] : -0.f);
+  };
+  int j = 0;
+  load(pA, qA, 0);
+  while (true) {
+    load(pB, qB, j + GR);
+    run(pA, qA, j);
+    j += GR;
+    if (j >= psize) break;
+    load(pA, qA, j + GR);
+    run(pB, qB, j);
+    j += GR;
+    if (j >= psize) break;
+  }
+  return chain_result(acc);
+}
+
+// the codebook (cs floats) into LDS, for a one-wave block
+__device__ __forceinline__ void cb_to_lds(float *lcb, const float *cb, int cs) {
+  for (int k = threadIdx.x; k < cs; k += 64) lcb[k] = cb[k];
+  __syncthreads();
+}
 
 template <bool DIST>
 __global__ void __launch_bounds__(64)
@@ -289,16 +397,10 @@ ll_chains(const float *__restrict__ pts, const int32_t *__restrict__ grp,
     if (nr == 1) {
       const int a = uni(first[chain]), b = uni(last[chain] + 1);
       c = (uint32_t)(b - a);
-      int i = a;
-      for (; i + 16 <= b; i += 16) {
-        float v[16];
-#pragma unroll
-        for (int k = 0; k < 16; k++) v[k] = pts[i + k];
-#pragma unroll
-        for (int k = 0; k < 16; k++) s += v[k];
-      }
-      for (; i < b; i++) s += pts[i];
+      s = run_chain(pts, a, b);
     } else if (nr > 1) {
+      // members scattered over several runs: scan every point in order
+      // (wave-uniform scalar loads, one dependent add per member)
       int i = 0;
       for (; i + 16 <= psize; i += 16) {
         float v[16];
@@ -315,16 +417,9 @@ ll_chains(const float *__restrict__ pts, const int32_t *__restrict__ grp,
     if (threadIdx.x == 0) { sum[chain] = s; count[chain] = c; }
   } else if (DIST) {
     // dist += codebook[idx] - p over all points in order (lloyds.cpp:64,132)
-    float d = 0.f;
-    int i = 0;
-    for (; i + 16 <= psize; i += 16) {
-      float t[16];
-#pragma unroll
-      for (int k = 0; k < 16; k++) t[k] = cb[grp[i + k]] - pts[i + k];
-#pragma unroll
-      for (int k = 0; k < 16; k++) d += t[k];
-    }
-    for (; i < psize; i++) d += cb[grp[i]] - pts[i];
+    extern __shared__ float lcb[];
+    cb_to_lds(lcb, cb, cs);
+    const float d = dist_chain(pts, grp, lcb, psize);
     if (threadIdx.x == 0) *dist_out = d;
   }
 }
@@ -393,19 +488,9 @@ ll_window_dist(const float *__restrict__ pts, const uint16_t *__restrict__ ring,
                const LloydState *S, float *dsum) {
   if (S->done) return;
   const int w = blockIdx.x;
-  const uint16_t *g = ring + (int64_t)w * ring_stride;
-  const float *cb = cbs + (int64_t)w * cb_stride;
-  const int psize = uni(S->psize);
-  float d = 0.f;
-  int i = 0;
-  for (; i + 16 <= psize; i += 16) {
-    float t[16];
-#pragma unroll
-    for (int k = 0; k < 16; k++) t[k] = cb[g[i + k]] - pts[i + k];
-#pragma unroll
-    for (int k = 0; k < 16; k++) d += t[k];
-  }
-  for (; i < psize; i++) d += cb[g[i]] - pts[i];
+  extern __shared__ float lcb[];
+  cb_to_lds(lcb, cbs + (int64_t)w * cb_stride, uni(S->cs));
+  const float d = dist_chain(pts, ring + (int64_t)w * ring_stride, lcb, uni(S->psize));
   if (threadIdx.x == 0) dsum[w] = d;
 }
 
@@ -589,7 +674,7 @@ extern "C" int wv_lloyd(const float *coef_d, int64_t n, const wv_thresh_result *
     hipLaunchKernelGGL(ll_clear_runs, dim3(1), dim3(256), 0, s, S, nruns, first, last);
     hipLaunchKernelGGL(ll_runs, dim3(ag), dim3(WV_BLOCK), 0, s, S, grp, first, last, nruns);
     if (dist)
-      hipLaunchKernelGGL(ll_chains<true>, dim3(cs + 1), dim3(64), 0, s, pts, grp, cb, S, first,
+      hipLaunchKernelGGL(ll_chains<true>, dim3(cs + 1), dim3(64), cb_lds, s, pts, grp, cb, S, first,
                          last, nruns, sum, count, dsum);
     else
       hipLaunchKernelGGL(ll_chains<false>, dim3(cs), dim3(64), 0, s, pts, grp, cb, S, first,
@@ -614,7 +699,7 @@ extern "C" int wv_lloyd(const float *coef_d, int64_t n, const wv_thresh_result *
       assign_and_chains(true, false, ring + (int64_t)k * L.ring_stride);
       hipLaunchKernelGGL(ll_snapshot, dim3(1), dim3(256), 0, s, S, cb, cbring + (int64_t)k * (cs + 1));
     }
-    hipLaunchKernelGGL(ll_window_dist, dim3(nw), dim3(64), 0, s, pts, ring, L.ring_stride, cbring,
+    hipLaunchKernelGGL(ll_window_dist, dim3(nw), dim3(64), cb_lds, s, pts, ring, L.ring_stride, cbring,
                        cs + 1, S, wdsum);
     hipLaunchKernelGGL(ll_window_eval, dim3(1), dim3(1), 0, s, S, wdsum, nw);
     enqueued += nw;
