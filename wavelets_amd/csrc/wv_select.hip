@@ -27,14 +27,15 @@
 //   radix    three rounds over the candidates, relative to the window:
 //            each splits [base, base + 2^bits) into up to 2048 buckets (LDS
 //            histograms merged with one atomic per bin).  Every block of a
-//            round first derives the plan itself from the state -- the
-//            partials reduced and the window verified (the rank falls inside
-//            it when below <= idx < below + candidates, else the exact
-//            fallback takes every key from the full 31-bit range), then the
-//            buckets of the earlier rounds -- so no launch needs a grid-wide
-//            synchronisation.  A window of 20-24 bits takes two or three
-//            rounds, the fallback's 31 bits three; a round with no bits
-//            left returns at once;
+//            round first derives the plan itself from the state, so no
+//            launch needs a grid-wide synchronisation: round 0 reduces the
+//            partials and verifies the window (the rank falls inside it when
+//            below <= idx < below + candidates, else the exact fallback
+//            takes every key from the full 31-bit range); a later round
+//            takes the plan the round before it ran (stored in the state by
+//            that round's block 0) and narrows it to the rank's bucket.  A
+//            window of 20-24 bits takes two or three rounds, the fallback's
+//            31 bits three; a round with no bits left returns at once;
 //   finish   one block derives the final plan and writes the result record.
 // Seven launches (gather, sample, collect, three rounds, finish);
 // everything stays on the device.
@@ -51,6 +52,13 @@ namespace wv {
 #define SEL_GRID 2048       // collect blocks (each streams n / SEL_GRID keys)
 #define SEL_WAVE_CAP 512    // LDS candidate buffer per collect wave
 #define SEL_RGRID 256       // blocks of a radix round
+#define SEL_RBLOCK 1024     // threads of a radix round block
+
+// The plan of a radix round (see sel_plan).
+struct SelPlan {
+  uint32_t below, min_ord, max_ord, max_key, zwin, crank, base, bits, ncand;
+  int fallback;
+};
 
 struct SelState {
   int32_t idx;          // target rank
@@ -66,6 +74,7 @@ struct SelState {
   uint32_t overflow;    // the candidate list outgrew its capacity
   uint32_t pad;
   uint32_t rhist[3][WV_H1_BINS];  // the radix rounds' digit histograms
+  SelPlan splan[3];               // the plan each round ran (written by its block 0)
   uint32_t skeys[SEL_SAMPLE];     // the sampled keys (plain path: gathered by many blocks)
   // per collect block: below, min, max, max key, zeros, candidates
   uint32_t part[SEL_GRID][6];
@@ -343,29 +352,25 @@ sel_sample(const float *__restrict__ coef, int64_t n, float frac, SelState *S) {
 
 // ---- 2. collect ------------------------------------------------------------
 // nblk blocks, each streaming a contiguous range in sub-chunks of SEL_CHUNK
-// keys (16-byte loads, 16 keys per thread).  Each wave appends its
+// keys (16-byte loads, 16 keys per thread; the next chunk's loads are in
+// flight while the current one is counted).  Each wave appends its
 // in-window keys to its own LDS buffer and flushes it to the candidate
-// list (one global atomic per flush) when it is nearly full, so the
-// streaming loop has no block barrier.  Per-block results go to S->part
-// (no same-address global atomics per element).
-__device__ inline void sel_load_chunk(const float *__restrict__ coef, int64_t n, int64_t c,
-                                      int tid, float *v) {
-  const int64_t start = c * SEL_CHUNK;
-  const WV_GLOBAL float *cg = gptr(coef);
-  if (start + SEL_CHUNK <= n) {
-    const WV_GLOBAL wv_f4 *c4 = (const WV_GLOBAL wv_f4 *)(cg + start);
+// list when it is nearly full, so the streaming loop has no block barrier.
+// Per-block results go to S->part (no same-address global atomics per
+// element).
+__device__ inline void sel_load_full(const float *__restrict__ coef, int64_t c, int tid,
+                                     float *v) {
+  const WV_GLOBAL wv_f4 *c4 = (const WV_GLOBAL wv_f4 *)(gptr(coef) + c * SEL_CHUNK);
 #pragma unroll
-    for (int u = 0; u < SEL_ITEMS / 4; u++) {
-      const wv_f4 q = c4[tid + u * WV_BLOCK];
-      v[4 * u] = q.x; v[4 * u + 1] = q.y; v[4 * u + 2] = q.z; v[4 * u + 3] = q.w;
-    }
-  } else {
-#pragma unroll
-    for (int u = 0; u < SEL_ITEMS; u++) {
-      const int64_t i = start + (int64_t)(u >> 2) * 4 * WV_BLOCK + 4 * tid + (u & 3);
-      v[u] = i < n ? cg[i] : __builtin_bit_cast(float, 0xffffffffu);
-    }
+  for (int u = 0; u < SEL_ITEMS / 4; u++) {
+    const wv_f4 q = c4[tid + u * WV_BLOCK];
+    v[4 * u] = q.x; v[4 * u + 1] = q.y; v[4 * u + 2] = q.z; v[4 * u + 3] = q.w;
   }
+}
+
+// the index of element u of thread tid in a chunk (the order of the loads)
+__device__ inline int64_t sel_elem_index(int64_t c, int tid, int u) {
+  return c * SEL_CHUNK + (int64_t)(u >> 2) * 4 * WV_BLOCK + 4 * tid + (u & 3);
 }
 
 // A wave's buffered candidates to the candidate list (wave-uniform call).
@@ -406,13 +411,21 @@ __device__ __attribute__((always_inline)) void
 sel_collect_dev(const float *__restrict__ coef, int64_t n, SelState *S, uint32_t *cand,
                 uint32_t cap, int bidx, int nblk) {
   __shared__ uint32_t wbuf[WV_BLOCK / 64][SEL_WAVE_CAP];
-  __shared__ uint32_t red[5][WV_BLOCK / 64];
+  __shared__ uint32_t red[3][WV_BLOCK / 64];
   __shared__ uint32_t seg_cnt;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   uint32_t *buf = wbuf[wave];
   uint32_t wcnt = 0;   // wave-uniform
   const uint32_t klo = S->key_lo, khi = S->key_hi;
-  const int64_t nchunk = (n + SEL_CHUNK - 1) / SEL_CHUNK;
+  // One count serves both the keys below the window and the zeros: keys
+  // below cut = max(klo, 1) are counted.  With klo > 0 that is the keys
+  // below the window (zeros included); with klo = 0 nothing lies below and
+  // the count is of the zeros, which the window holds but never lists.
+  const uint32_t cut = klo > 1u ? klo : 1u;
+  // listed: nonzero keys in the window, key - wlo <= span (unsigned)
+  const uint32_t wlo = khi >= cut ? cut : 0x80000000u;   // empty: no 31-bit key
+  const uint32_t span = khi >= cut ? khi - cut : 0u;
+  const int64_t nchunk = (n + SEL_CHUNK - 1) / SEL_CHUNK, nfull = n / SEL_CHUNK;
   const int64_t c0 = nchunk * bidx / nblk;
   const int64_t c1 = nchunk * (bidx + 1) / nblk;
   const int64_t seg0 = c0 * SEL_CHUNK;
@@ -420,58 +433,92 @@ sel_collect_dev(const float *__restrict__ coef, int64_t n, SelState *S, uint32_t
     if (tid == 0) seg_cnt = 0;
     __syncthreads();
   }
-  uint32_t nb = 0, mn = 0xffffffffu, mx = 0, mk = 0, nz = 0;
-  for (int64_t c = c0; c < c1; c++) {
+  uint32_t cnt = 0, mn = 0xffffffffu, mx = 0;
+  auto elem = [&](float x, bool valid) {
+    const uint32_t bits = __builtin_bit_cast(uint32_t, x);
+    const uint32_t key = bits & 0x7fffffffu;
+    if (valid) {
+      const uint32_t o = wv_f2ord(x);
+      mn = min(mn, o);
+      mx = max(mx, o);
+      cnt += key < cut;
+    }
+    const bool hit = valid && key - wlo <= span;
+    const uint64_t m = __ballot(hit);
+    if (m) {
+      if (hit) buf[wcnt + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = key;
+      wcnt += (uint32_t)__popcll(m);
+    }
+  };
+  // the buffer is checked once per four elements (at most 256 new keys),
+  // which keeps the flush code out of the per-element path
+  auto check = [&]() {
+    if (wcnt > SEL_WAVE_CAP - 256) sel_flush<SEG>(buf, wcnt, S, cand, cap, lane, &seg_cnt, seg0);
+  };
+  // full chunks, two register sets taking turns; the prefetch index is
+  // clamped to the block's last full chunk (a re-read of a chunk just read,
+  // an L2 hit), so it needs no branch
+  const int64_t cend = c1 < nfull ? c1 : nfull;
+  if (c0 < cend) {
+    float A[SEL_ITEMS], B[SEL_ITEMS];
+    sel_load_full(coef, c0, tid, A);
+    for (int64_t c = c0;;) {
+      sel_load_full(coef, c + 1 < cend ? c + 1 : cend - 1, tid, B);
+#pragma unroll
+      for (int u = 0; u < SEL_ITEMS; u++) {
+        elem(A[u], true);
+        if ((u & 3) == 3) check();
+      }
+      if (++c >= cend) break;
+      sel_load_full(coef, c + 1 < cend ? c + 1 : cend - 1, tid, A);
+#pragma unroll
+      for (int u = 0; u < SEL_ITEMS; u++) {
+        elem(B[u], true);
+        if ((u & 3) == 3) check();
+      }
+      if (++c >= cend) break;
+    }
+  }
+  if (c1 > nfull) {
+    // the partial last chunk
     float v[SEL_ITEMS];
-    sel_load_chunk(coef, n, c, tid, v);
 #pragma unroll
     for (int u = 0; u < SEL_ITEMS; u++) {
-      const uint32_t bits = __builtin_bit_cast(uint32_t, v[u]);
-      const bool ok = bits != 0xffffffffu;     // this NaN pattern marks the tail
-      const uint32_t key = bits & 0x7fffffffu;
-      if (ok) {
-        const uint32_t o = wv_f2ord(v[u]);
-        mn = min(mn, o);
-        mx = max(mx, o);
-        mk = max(mk, key);
-        nb += key < klo;
-        nz += key == 0;
-      }
-      const bool hit = ok && key >= klo && key <= khi && key != 0;
-      const uint64_t m = __ballot(hit);
-      if (m) {
-        if (hit) buf[wcnt + (uint32_t)__popcll(m & ((1ull << lane) - 1))] = key;
-        wcnt += (uint32_t)__popcll(m);
-        if (wcnt > SEL_WAVE_CAP - 64)
-          sel_flush<SEG>(buf, wcnt, S, cand, cap, lane, &seg_cnt, seg0);
-      }
+      const int64_t i = sel_elem_index(nfull, tid, u);
+      v[u] = i < n ? gptr(coef)[i] : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < SEL_ITEMS; u++) {
+      elem(v[u], sel_elem_index(nfull, tid, u) < n);
+      if ((u & 3) == 3) check();
     }
   }
   if (wcnt) sel_flush<SEG>(buf, wcnt, S, cand, cap, lane, &seg_cnt, seg0);
   for (int off = 32; off > 0; off >>= 1) {
-    nb += (uint32_t)__shfl_xor((int)nb, off);
+    cnt += (uint32_t)__shfl_xor((int)cnt, off);
     mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
     mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
-    mk = max(mk, (uint32_t)__shfl_xor((int)mk, off));
-    nz += (uint32_t)__shfl_xor((int)nz, off);
   }
-  if (lane == 0) {
-    red[0][wave] = nb; red[1][wave] = mn; red[2][wave] = mx; red[3][wave] = mk; red[4][wave] = nz;
-  }
+  if (lane == 0) { red[0][wave] = cnt; red[1][wave] = mn; red[2][wave] = mx; }
   __syncthreads();
   if (tid == 0) {
     for (int w = 1; w < WV_BLOCK / 64; w++) {
-      nb += red[0][w];
+      cnt += red[0][w];
       mn = min(mn, red[1][w]);
       mx = max(mx, red[2][w]);
-      mk = max(mk, red[3][w]);
-      nz += red[4][w];
     }
-    S->part[bidx][0] = nb;
+    // the largest |x| is at one of the ends of the value order: the most
+    // negative or the most positive value
+    uint32_t mk = 0;
+    if (mn <= mx) {
+      mk = __builtin_bit_cast(uint32_t, wv_ord2f(mn)) & 0x7fffffffu;
+      mk = max(mk, __builtin_bit_cast(uint32_t, wv_ord2f(mx)) & 0x7fffffffu);
+    }
+    S->part[bidx][0] = klo > 0 ? cnt : 0u;   // keys below the window
     S->part[bidx][1] = mn;
     S->part[bidx][2] = mx;
     S->part[bidx][3] = mk;
-    S->part[bidx][4] = nz;
+    S->part[bidx][4] = klo > 0 ? 0u : cnt;   // zeros inside the window
     S->part[bidx][5] = SEG ? seg_cnt : 0u;
   }
 }
@@ -579,92 +626,113 @@ __device__ void sel_result_dev(int64_t n, const SelState *S, wv_thresh_result *r
 }
 
 // The plan every block of a round derives on its own, from what earlier
-// launches left in the state: the verify (the partials reduced, the window
-// checked) and the buckets of the rounds already run.  It is deterministic,
-// so every block agrees, and no launch needs a grid-wide synchronisation
-// (a device-scope release fence in every block costs an L2 write-back).
-struct SelPlan {
-  uint32_t below, min_ord, max_ord, max_key, zwin, crank, base, bits, ncand;
-  int fallback;
-};
-
-__device__ void sel_plan(const SelState *S, int nparts, int rounds_done, SelPlan *pl) {
+// launches left in the state, so no launch needs a grid-wide
+// synchronisation (a device-scope release fence in every block costs an L2
+// write-back).  Round 0 reduces the collect partials and checks the window;
+// round r > 0 takes the plan round r - 1 ran (stored by that round's block
+// 0) and narrows it to the rank's bucket in round r - 1's histogram.  The
+// plan is deterministic, so every block agrees.  round = 3: the finish.
+__device__ void sel_plan(SelState *S, int nparts, int round, SelPlan *pl) {
   __shared__ uint32_t r[6][16];
   __shared__ uint32_t rb, rr;
   const int t = threadIdx.x, nw = blockDim.x >> 6;
-  uint32_t nb = 0, mn = 0xffffffffu, mx = 0, mk = 0, nz = 0, nc = 0;
-  for (int i = t; i < nparts; i += blockDim.x) {
-    nb += S->part[i][0];
-    mn = min(mn, S->part[i][1]);
-    mx = max(mx, S->part[i][2]);
-    mk = max(mk, S->part[i][3]);
-    nz += S->part[i][4];
-    nc += S->part[i][5];
-  }
-  for (int off = 32; off > 0; off >>= 1) {
-    nb += (uint32_t)__shfl_xor((int)nb, off);
-    mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
-    mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
-    mk = max(mk, (uint32_t)__shfl_xor((int)mk, off));
-    nz += (uint32_t)__shfl_xor((int)nz, off);
-    nc += (uint32_t)__shfl_xor((int)nc, off);
-  }
-  if ((t & 63) == 0) {
-    r[0][t >> 6] = nb; r[1][t >> 6] = mn; r[2][t >> 6] = mx; r[3][t >> 6] = mk; r[4][t >> 6] = nz;
-    r[5][t >> 6] = nc;
-  }
-  __syncthreads();
-  if (t == 0) {
-    for (int w = 1; w < nw; w++) {
-      nb += r[0][w];
-      mn = min(mn, r[1][w]);
-      mx = max(mx, r[2][w]);
-      mk = max(mk, r[3][w]);
-      nz += r[4][w];
-      nc += r[5][w];
+  if (round == 0) {
+    uint32_t nb = 0, mn = 0xffffffffu, mx = 0, mk = 0, nz = 0, nc = 0;
+    // every load issued before the first use
+    constexpr int PER = SEL_GRID / SEL_RBLOCK;
+    uint32_t v[PER][6];
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int i = t + k * (int)blockDim.x;
+#pragma unroll
+      for (int f = 0; f < 6; f++) v[k][f] = i < nparts ? S->part[i][f] : 0u;
+      if (i >= nparts) { v[k][1] = 0xffffffffu; }
     }
-    pl->ncand = nc;
-    pl->below = nb;
-    pl->min_ord = mn;
-    pl->max_ord = mx;
-    pl->max_key = mk;
-    // the zeros are in the window when it starts at 0 (else they are below)
-    pl->zwin = S->key_lo == 0 ? nz : 0u;
-    const uint32_t idx = (uint32_t)S->idx;
-    if (S->special) {
-      pl->fallback = 0;
-      pl->crank = 0;
-      pl->base = S->base;
-      pl->bits = 0;
-    } else if (nb <= idx && idx < nb + nc + pl->zwin) {
-      pl->fallback = 0;
-      pl->crank = idx - nb;
-      pl->base = S->base;
-      pl->bits = S->bits;
-    } else {
-      // exact fallback: the rank among all keys, from the full key range
-      pl->fallback = 1;
-      pl->zwin = 0;
-      pl->crank = idx;
-      pl->base = 0;
-      pl->bits = 31;
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      nb += v[k][0];
+      mn = min(mn, v[k][1]);
+      mx = max(mx, v[k][2]);
+      mk = max(mk, v[k][3]);
+      nz += v[k][4];
+      nc += v[k][5];
     }
-  }
-  __syncthreads();
-  for (int k = 0; k < rounds_done; k++) {
-    if (pl->bits == 0) break;
-    if (t == 0) { rb = 0; rr = 0; }
+    for (int i = t + PER * (int)blockDim.x; i < nparts; i += blockDim.x) {
+      nb += S->part[i][0];
+      mn = min(mn, S->part[i][1]);
+      mx = max(mx, S->part[i][2]);
+      mk = max(mk, S->part[i][3]);
+      nz += S->part[i][4];
+      nc += S->part[i][5];
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      nb += (uint32_t)__shfl_xor((int)nb, off);
+      mn = min(mn, (uint32_t)__shfl_xor((int)mn, off));
+      mx = max(mx, (uint32_t)__shfl_xor((int)mx, off));
+      mk = max(mk, (uint32_t)__shfl_xor((int)mk, off));
+      nz += (uint32_t)__shfl_xor((int)nz, off);
+      nc += (uint32_t)__shfl_xor((int)nc, off);
+    }
+    if ((t & 63) == 0) {
+      r[0][t >> 6] = nb; r[1][t >> 6] = mn; r[2][t >> 6] = mx; r[3][t >> 6] = mk;
+      r[4][t >> 6] = nz; r[5][t >> 6] = nc;
+    }
     __syncthreads();
-    find_bucket_blk<WV_H1_BINS>(S->rhist[k], pl->crank, &rb, &rr);
     if (t == 0) {
-      const uint32_t bits = pl->bits;
-      const uint32_t d = bits < 11u ? bits : 11u;
-      pl->base += rb << (bits - d);
-      pl->bits = bits - d;
-      pl->crank = rr;
+      for (int w = 1; w < nw; w++) {
+        nb += r[0][w];
+        mn = min(mn, r[1][w]);
+        mx = max(mx, r[2][w]);
+        mk = max(mk, r[3][w]);
+        nz += r[4][w];
+        nc += r[5][w];
+      }
+      pl->ncand = nc;
+      pl->below = nb;
+      pl->min_ord = mn;
+      pl->max_ord = mx;
+      pl->max_key = mk;
+      // the zeros are in the window when it starts at 0 (else they are below)
+      pl->zwin = S->key_lo == 0 ? nz : 0u;
+      const uint32_t idx = (uint32_t)S->idx;
+      if (S->special) {
+        pl->fallback = 0;
+        pl->crank = 0;
+        pl->base = S->base;
+        pl->bits = 0;
+      } else if (nb <= idx && idx < nb + nc + pl->zwin) {
+        pl->fallback = 0;
+        pl->crank = idx - nb;
+        pl->base = S->base;
+        pl->bits = S->bits;
+      } else {
+        // exact fallback: the rank among all keys, from the full key range
+        pl->fallback = 1;
+        pl->zwin = 0;
+        pl->crank = idx;
+        pl->base = 0;
+        pl->bits = 31;
+      }
     }
     __syncthreads();
+  } else {
+    if (t == 0) *pl = S->splan[round - 1];
+    __syncthreads();
+    if (pl->bits != 0) {
+      if (t == 0) { rb = 0; rr = 0; }
+      __syncthreads();
+      find_bucket_blk<WV_H1_BINS>(S->rhist[round - 1], pl->crank, &rb, &rr);
+      if (t == 0) {
+        const uint32_t bits = pl->bits;
+        const uint32_t d = bits < 11u ? bits : 11u;
+        pl->base += rb << (bits - d);
+        pl->bits = bits - d;
+        pl->crank = rr;
+      }
+      __syncthreads();
+    }
   }
+  if (round < 3 && blockIdx.x == 0 && t == 0) S->splan[round] = *pl;
 }
 
 // Round k over SEL_RGRID blocks: the next min(bits, 11) bits of (key - base)
@@ -676,11 +744,11 @@ __device__ void sel_plan(const SelState *S, int nparts, int rounds_done, SelPlan
 __device__ inline void sel_hist_range(const uint32_t *src, int64_t a, int64_t e, uint32_t base,
                                       uint32_t bits, uint32_t *h) {
   const WV_GLOBAL uint32_t *g = gptr(src);
-  for (int64_t i0 = a; i0 < e; i0 += WV_BLOCK * 4) {
+  for (int64_t i0 = a; i0 < e; i0 += (int64_t)blockDim.x * 4) {
     uint32_t kk[4];
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const int64_t i = i0 + u * WV_BLOCK + threadIdx.x;
+      const int64_t i = i0 + (int64_t)u * blockDim.x + threadIdx.x;
       kk[u] = i < e ? g[i] & 0x7fffffffu : 0xffffffffu;
     }
 #pragma unroll
@@ -692,14 +760,60 @@ __device__ inline void sel_hist_range(const uint32_t *src, int64_t a, int64_t e,
   }
 }
 
-__global__ void __launch_bounds__(WV_BLOCK)
+// The histogram of the listed keys of a round block's segments (segments
+// sg = blockIdx.x + k * gridDim.x), walked as one run of keys so that every
+// thread has SEL_ITEMS loads in flight at a time across segment ends: the
+// segment lengths are loaded first, then position v of the run maps to
+// segment k with pre[k] <= v < pre[k + 1].
+#define SEL_RSEG ((SEL_GRID + SEL_RGRID - 1) / SEL_RGRID)
+__device__ inline void sel_hist_segments(const uint32_t *cand, const SelState *S, int64_t n,
+                                         int nparts, uint32_t base, uint32_t bits,
+                                         uint32_t *h) {
+  const int64_t nchunk = (n + SEL_CHUNK - 1) / SEL_CHUNK;
+  int64_t off[SEL_RSEG];    // segment start minus its position in the run
+  uint32_t pre[SEL_RSEG + 1];
+  uint32_t len[SEL_RSEG];
+#pragma unroll
+  for (int k = 0; k < SEL_RSEG; k++) {
+    const int sg = blockIdx.x + k * gridDim.x;
+    len[k] = sg < nparts ? S->part[sg][5] : 0u;
+  }
+  pre[0] = 0;
+#pragma unroll
+  for (int k = 0; k < SEL_RSEG; k++) {
+    const int sg = blockIdx.x + k * gridDim.x;
+    pre[k + 1] = pre[k] + len[k];
+    off[k] = (sg < nparts ? nchunk * sg / nparts * SEL_CHUNK : 0) - (int64_t)pre[k];
+  }
+  const uint32_t total = pre[SEL_RSEG];
+  const WV_GLOBAL uint32_t *g = gptr(cand);
+  for (uint32_t i0 = 0; i0 < total; i0 += SEL_RBLOCK * SEL_ITEMS) {
+    uint32_t kk[SEL_ITEMS];
+#pragma unroll
+    for (int u = 0; u < SEL_ITEMS; u++) {
+      const uint32_t v = i0 + u * SEL_RBLOCK + threadIdx.x;
+      int64_t o = off[0];
+#pragma unroll
+      for (int k = 1; k < SEL_RSEG; k++) o = v >= pre[k] ? off[k] : o;
+      kk[u] = v < total ? g[o + v] & 0x7fffffffu : 0xffffffffu;
+    }
+#pragma unroll
+    for (int u = 0; u < SEL_ITEMS; u++) {
+      const uint32_t key = kk[u];
+      if (key != 0xffffffffu && in_range(key, base, bits))
+        atomicAdd(&h[round_digit(key, base, bits)], 1u);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(SEL_RBLOCK)
 sel_round(const uint32_t *cand, const float *coef, int64_t n, SelState *S, int nparts,
           int round) {
   __shared__ uint32_t h[WV_H1_BINS];
   __shared__ SelPlan pl;
   sel_plan(S, nparts, round, &pl);
   if (S->special || pl.bits == 0) return;
-  for (int i = threadIdx.x; i < WV_H1_BINS; i += WV_BLOCK) h[i] = 0;
+  for (int i = threadIdx.x; i < WV_H1_BINS; i += SEL_RBLOCK) h[i] = 0;
   __syncthreads();
   const uint32_t base = pl.base, bits = pl.bits;
   if (pl.fallback) {
@@ -709,17 +823,13 @@ sel_round(const uint32_t *cand, const float *coef, int64_t n, SelState *S, int n
     sel_hist_range((const uint32_t *)coef, a, min(n, a + per), base, bits, h);
   } else {
     // the collect blocks' segments, a share of them per block
-    const int64_t nchunk = (n + SEL_CHUNK - 1) / SEL_CHUNK;
-    for (int sg = blockIdx.x; sg < nparts; sg += gridDim.x) {
-      const int64_t a = nchunk * sg / nparts * SEL_CHUNK;
-      sel_hist_range(cand, a, a + S->part[sg][5], base, bits, h);
-    }
+    sel_hist_segments(cand, S, n, nparts, base, bits, h);
   }
   // the listed keys exclude the window's zeros: block 0 adds them
   if (threadIdx.x == 0 && blockIdx.x == 0 && pl.zwin && in_range(0u, base, bits))
     atomicAdd(&h[round_digit(0u, base, bits)], pl.zwin);
   __syncthreads();
-  for (int i = threadIdx.x; i < WV_H1_BINS; i += WV_BLOCK)
+  for (int i = threadIdx.x; i < WV_H1_BINS; i += SEL_RBLOCK)
     if (h[i]) atomicAdd(&S->rhist[round][i], h[i]);
 }
 
@@ -743,6 +853,8 @@ sel_finish(int64_t n, SelState *S, int nparts, wv_thresh_result *res) {
   }
 }
 
+static_assert(SEL_RGRID * SEL_RSEG >= SEL_GRID, "a round must cover every collect segment");
+
 hipError_t threshold_select(const float *coef, int64_t n, float frac,
                             const void *stats, void *ws,
                             wv_thresh_result *res, hipStream_t s) {
@@ -757,7 +869,7 @@ hipError_t threshold_select(const float *coef, int64_t n, float frac,
   hipLaunchKernelGGL(sel_collect, dim3(grid), dim3(WV_BLOCK), 0, s, coef, n, S, cand);
   // three rounds cover any window (11 + 11 + 11 >= 31 bits)
   for (int r = 0; r < 3; r++)
-    hipLaunchKernelGGL(sel_round, dim3(SEL_RGRID), dim3(WV_BLOCK), 0, s, cand, coef, n, S, grid,
+    hipLaunchKernelGGL(sel_round, dim3(SEL_RGRID), dim3(SEL_RBLOCK), 0, s, cand, coef, n, S, grid,
                        r);
   hipLaunchKernelGGL(sel_finish, dim3(1), dim3(WV_BLOCK), 0, s, n, S, grid, res);
   return hipGetLastError();
