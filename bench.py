@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--split-overlap", type=int, default=8, help="split: overlap")
     ap.add_argument("--serve-count", type=int, default=300, help="requests of the serving run")
     ap.add_argument("--serve-slots", type=int, default=8, help="serving device workers")
+    ap.add_argument("--serve-edge", type=int, default=0,
+                    help="serving: equal int16 cubes of this edge instead of the lognormal mix")
     ap.add_argument("--serve-load", type=float, default=0.7,
                     help="offered rate as a fraction of the measured saturation rate")
     return ap.parse_args()
@@ -631,14 +633,26 @@ def serving(args, wv, torch, dev):
     N = args.serve_count
     reqs, in_bytes = [], 0
     t0 = time.time()
-    for i, (size, dt, keep) in enumerate(synth.serving_requests(N, seed=5)):
-        vol = synth.cubelet_device(size, dt, 5000 + i, device=dev).cpu().numpy()
-        in_bytes += vol.nbytes
-        reqs.append((vol, 1.0 - keep))
+    if args.serve_edge:
+        # equal requests: int16 cubes of one edge, keep 10%
+        e = args.serve_edge
+        for i in range(N):
+            vol = synth.cubelet_device((e, e, e), "int16", 7000 + i, device=dev).cpu().numpy()
+            in_bytes += vol.nbytes
+            reqs.append((vol, 0.9))
+        mix = "%d requests, int16 %d^3 each, keep 10%%" % (N, e)
+    else:
+        for i, (size, dt, keep) in enumerate(synth.serving_requests(N, seed=5)):
+            vol = synth.cubelet_device(size, dt, 5000 + i, device=dev).cpu().numpy()
+            in_bytes += vol.nbytes
+            reqs.append((vol, 1.0 - keep))
+        mix = ("%d requests, edges lognormal [32,512], depth [1,512], u8/i16/f32, "
+               "keep U[0.02,0.2]" % N)
     log("generated %d serving requests (%.2f GB) in %.1f s" % (N, in_bytes / 1e9, time.time() - t0))
 
-    def run(rate):
-        srv = wv.Server(slots=args.serve_slots, hbm_budget=200 << 30, encode=True)
+    def run(rate, lloyd_batch=False):
+        srv = wv.Server(slots=args.serve_slots, hbm_budget=200 << 30, encode=True,
+                        lloyd_batch=lloyd_batch)
         arrivals = (np.cumsum(rng.exponential(1.0 / rate, N)) if rate else np.zeros(N))
         done = []
         start = time.time()
@@ -655,25 +669,36 @@ def serving(args, wv, torch, dev):
         assert len(done) == N and all(d["status"] == 0 for d in done)
         lat = np.array([d["t_done"] - d["t_submit"] for d in done]) * 1e3
         span = max(d["t_done"] for d in done) - min(d["t_submit"] for d in done)
-        return dict(p50_ms=round(float(np.percentile(lat, 50)), 3),
-                    p99_ms=round(float(np.percentile(lat, 99)), 3),
-                    gbps=round(in_bytes / span / 1e9, 3), rps=round(N / span, 2),
-                    peak_inflight=st["peak_inflight"],
-                    peak_device_gb=round(st["peak_bytes"] / 1e9, 3))
+        out = dict(p50_ms=round(float(np.percentile(lat, 50)), 3),
+                   p99_ms=round(float(np.percentile(lat, 99)), 3),
+                   gbps=round(in_bytes / span / 1e9, 3), rps=round(N / span, 2),
+                   peak_inflight=st["peak_inflight"],
+                   peak_device_gb=round(st["peak_bytes"] / 1e9, 3))
+        if lloyd_batch and st["lloyd_windows"]:
+            out["lloyd_runs_per_window"] = round(st["lloyd_run_windows"] / st["lloyd_windows"], 2)
+        return out
 
     sat = run(None)                    # all requests at once: capacity
     rate = args.serve_load * sat["rps"]
     fixed = run(rate)
     log("serving: saturation %.1f req/s %.2f GB/s; at %.1f req/s p50 %.2f ms p99 %.2f ms" % (
         sat["rps"], sat["gbps"], rate, fixed["p50_ms"], fixed["p99_ms"]))
-    return {"workload": "%d requests, edges lognormal [32,512], depth [1,512], u8/i16/f32, "
-                        "Haar 4,4,4, Lloyd 256 (30 it), keep U[0.02,0.2]; %d device workers, "
-                        "200 GB HBM budget; latency = submit -> bins+counts+.cube on host"
-                        % (N, args.serve_slots),
+    # the same stream with the Lloyd windows of the requests in flight
+    # sharing launches (lloyd_batch)
+    sat_b = run(None, lloyd_batch=True)
+    rate_b = args.serve_load * sat_b["rps"]
+    fixed_b = run(rate_b, lloyd_batch=True)
+    log("serving, batched Lloyd windows: saturation %.1f req/s; at %.1f req/s p99 %.2f ms" % (
+        sat_b["rps"], rate_b, fixed_b["p99_ms"]))
+    return {"workload": "%s; Haar 4,4,4, Lloyd 256 (30 it); %d device workers, 200 GB HBM "
+                        "budget; latency = submit -> bins+counts+.cube on host; each worker "
+                        "runs its own Lloyd windows" % (mix, args.serve_slots),
             "input_bytes": int(in_bytes),
             "saturation": sat,
             "offered_rps": round(rate, 2),
-            "at_offered_rate": fixed}
+            "at_offered_rate": fixed,
+            "lloyd_batch": {"saturation": sat_b, "offered_rps": round(rate_b, 2),
+                            "at_offered_rate": fixed_b}}
 
 
 def main():

@@ -500,6 +500,13 @@ typedef struct {
   int64_t hbm_budget_bytes;  /* device bytes admitted requests may hold */
   int encode;                /* 1: host Huffman + .cube image per request */
   int host_threads;          /* encoder threads (0: = slots) */
+  int lloyd_batch;           /* 1: the Lloyd windows of the requests in flight share
+                                launches: runs of a power-of-two size class join one
+                                thread and stream at window boundaries, and each
+                                window is one set of launches for all of them.  Pays
+                                for streams of similar sizes (a window lasts as long
+                                as its longest run); 0 (default): each worker runs
+                                its own windows */
 } wv_serve_config;
 
 typedef struct {
@@ -529,6 +536,9 @@ int wv_serve_submit(wv_server *s, const wv_serve_request *r, int64_t *id);
 int wv_serve_poll(wv_server *s, wv_serve_result *out, int max);
 int wv_serve_drain(wv_server *s);   /* until every submitted request completed */
 int wv_serve_stats(wv_server *s, int64_t *peak_bytes, int *peak_inflight, double *now_s);
+/* with lloyd_batch: the window launch sets so far and the Lloyd runs they
+ * served (runs / windows = runs per launch set); zeros without it */
+int wv_serve_lloyd_stats(wv_server *s, int64_t *windows, int64_t *run_windows);
 int wv_serve_destroy(wv_server *s);  /* drains, then stops the threads */
 
 /* ------------------------------------------------------------------ *

@@ -54,7 +54,8 @@ def _submit_all(srv, vols):
     return ids
 
 
-def test_serve_results_same_alone_and_in_mix(wv):
+@pytest.mark.parametrize("lloyd_batch", [False, True])
+def test_serve_results_same_alone_and_in_mix(wv, lloyd_batch):
     vols = [_vol(r[0], r[1], 40 + k) for k, r in enumerate(REQS)]
     alone = {}
     srv = wv.Server(slots=1)
@@ -65,7 +66,7 @@ def test_serve_results_same_alone_and_in_mix(wv):
         assert r["status"] == 0
         alone[k] = r
     srv.close()
-    mix = wv.Server(slots=6)
+    mix = wv.Server(slots=6, lloyd_batch=lloyd_batch)
     ids = _submit_all(mix, vols)
     res = {r["id"]: r for r in mix.drain()}
     st = mix.stats()
@@ -119,3 +120,48 @@ def test_serve_rejects_request_over_budget(wv):
     with pytest.raises(wv.WvError):
         srv.submit(_vol((32, 32, 32), np.float32, 5), 2, [3, 3, 3], 1, 256, 0.9)
     srv.close()
+
+
+# Lloyd requests of different sizes, codebook sizes and iteration caps: with
+# lloyd_batch their windows share launches (runs join and leave at window
+# boundaries, a capped run's last window is shorter than the others')
+LLOYD_REQS = [
+    ((16, 32, 32), np.uint8, 256, 30),
+    ((24, 40, 40), np.int16, 64, 3),
+    ((32, 32, 32), np.float32, 1024, 30),
+    ((8, 64, 48), np.int16, 256, 9),
+    ((40, 24, 24), np.float32, 256, -1),
+    ((16, 16, 16), np.uint8, 128, 17),
+    ((32, 48, 32), np.int32, 512, 30),
+    ((12, 36, 52), np.float32, 256, 1),
+]
+
+
+def test_serve_lloyd_batch_same_as_alone(wv):
+    vols = [_vol(r[0], r[1], 120 + k) for k, r in enumerate(LLOYD_REQS)]
+    alone = {}
+    srv = wv.Server(slots=1, lloyd_batch=False)
+    for k, v in enumerate(vols):
+        shape, dt, bins, iters = LLOYD_REQS[k]
+        srv.submit(v, 1, [4, 4, 4], 3, bins, 0.85, lloyd_iters=iters, want_bins=True)
+        (r,) = srv.drain()
+        assert r["status"] == 0
+        alone[k] = r
+    srv.close()
+    mix = wv.Server(slots=8, lloyd_batch=True)
+    ids = [mix.submit(v, 1, [4, 4, 4], 3, r[2], 0.85, lloyd_iters=r[3], want_bins=True)
+           for v, r in zip(vols, LLOYD_REQS)]
+    res = {r["id"]: r for r in mix.drain()}
+    st = mix.stats()
+    mix.close()
+    for k, i in enumerate(ids):
+        a, m = alone[k], res[i]
+        assert m["status"] == 0, k
+        assert a["lloyd_iters"] == m["lloyd_iters"], k
+        assert np.array_equal(a["bins"], m["bins"]), k
+        assert a["file"] == m["file"], k
+    cap = LLOYD_REQS[1][3]
+    assert res[ids[1]]["lloyd_iters"] <= cap
+    print("lloyd windows %d serving %d runs" % (st["lloyd_windows"], st["lloyd_run_windows"]))
+    assert st["lloyd_windows"] > 0
+    assert st["lloyd_run_windows"] > st["lloyd_windows"]     # windows were shared

@@ -107,7 +107,7 @@ class SplitInfo(C.Structure):
 
 class ServeConfig(C.Structure):
     _fields_ = [("slots", C.c_int), ("hbm_budget_bytes", C.c_int64), ("encode", C.c_int),
-                ("host_threads", C.c_int)]
+                ("host_threads", C.c_int), ("lloyd_batch", C.c_int)]
 
 P = C.c_void_p
 I64 = C.c_int64
@@ -264,6 +264,7 @@ def _declare(L):
         L.wv_serve_stats.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int),
                                      C.POINTER(C.c_double)]
         L.wv_serve_destroy.argtypes = [P]
+        L.wv_serve_lloyd_stats.argtypes = [P, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     if hasattr(L, "wv_lloyd"):
         L.wv_lloyd_ws_bytes.restype = I64
         L.wv_lloyd_ws_bytes.argtypes = [I64, C.c_int]
@@ -888,8 +889,10 @@ class Server:
     """The serving loop (wv_serve_*): requests are host arrays [z, y, x];
     results come back with the bins, counts and .cube image of each."""
 
-    def __init__(self, slots=8, hbm_budget=200 << 30, encode=True, host_threads=0):
-        cfg = ServeConfig(slots, hbm_budget, 1 if encode else 0, host_threads)
+    def __init__(self, slots=8, hbm_budget=200 << 30, encode=True, host_threads=0,
+                 lloyd_batch=False):
+        cfg = ServeConfig(slots, hbm_budget, 1 if encode else 0, host_threads,
+                          1 if lloyd_batch else 0)
         self.h = C.c_void_p()
         check(lib().wv_serve_create(C.byref(cfg), C.byref(self.h)))
         self.live = {}
@@ -942,7 +945,10 @@ class Server:
     def stats(self):
         pb, pi, now = C.c_int64(), C.c_int(), C.c_double()
         check(lib().wv_serve_stats(self.h, C.byref(pb), C.byref(pi), C.byref(now)))
-        return dict(peak_bytes=pb.value, peak_inflight=pi.value, now=now.value)
+        w, rw = C.c_int64(), C.c_int64()
+        check(lib().wv_serve_lloyd_stats(self.h, C.byref(w), C.byref(rw)))
+        return dict(peak_bytes=pb.value, peak_inflight=pi.value, now=now.value,
+                    lloyd_windows=w.value, lloyd_run_windows=rw.value)
 
     def close(self):
         if self.h:

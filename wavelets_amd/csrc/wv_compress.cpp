@@ -81,7 +81,8 @@ struct Stages {
 };
 
 int device_stages(const void *in_d, int dtype, const int orig[3], const wv_options *opt,
-                  void *work_d, int64_t work_bytes, hipStream_t s, Stages &st) {
+                  void *work_d, int64_t work_bytes, hipStream_t s, Stages &st,
+                  LloydBatcher *lb = nullptr) {
   if (!in_d || !orig || !opt || !work_d) { set_error("null argument"); return WV_ERR_ARG; }
   if (opt->bin_count < 2 || opt->bin_count > WV_MAX_BINS) { set_error("bin count"); return WV_ERR_ARG; }
   CHK(wv_geom_init(&st.g, orig, opt->steps, opt->wavelet, dtype));
@@ -121,8 +122,8 @@ int device_stages(const void *in_d, int dtype, const int orig[3], const wv_optio
   if (opt->quant_alg == WV_QUANT_LLOYD) {
     st.bounds.resize(opt->bin_count - 1);
     st.values.resize(opt->bin_count);
-    CHK(wv_lloyd(coef, g.count, &res, opt->bin_count, opt->lloyd_max_iters, w + L.lloyd,
-                 st.bounds.data(), st.values.data(), &st.lloyd_iters, s));
+    CHK(lloyd_run(coef, g.count, &res, opt->bin_count, opt->lloyd_max_iters, w + L.lloyd,
+                  st.bounds.data(), st.values.data(), &st.lloyd_iters, s, lb));
     CHK(wv_quant_params_init(&st.qp, WV_QUANT_LLOYD, opt->bin_count, res.threshold, 0,
                              st.bounds.data(), st.values.data()));
   } else {
@@ -198,14 +199,16 @@ extern "C" int64_t wv_compress_counts_offset(const int orig[3], const wv_options
   return layout_for(g, opt).counts;
 }
 
-extern "C" int wv_compress_device(const void *in_d, int dtype, const int orig[3],
-                                  const wv_options *opt, void *work_d, int64_t work_bytes,
-                                  float *codebook_out, wv_compress_info *info,
-                                  void *stream) {
+namespace wv {
+
+// wv_compress_device with an optional Lloyd batcher (the serving loop's)
+int compress_device(const void *in_d, int dtype, const int orig[3], const wv_options *opt,
+                    void *work_d, int64_t work_bytes, float *codebook_out,
+                    wv_compress_info *info, hipStream_t stream, LloydBatcher *lb) {
   if (!info) { set_error("null argument"); return WV_ERR_ARG; }
   std::memset(info, 0, sizeof *info);
   Stages st;
-  CHK(device_stages(in_d, dtype, orig, opt, work_d, work_bytes, (hipStream_t)stream, st));
+  CHK(device_stages(in_d, dtype, orig, opt, work_d, work_bytes, stream, st, lb));
   if (codebook_out && opt->quant_alg == WV_QUANT_LLOYD) {
     std::memcpy(codebook_out, st.bounds.data(), st.bounds.size() * sizeof(float));
     std::memcpy(codebook_out + st.bounds.size(), st.values.data(),
@@ -217,6 +220,16 @@ extern "C" int wv_compress_device(const void *in_d, int dtype, const int orig[3]
   info->zero_bin = st.qp.zero_bin;
   info->lloyd_iters = st.lloyd_iters;
   return WV_OK;
+}
+
+}  // namespace wv
+
+extern "C" int wv_compress_device(const void *in_d, int dtype, const int orig[3],
+                                  const wv_options *opt, void *work_d, int64_t work_bytes,
+                                  float *codebook_out, wv_compress_info *info,
+                                  void *stream) {
+  return compress_device(in_d, dtype, orig, opt, work_d, work_bytes, codebook_out, info,
+                         (hipStream_t)stream, nullptr);
 }
 
 extern "C" int wv_compress(const void *in_d, int dtype, const int orig[3],

@@ -98,6 +98,22 @@ hipError_t stats_compute(const float *coef, int64_t n, void *stats,
 int64_t radix_sort_hist_bytes(int64_t max_n);
 hipError_t radix_sort_u32(uint32_t *keys, uint32_t *tmp, uint32_t *hist,
                           const uint32_t *count_d, int64_t max_n, hipStream_t s);
+
+// Lloyd runs (wv_lloyd.hip).  lloyd_run is wv_lloyd with an optional
+// batcher: with one, the run's windows share launches with the other runs
+// in flight (the serving loop), on the batcher's own thread and stream.
+struct LloydBatcher;
+LloydBatcher *lloyd_batcher_create(int device);
+void lloyd_batcher_destroy(LloydBatcher *b);
+// window launch sets so far, and the runs they served (their ratio: runs per launch set)
+void lloyd_batcher_stats(LloydBatcher *b, int64_t *windows, int64_t *run_windows);
+int lloyd_run(const float *coef_d, int64_t n, const wv_thresh_result *res, int bin_count,
+              int max_iters, void *ws_d, float *bin_boundaries, float *bin_values,
+              int *iters_out, hipStream_t s, LloydBatcher *batcher);
+// wv_compress_device with an optional Lloyd batcher (wv_compress.cpp)
+int compress_device(const void *in_d, int dtype, const int orig[3], const wv_options *opt,
+                    void *work_d, int64_t work_bytes, float *codebook_out,
+                    wv_compress_info *info, hipStream_t stream, LloydBatcher *lb);
 hipError_t threshold_select(const float *coef, int64_t n, float frac,
                             const void *stats, void *ws,
                             wv_thresh_result *res, hipStream_t s);

@@ -24,6 +24,11 @@
 #include "wv_internal.h"
 #include <algorithm>
 #include <cmath>
+#include <condition_variable>
+#include <cstdio>
+#include <deque>
+#include <mutex>
+#include <thread>
 #include <vector>
 
 namespace wv {
@@ -218,48 +223,75 @@ __device__ inline int between(const float *cb, int cs, float p) {
   return -1;
 }
 
-// assignment of every point; records the first index of every group run
-// and flags a group that appears in more than one run
-// (ring: a 16-bit copy of the nearest groups, kept for the distortion
-// chain of this iteration, which runs later)
+// ---- one run's device pointers ------------------------------------------
+// Kernels read their run from a table (blockIdx.y), so one launch serves
+// one run or many: in the serving loop the Lloyd runs of all admitted
+// requests share every launch of a window.
+struct LJob {
+  LloydState *S;
+  const float *pts;
+  int32_t *grp;
+  float *cb, *part, *sum;
+  uint32_t *count;
+  int32_t *first, *last, *nruns;
+  float *dsum;
+  uint16_t *ring;      // LL_WINDOW slots of ring_stride 16-bit groups
+  float *cbring;       // LL_WINDOW codebooks of cs + 1 floats
+  float *wdsum;        // LL_WINDOW distortion sums
+  int64_t ring_stride;
+  int32_t cs, psize;
+  int32_t nw;          // iterations of the current window for this run
+  int32_t pad;
+};
+
+// k: the iteration of the current window (-1 outside the windows); a run
+// whose window holds fewer iterations (its iteration cap) skips the rest
+__device__ inline bool ll_skip(const LJob &j, int k) {
+  return j.S->done || (k >= 0 && k >= j.nw);
+}
+
+// assignment of every point (ring: a 16-bit copy of the nearest groups,
+// kept for the distortion chain of window iteration k, which runs later)
 template <bool NEAREST>
 __global__ void __launch_bounds__(WV_BLOCK)
-ll_assign(const float *pts, const float *cb_g, LloydState *S, int32_t *grp,
-          uint16_t *ring) {
+ll_assign(const LJob *J, int k) {
   extern __shared__ float cb[];
-  if (S->done) return;
-  const int cs = S->cs;
-  for (int j = threadIdx.x; j < cs; j += WV_BLOCK) cb[j] = cb_g[j];
+  const LJob j = J[blockIdx.y];
+  if (ll_skip(j, k)) return;
+  const int cs = j.cs;
+  for (int t = threadIdx.x; t < cs; t += WV_BLOCK) cb[t] = j.cb[t];
   __syncthreads();
-  const int psize = S->psize;
+  uint16_t *ring = NEAREST && k >= 0 ? j.ring + (int64_t)k * j.ring_stride : nullptr;
+  const int psize = j.psize;
   for (int i = blockIdx.x * WV_BLOCK + threadIdx.x; i < psize; i += gridDim.x * WV_BLOCK) {
-    const int g = NEAREST ? nearest(cb, cs, pts[i]) : between(cb, cs, pts[i]);
-    grp[i] = g;
-    if (NEAREST && ring) ring[i] = (uint16_t)g;
+    const float p = j.pts[i];
+    const int g = NEAREST ? nearest(cb, cs, p) : between(cb, cs, p);
+    j.grp[i] = g;
+    if (ring) ring[i] = (uint16_t)g;
   }
 }
 
 // run boundaries: a group whose members are contiguous has exactly one run
 __global__ void __launch_bounds__(WV_BLOCK)
-ll_runs(LloydState *S, const int32_t *grp, int32_t *first, int32_t *last,
-        int32_t *nruns) {
-  if (S->done) return;
-  const int psize = S->psize;
+ll_runs(const LJob *J, int k) {
+  const LJob j = J[blockIdx.y];
+  if (ll_skip(j, k)) return;
+  const int psize = j.psize;
   for (int i = blockIdx.x * WV_BLOCK + threadIdx.x; i < psize; i += gridDim.x * WV_BLOCK) {
-    const int g = grp[i];
+    const int g = j.grp[i];
     if (g < 0) continue;
-    if (i == 0 || grp[i - 1] != g) { first[g] = i; atomicAdd(&nruns[g], 1); }
-    if (i == psize - 1 || grp[i + 1] != g) last[g] = i;
+    if (i == 0 || j.grp[i - 1] != g) { j.first[g] = i; atomicAdd(&j.nruns[g], 1); }
+    if (i == psize - 1 || j.grp[i + 1] != g) j.last[g] = i;
   }
 }
 
-__global__ void ll_clear_runs(LloydState *S, int32_t *nruns, int32_t *first,
-                              int32_t *last) {
-  if (S->done) return;
-  for (int j = threadIdx.x; j < S->cs; j += blockDim.x) {
-    nruns[j] = 0;
-    first[j] = 0;
-    last[j] = -1;
+__global__ void ll_clear_runs(const LJob *J, int k) {
+  const LJob j = J[blockIdx.y];
+  if (ll_skip(j, k)) return;
+  for (int t = threadIdx.x; t < j.cs; t += blockDim.x) {
+    j.nruns[t] = 0;
+    j.first[t] = 0;
+    j.last[t] = -1;
   }
 }
 
@@ -381,60 +413,62 @@ __device__ __forceinline__ void cb_to_lds(float *lcb, const float *cb, int cs) {
 
 template <bool DIST>
 __global__ void __launch_bounds__(64)
-ll_chains(const float *__restrict__ pts, const int32_t *__restrict__ grp,
-          const float *__restrict__ cb, LloydState *S, const int32_t *__restrict__ first,
-          const int32_t *__restrict__ last, const int32_t *__restrict__ nruns,
-          float *sum, uint32_t *count, float *dist_out) {
-  if (S->done) return;
+ll_chains(const LJob *J, int k) {
+  const LJob j = J[blockIdx.y];
   const int chain = blockIdx.x;
-  const int cs = uni(S->cs);
-  const int psize = uni(S->psize);
+  const int cs = uni(j.cs);
+  if (chain > cs || (!DIST && chain == cs)) return;   // a larger run's grid
+  if (ll_skip(j, k)) return;
+  const int psize = uni(j.psize);
   if (chain < cs) {
-    const int nr = uni(nruns[chain]);
+    const int nr = uni(j.nruns[chain]);
     float s = 0.f;
     uint32_t c = 0;
     if (nr == 1) {
-      const int a = uni(first[chain]), b = uni(last[chain] + 1);
+      const int a = uni(j.first[chain]), b = uni(j.last[chain] + 1);
       c = (uint32_t)(b - a);
-      s = run_chain(pts, a, b);
+      s = run_chain(j.pts, a, b);
     } else if (nr > 1) {
       // members scattered over several runs: scan every point in order
       // (wave-uniform scalar loads, one dependent add per member)
+      const float *pts = j.pts;
+      const int32_t *grp = j.grp;
       int i = 0;
       for (; i + 16 <= psize; i += 16) {
         float v[16];
         int g[16];
 #pragma unroll
-        for (int k = 0; k < 16; k++) { v[k] = pts[i + k]; g[k] = grp[i + k]; }
+        for (int q = 0; q < 16; q++) { v[q] = pts[i + q]; g[q] = grp[i + q]; }
 #pragma unroll
-        for (int k = 0; k < 16; k++)
-          if (g[k] == chain) { s += v[k]; c++; }
+        for (int q = 0; q < 16; q++)
+          if (g[q] == chain) { s += v[q]; c++; }
       }
       for (; i < psize; i++)
         if (grp[i] == chain) { s += pts[i]; c++; }
     }
-    if (threadIdx.x == 0) { sum[chain] = s; count[chain] = c; }
-  } else if (DIST) {
+    if (threadIdx.x == 0) { j.sum[chain] = s; j.count[chain] = c; }
+  } else {
     // dist += codebook[idx] - p over all points in order (lloyds.cpp:64,132)
     extern __shared__ float lcb[];
-    cb_to_lds(lcb, cb, cs);
-    const float d = dist_chain(pts, grp, lcb, psize);
-    if (threadIdx.x == 0) *dist_out = d;
+    cb_to_lds(lcb, j.cb, cs);
+    const float d = dist_chain(j.pts, j.grp, lcb, psize);
+    if (threadIdx.x == 0) *j.dsum = d;
   }
 }
 
 // before the loop: partition from the initial codebook, dist, reldist
-__global__ void ll_init_finish(LloydState *S, const float *cb, float *part,
-                               const float *dsum, const float *pts) {
-  const int cs = S->cs;
+__global__ void ll_init_finish(const LJob *J) {
+  const LJob j = J[blockIdx.y];
+  LloydState *S = j.S;
+  const int cs = j.cs;
   for (int i = threadIdx.x; i < cs - 1; i += blockDim.x)
-    part[i] = (cb[i] + cb[i + 1]) / 2;
+    j.part[i] = (j.cb[i] + j.cb[i + 1]) / 2;
   if (threadIdx.x == 0) {
     // pmax / pmin by sequential comparisons equal the ends of the sorted
     // points (lloyds.cpp:40-49)
-    S->pmin = pts[0];
-    S->pmax = pts[S->psize - 1];
-    float dist = *dsum / (float)(unsigned)S->psize;
+    S->pmin = j.pts[0];
+    S->pmax = j.pts[S->psize - 1];
+    float dist = *j.dsum / (float)(unsigned)S->psize;
     S->dist = dist;
     S->reldist = fabsf(dist);
     S->iters = 0;
@@ -443,69 +477,78 @@ __global__ void ll_init_finish(LloydState *S, const float *cb, float *part,
 }
 
 // loop head + codebook update (lloyds.cpp:69-87)
-__global__ void ll_update_codebook(LloydState *S, float *cb, const float *part,
-                                   const float *sum, const uint32_t *count) {
-  if (S->done) return;
-  const int cs = S->cs;
-  __syncthreads();
+__global__ void ll_update_codebook(const LJob *J, int k) {
+  const LJob j = J[blockIdx.y];
+  if (ll_skip(j, k)) return;
+  const LloydState *S = j.S;
+  const int cs = j.cs;
   for (int i = threadIdx.x; i < cs; i += blockDim.x) {
-    if (count[i] != 0) cb[i] = sum[i] / (float)count[i];
-    else if (i == 0) cb[i] = (part[0] + S->pmin) / 2;
-    else if (i == cs - 1) cb[i] = (part[i - 1] + S->pmax) / 2;
-    else cb[i] = (part[i - 1] + part[i]) / 2;
+    if (j.count[i] != 0) j.cb[i] = j.sum[i] / (float)j.count[i];
+    else if (i == 0) j.cb[i] = (j.part[0] + S->pmin) / 2;
+    else if (i == cs - 1) j.cb[i] = (j.part[i - 1] + S->pmax) / 2;
+    else j.cb[i] = (j.part[i - 1] + j.part[i]) / 2;
   }
 }
 
 // partition update (lloyds.cpp:102-111)
-__global__ void ll_update_partition(LloydState *S, const float *cb, float *part,
-                                    const float *sum, const uint32_t *count) {
-  if (S->done) return;
-  const int cs = S->cs;
+__global__ void ll_update_partition(const LJob *J, int k) {
+  const LJob j = J[blockIdx.y];
+  if (ll_skip(j, k)) return;
+  const int cs = j.cs;
   for (int i = threadIdx.x; i < cs - 1; i += blockDim.x) {
-    if (count[i] != 0) part[i] = sum[i] / (float)count[i];
-    else part[i] = (cb[i] + cb[i + 1]) / 2;
+    if (j.count[i] != 0) j.part[i] = j.sum[i] / (float)j.count[i];
+    else j.part[i] = (j.cb[i] + j.cb[i + 1]) / 2;
   }
 }
 
 // ---- speculative windows ---------------------------------------------------
 // The codebook trajectory depends only on the cell sums and counts
 // (lloyds.cpp:73-111); the signed distortion decides only when the loop
-// stops.  So a window runs W iterations of the trajectory back to back,
-// each leaving its codebook and 16-bit nearest groups in a slot, then the W
-// distortion chains run side by side (one wavefront each, the same
-// sequential sum over the points in order as lloyds.cpp:124-133), and one
-// thread applies the loop test to them in order: the first iteration that
-// stops the loop is the result, the later ones of the window are dropped.
-__global__ void ll_snapshot(const LloydState *S, const float *cb, float *dst) {
-  if (S->done) return;
-  for (int i = threadIdx.x; i < S->cs; i += blockDim.x) dst[i] = cb[i];
+// stops.  So a window runs its iterations of the trajectory back to back,
+// each leaving its codebook and 16-bit nearest groups in a slot, then the
+// distortion chains of the window run side by side (one wavefront each, the
+// same sequential sum over the points in order as lloyds.cpp:124-133), and
+// one thread applies the loop test to them in order: the first iteration
+// that stops the loop is the result, the later ones of the window are
+// dropped.
+__global__ void ll_snapshot(const LJob *J, int k) {
+  const LJob j = J[blockIdx.y];
+  if (ll_skip(j, k)) return;
+  float *dst = j.cbring + (int64_t)k * (j.cs + 1);
+  for (int i = threadIdx.x; i < j.cs; i += blockDim.x) dst[i] = j.cb[i];
 }
 
 __global__ void __launch_bounds__(64)
-ll_window_dist(const float *__restrict__ pts, const uint16_t *__restrict__ ring,
-               int64_t ring_stride, const float *__restrict__ cbs, int cb_stride,
-               const LloydState *S, float *dsum) {
-  if (S->done) return;
+ll_window_dist(const LJob *J) {
+  const LJob j = J[blockIdx.y];
   const int w = blockIdx.x;
+  if (j.S->done || w >= j.nw) return;
   extern __shared__ float lcb[];
-  cb_to_lds(lcb, cbs + (int64_t)w * cb_stride, uni(S->cs));
-  const float d = dist_chain(pts, ring + (int64_t)w * ring_stride, lcb, uni(S->psize));
-  if (threadIdx.x == 0) dsum[w] = d;
+  cb_to_lds(lcb, j.cbring + (int64_t)w * (j.cs + 1), uni(j.cs));
+  const float d = dist_chain(j.pts, j.ring + (int64_t)w * j.ring_stride, lcb, uni(j.psize));
+  if (threadIdx.x == 0) j.wdsum[w] = d;
 }
 
 // the loop tail of each iteration of the window, in order: reldist =
 // |dist_old - dist_new|, then the loop test (lloyds.cpp:122-135, plus the
 // iteration cap)
-__global__ void ll_window_eval(LloydState *S, const float *dsum, int nw) {
-  for (int w = 0; w < nw && !S->done; w++) {
+__global__ void ll_window_eval(const LJob *J) {
+  const LJob j = J[blockIdx.y];
+  LloydState *S = j.S;
+  for (int w = 0; w < j.nw && !S->done; w++) {
     const float prev = S->dist;
-    const float dist = dsum[w] / (float)(unsigned)S->psize;
+    const float dist = j.wdsum[w] / (float)(unsigned)S->psize;
     S->dist = dist;
     S->reldist = fabsf(prev - dist);
     S->iters++;
     S->final_slot = w;
     S->done = !(S->reldist > S->stop) || (S->max_iters >= 0 && S->iters >= S->max_iters);
   }
+}
+
+// the done flags of a table of runs, for one host poll
+__global__ void ll_poll(const LJob *J, int njobs, int32_t *done) {
+  for (int t = threadIdx.x; t < njobs; t += blockDim.x) done[t] = J[t].S->done;
 }
 
 __global__ void ll_fill_dups(uint32_t *tail, uint32_t ndup, uint32_t tkey) {
@@ -534,7 +577,7 @@ inline int64_t a256(int64_t v) { return (v + 255) & ~(int64_t)255; }
 constexpr int LL_WINDOW = 8;   // iterations per speculative window
 struct LWs {
   int64_t state, keys, keys2, hist, grp, cb, part, sum, count, first, last, nruns, dsum;
-  int64_t ring, ring_stride, cbring, wdsum, total;
+  int64_t ring, ring_stride, cbring, wdsum, jobtab, total;
 };
 LWs lloyd_layout(int64_t n, int bins) {
   LWs L{};
@@ -558,6 +601,7 @@ LWs lloyd_layout(int64_t n, int bins) {
   L.ring = o; o += L.ring_stride * 2 * LL_WINDOW;
   L.cbring = o; o += a256((int64_t)(cs + 1) * 4 * LL_WINDOW);
   L.wdsum = o; o += a256(4 * LL_WINDOW);
+  L.jobtab = o; o += a256(sizeof(LJob));
   L.total = o;
   return L;
 }
@@ -598,15 +642,21 @@ static int bins_from_codebook(float *v, float *b, int bins, const float *c, int 
   return (nv == bins && nb == bins - 1) ? WV_OK : WV_ERR_STATE;
 }
 
-extern "C" int wv_lloyd(const float *coef_d, int64_t n, const wv_thresh_result *res,
-                        int bin_count, int max_iters, void *ws_d,
-                        float *bin_boundaries, float *bin_values, int *iters_out,
-                        void *stream) {
-  hipStream_t s = (hipStream_t)stream;
-  if (!coef_d || !res || !ws_d || !bin_boundaries || !bin_values || n <= 0) {
-    set_error("bad arguments");
-    return WV_ERR_ARG;
-  }
+namespace wv {
+
+// One Lloyd run between its set-up and its result: the device pointers
+// into its workspace, its one-entry run table there, and the window count.
+struct LloydRun {
+  LJob job;
+  LJob *job_d;
+  int ag, max_iters, enqueued;
+  LloydState hs;
+};
+
+// points, initial codebook, initial assignment and distortion; the state
+// comes back to R.hs (R.hs.done: the loop never runs)
+static int lloyd_setup(const float *coef_d, int64_t n, const wv_thresh_result *res,
+                       int bin_count, int max_iters, void *ws_d, LloydRun &R, hipStream_t s) {
   const int cs = (bin_count - 1) / 2;
   const int64_t nz = res->nonzero_count;
   const int64_t psize = nz - 1;
@@ -619,26 +669,18 @@ extern "C" int wv_lloyd(const float *coef_d, int64_t n, const wv_thresh_result *
   LloydState *S = (LloydState *)(w + L.state);
   uint32_t *keys = (uint32_t *)(w + L.keys), *keys2 = (uint32_t *)(w + L.keys2);
   uint32_t *hist = (uint32_t *)(w + L.hist);
-  int32_t *grp = (int32_t *)(w + L.grp);
-  float *cb = (float *)(w + L.cb), *part = (float *)(w + L.part), *sum = (float *)(w + L.sum);
-  uint32_t *count = (uint32_t *)(w + L.count);
-  int32_t *first = (int32_t *)(w + L.first), *last = (int32_t *)(w + L.last),
-          *nruns = (int32_t *)(w + L.nruns);
-  float *dsum = (float *)(w + L.dsum);
 
   hipLaunchKernelGGL(ll_state_init, dim3(1), dim3(1), 0, s, S, (int)psize, cs, max_iters);
   // 1. tail = threshold duplicates ranked after idx, then sorted keys > T
   const int64_t gb = (n + LL_GCHUNK - 1) / LL_GCHUNK;
   hipLaunchKernelGGL(ll_gather, dim3((unsigned)gb), dim3(WV_BLOCK), 0, s, coef_d, n,
                      res->threshold_key, keys, S);
-  LloydState hs;
-  LCHK(hipMemcpyAsync(&hs, S, sizeof hs, hipMemcpyDeviceToHost, s), "D2H");
+  LCHK(hipMemcpyAsync(&R.hs, S, sizeof R.hs, hipMemcpyDeviceToHost, s), "D2H");
   LCHK(hipStreamSynchronize(s), "sync");
-  const int64_t ngt = hs.n_gt;
+  const int64_t ngt = R.hs.n_gt;
   const int64_t ndup = nz - ngt;
   if (ndup < 0 || ndup > n) { set_error("lloyd: inconsistent threshold result"); return WV_ERR_STATE; }
   LCHK(radix_sort_u32(keys, keys2, hist, &S->n_gt, ngt, s), "sort");
-  const uint32_t *src = keys;
   // the sorted keys are in keys; lay out the tail [dups..., sorted...] in
   // keys2
   uint32_t *tail = keys2;
@@ -646,8 +688,32 @@ extern "C" int wv_lloyd(const float *coef_d, int64_t n, const wv_thresh_result *
     hipLaunchKernelGGL(ll_fill_dups, dim3(64), dim3(256), 0, s, tail, (uint32_t)ndup,
                        res->threshold_key);
   if (ngt > 0)
-    LCHK(hipMemcpyAsync(tail + ndup, src, ngt * 4, hipMemcpyDeviceToDevice, s), "tail copy");
-  const float *pts = (const float *)tail;   // points = tail[0 .. nz-2]
+    LCHK(hipMemcpyAsync(tail + ndup, keys, ngt * 4, hipMemcpyDeviceToDevice, s), "tail copy");
+
+  LJob &J = R.job;
+  J = LJob{};
+  J.S = S;
+  J.pts = (const float *)tail;   // points = tail[0 .. nz-2]
+  J.grp = (int32_t *)(w + L.grp);
+  J.cb = (float *)(w + L.cb);
+  J.part = (float *)(w + L.part);
+  J.sum = (float *)(w + L.sum);
+  J.count = (uint32_t *)(w + L.count);
+  J.first = (int32_t *)(w + L.first);
+  J.last = (int32_t *)(w + L.last);
+  J.nruns = (int32_t *)(w + L.nruns);
+  J.dsum = (float *)(w + L.dsum);
+  J.ring = (uint16_t *)(w + L.ring);
+  J.cbring = (float *)(w + L.cbring);
+  J.wdsum = (float *)(w + L.wdsum);
+  J.ring_stride = L.ring_stride;
+  J.cs = cs;
+  J.psize = (int32_t)psize;
+  J.nw = 0;
+  R.job_d = (LJob *)(w + L.jobtab);
+  R.ag = (int)std::min<int64_t>((psize + WV_BLOCK - 1) / WV_BLOCK, 4096);
+  R.max_iters = max_iters;
+  R.enqueued = 0;
 
   // 2. initial codebook on the host: minAbs = threshold, maxAbs = tail[nz-2]
   float max2;
@@ -655,68 +721,304 @@ extern "C" int wv_lloyd(const float *coef_d, int64_t n, const wv_thresh_result *
   LCHK(hipStreamSynchronize(s), "sync");
   std::vector<float> hcb(cs);
   initial_codebook(hcb.data(), cs, res->threshold, max2);
-  LCHK(hipMemcpyAsync(cb, hcb.data(), cs * 4, hipMemcpyHostToDevice, s), "H2D codebook");
+  LCHK(hipMemcpyAsync(J.cb, hcb.data(), cs * 4, hipMemcpyHostToDevice, s), "H2D codebook");
+  LCHK(hipMemcpyAsync(R.job_d, &J, sizeof J, hipMemcpyHostToDevice, s), "H2D run table");
 
-  const unsigned ag = (unsigned)std::min<int64_t>((psize + WV_BLOCK - 1) / WV_BLOCK, 4096);
+  // 3. initial assignment (lloyds.cpp:35-67): nearest groups, their cell
+  // sums and the distortion
   const size_t cb_lds = (size_t)cs * 4;
-  uint16_t *ring = (uint16_t *)(w + L.ring);
-  float *cbring = (float *)(w + L.cbring), *wdsum = (float *)(w + L.wdsum);
-  // one assignment pass and its cell chains (+ the distortion chain with
-  // dist); ring: the slot for the nearest groups
-  auto assign_and_chains = [&](bool nearest_pass, bool dist, uint16_t *slot) {
-    if (nearest_pass)
-      hipLaunchKernelGGL(ll_assign<true>, dim3(ag), dim3(WV_BLOCK), cb_lds, s, pts, cb, S, grp,
-                         slot);
-    else
-      hipLaunchKernelGGL(ll_assign<false>, dim3(ag), dim3(WV_BLOCK), cb_lds, s, pts, cb, S, grp,
-                         (uint16_t *)nullptr);
-    hipLaunchKernelGGL(ll_clear_runs, dim3(1), dim3(256), 0, s, S, nruns, first, last);
-    hipLaunchKernelGGL(ll_runs, dim3(ag), dim3(WV_BLOCK), 0, s, S, grp, first, last, nruns);
-    if (dist)
-      hipLaunchKernelGGL(ll_chains<true>, dim3(cs + 1), dim3(64), cb_lds, s, pts, grp, cb, S, first,
-                         last, nruns, sum, count, dsum);
-    else
-      hipLaunchKernelGGL(ll_chains<false>, dim3(cs), dim3(64), 0, s, pts, grp, cb, S, first,
-                         last, nruns, sum, count, dsum);
-  };
-
-  // 3. initial assignment (lloyds.cpp:35-67)
-  assign_and_chains(true, true, nullptr);
-  hipLaunchKernelGGL(ll_init_finish, dim3(1), dim3(256), 0, s, S, cb, part, dsum, pts);
-  LCHK(hipMemcpyAsync(&hs, S, sizeof hs, hipMemcpyDeviceToHost, s), "D2H state");
+  hipLaunchKernelGGL(ll_assign<true>, dim3(R.ag), dim3(WV_BLOCK), cb_lds, s, R.job_d, -1);
+  hipLaunchKernelGGL(ll_clear_runs, dim3(1), dim3(256), 0, s, R.job_d, -1);
+  hipLaunchKernelGGL(ll_runs, dim3(R.ag), dim3(WV_BLOCK), 0, s, R.job_d, -1);
+  hipLaunchKernelGGL(ll_chains<true>, dim3(cs + 1), dim3(64), cb_lds, s, R.job_d, -1);
+  hipLaunchKernelGGL(ll_init_finish, dim3(1), dim3(256), 0, s, R.job_d);
+  LCHK(hipMemcpyAsync(&R.hs, S, sizeof R.hs, hipMemcpyDeviceToHost, s), "D2H state");
   LCHK(hipStreamSynchronize(s), "sync");
+  return WV_OK;
+}
 
-  // 4. iterations in speculative windows, one host poll per window
-  int enqueued = 0;
-  while (!hs.done) {
-    int nw = LL_WINDOW;
-    if (max_iters >= 0 && max_iters - enqueued < nw) nw = max_iters - enqueued;
-    for (int k = 0; k < nw; k++) {
-      hipLaunchKernelGGL(ll_update_codebook, dim3(1), dim3(256), 0, s, S, cb, part, sum, count);
-      assign_and_chains(false, false, nullptr);
-      hipLaunchKernelGGL(ll_update_partition, dim3(1), dim3(256), 0, s, S, cb, part, sum, count);
-      assign_and_chains(true, false, ring + (int64_t)k * L.ring_stride);
-      hipLaunchKernelGGL(ll_snapshot, dim3(1), dim3(256), 0, s, S, cb, cbring + (int64_t)k * (cs + 1));
+// the window's iterations nw for a run, from its iteration cap
+static int window_iters(const LloydRun &R) {
+  int nw = LL_WINDOW;
+  if (R.max_iters >= 0 && R.max_iters - R.enqueued < nw) nw = R.max_iters - R.enqueued;
+  return nw;
+}
+
+// one window of iterations for the njobs runs of the table J (run j takes
+// iterations k < its nw; W = the largest nw); grids cover the largest run
+static void enqueue_window(const LJob *J, int njobs, int max_cs, int max_ag, int W,
+                           hipStream_t s) {
+  const size_t cb_lds = (size_t)max_cs * 4;
+  auto assign_and_chains = [&](bool nearest_pass, int k) {
+    if (nearest_pass)
+      hipLaunchKernelGGL(ll_assign<true>, dim3(max_ag, njobs), dim3(WV_BLOCK), cb_lds, s, J, k);
+    else
+      hipLaunchKernelGGL(ll_assign<false>, dim3(max_ag, njobs), dim3(WV_BLOCK), cb_lds, s, J, k);
+    hipLaunchKernelGGL(ll_clear_runs, dim3(1, njobs), dim3(256), 0, s, J, k);
+    hipLaunchKernelGGL(ll_runs, dim3(max_ag, njobs), dim3(WV_BLOCK), 0, s, J, k);
+    hipLaunchKernelGGL(ll_chains<false>, dim3(max_cs, njobs), dim3(64), 0, s, J, k);
+  };
+  for (int k = 0; k < W; k++) {
+    hipLaunchKernelGGL(ll_update_codebook, dim3(1, njobs), dim3(256), 0, s, J, k);
+    assign_and_chains(false, k);
+    hipLaunchKernelGGL(ll_update_partition, dim3(1, njobs), dim3(256), 0, s, J, k);
+    assign_and_chains(true, k);
+    hipLaunchKernelGGL(ll_snapshot, dim3(1, njobs), dim3(256), 0, s, J, k);
+  }
+  hipLaunchKernelGGL(ll_window_dist, dim3(W, njobs), dim3(64), cb_lds, s, J);
+  hipLaunchKernelGGL(ll_window_eval, dim3(1, njobs), dim3(1), 0, s, J);
+}
+
+// the windows of one run on its own stream, one host poll each
+static int lloyd_loop(LloydRun &R, hipStream_t s) {
+  while (!R.hs.done) {
+    const int nw = window_iters(R);
+    if (nw != R.job.nw) {
+      R.job.nw = nw;
+      LCHK(hipMemcpyAsync(R.job_d, &R.job, sizeof R.job, hipMemcpyHostToDevice, s),
+           "H2D run table");
     }
-    hipLaunchKernelGGL(ll_window_dist, dim3(nw), dim3(64), cb_lds, s, pts, ring, L.ring_stride, cbring,
-                       cs + 1, S, wdsum);
-    hipLaunchKernelGGL(ll_window_eval, dim3(1), dim3(1), 0, s, S, wdsum, nw);
-    enqueued += nw;
+    enqueue_window(R.job_d, 1, R.job.cs, R.ag, nw, s);
+    R.enqueued += nw;
     LCHK(hipGetLastError(), "launch");
-    LCHK(hipMemcpyAsync(&hs, S, sizeof hs, hipMemcpyDeviceToHost, s), "D2H state");
+    LCHK(hipMemcpyAsync(&R.hs, R.job.S, sizeof R.hs, hipMemcpyDeviceToHost, s), "D2H state");
     LCHK(hipStreamSynchronize(s), "sync");
-    if (!hs.done && enqueued >= LL_HARD_CAP) {
-      set_error("lloyd: no convergence after %d iterations", enqueued);
+    if (!R.hs.done && R.enqueued >= LL_HARD_CAP) {
+      set_error("lloyd: no convergence after %d iterations", R.enqueued);
       return WV_ERR_STATE;
     }
   }
+  return WV_OK;
+}
+
+// ---- the Lloyd runs of concurrent requests, batched ----------------------
+// A window batch: one thread owns a stream and a run table.  Runs join at
+// window boundaries; each window is one set of launches for every active
+// run (grid.y = run), then one poll of all their done flags.  A run leaves
+// when its loop stops, and its worker continues.
+struct LloydWindowBatch {
+  struct Entry {
+    LloydRun *run;
+    int status = WV_OK;
+    bool finished = false;
+    char msg[160] = {0};   // the error, set in the worker's thread (messages are per thread)
+  };
+  static constexpr int kCap = 64;
+  int device = 0;
+  hipStream_t s = nullptr;
+  LJob *tab_d = nullptr, *tab_h = nullptr;
+  int32_t *flags_d = nullptr, *flags_h = nullptr;
+  std::mutex mu;
+  std::condition_variable cv, cv_done;
+  std::deque<Entry *> pending;
+  bool stop = false;
+  int64_t windows = 0, run_windows = 0;   // launches sets, and runs served by them
+  std::thread th;
+
+  int init(int dev) {
+    device = dev;
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc((void **)&tab_d, kCap * sizeof(LJob)) != hipSuccess ||
+        hipMalloc((void **)&flags_d, kCap * sizeof(int32_t)) != hipSuccess ||
+        hipHostMalloc((void **)&tab_h, kCap * sizeof(LJob), hipHostMallocDefault) != hipSuccess ||
+        hipHostMalloc((void **)&flags_h, kCap * sizeof(int32_t), hipHostMallocDefault) !=
+            hipSuccess) {
+      set_error("lloyd batcher: device resources");
+      return WV_ERR_HIP;
+    }
+    th = std::thread(&LloydWindowBatch::loop, this);
+    return WV_OK;
+  }
+
+  ~LloydWindowBatch() {
+    {
+      std::lock_guard<std::mutex> lk(mu);
+      stop = true;
+    }
+    cv.notify_all();
+    if (th.joinable()) th.join();
+    if (tab_d) (void)hipFree(tab_d);
+    if (flags_d) (void)hipFree(flags_d);
+    if (tab_h) (void)hipHostFree(tab_h);
+    if (flags_h) (void)hipHostFree(flags_h);
+    if (s) (void)hipStreamDestroy(s);
+  }
+
+  // runs R's windows with the other runs in flight; returns its status
+  int run(LloydRun *R) {
+    Entry e;
+    e.run = R;
+    std::unique_lock<std::mutex> lk(mu);
+    pending.push_back(&e);
+    cv.notify_all();
+    cv_done.wait(lk, [&] { return e.finished; });
+    if (e.status != WV_OK) set_error("%s", e.msg);
+    return e.status;
+  }
+
+  void finish(std::vector<Entry *> &active, size_t i, int status, const char *msg) {
+    std::lock_guard<std::mutex> lk(mu);
+    active[i]->status = status;
+    if (msg) snprintf(active[i]->msg, sizeof active[i]->msg, "%s", msg);
+    active[i]->finished = true;
+    cv_done.notify_all();
+  }
+
+  void loop() {
+    (void)hipSetDevice(device);
+    std::vector<Entry *> active;
+    for (;;) {
+      {
+        std::unique_lock<std::mutex> lk(mu);
+        cv.wait(lk, [&] { return stop || !pending.empty() || !active.empty(); });
+        if (stop && pending.empty() && active.empty()) break;
+        while (!pending.empty() && (int)active.size() < kCap) {
+          active.push_back(pending.front());
+          pending.pop_front();
+        }
+      }
+      const int nj = (int)active.size();
+      int W = 0, max_cs = 0, max_ag = 0;
+      for (int i = 0; i < nj; i++) {
+        LloydRun &R = *active[i]->run;
+        R.job.nw = window_iters(R);
+        tab_h[i] = R.job;
+        W = std::max(W, R.job.nw);
+        max_cs = std::max(max_cs, R.job.cs);
+        max_ag = std::max(max_ag, R.ag);
+      }
+      hipError_t e = hipMemcpyAsync(tab_d, tab_h, nj * sizeof(LJob), hipMemcpyHostToDevice, s);
+      if (e == hipSuccess) {
+        enqueue_window(tab_d, nj, max_cs, max_ag, W, s);
+        hipLaunchKernelGGL(ll_poll, dim3(1), dim3(64), 0, s, tab_d, nj, flags_d);
+        e = hipGetLastError();
+      }
+      if (e == hipSuccess)
+        e = hipMemcpyAsync(flags_h, flags_d, nj * sizeof(int32_t), hipMemcpyDeviceToHost, s);
+      if (e == hipSuccess) e = hipStreamSynchronize(s);
+      {
+        std::lock_guard<std::mutex> lk(mu);
+        windows++;
+        run_windows += nj;
+      }
+      std::vector<Entry *> keep;
+      for (int i = 0; i < nj; i++) {
+        LloydRun &R = *active[i]->run;
+        R.enqueued += R.job.nw;
+        if (e != hipSuccess) {
+          char m[160];
+          snprintf(m, sizeof m, "lloyd batcher: %s", hipGetErrorString(e));
+          finish(active, i, WV_ERR_HIP, m);
+        } else if (flags_h[i]) {
+          finish(active, i, WV_OK, nullptr);
+        } else if (R.enqueued >= LL_HARD_CAP) {
+          char m[160];
+          snprintf(m, sizeof m, "lloyd: no convergence after %d iterations", R.enqueued);
+          finish(active, i, WV_ERR_STATE, m);
+        } else {
+          keep.push_back(active[i]);
+        }
+      }
+      active.swap(keep);
+    }
+  }
+};
+
+// A window lasts as long as its longest run, so runs are batched by size:
+// one window batch per power-of-two class of the point count (a run waits
+// for runs at most twice its size), each with its own thread and stream,
+// created when a run of its class first arrives.  Runs of equal size share
+// every launch; different sizes still run side by side.
+struct LloydBatcher {
+  int device = 0;
+  std::mutex mu;
+  LloydWindowBatch *cls[32] = {};
+
+  ~LloydBatcher() {
+    for (LloydWindowBatch *b : cls) delete b;
+  }
+
+  LloydWindowBatch *batch_for(int psize) {
+    int c = 0;
+    while (c < 31 && (2 << c) <= psize) c++;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!cls[c]) {
+      LloydWindowBatch *b = new LloydWindowBatch;
+      if (b->init(device) != WV_OK) {
+        delete b;
+        return nullptr;
+      }
+      cls[c] = b;
+    }
+    return cls[c];
+  }
+
+  int run(LloydRun *R) {
+    LloydWindowBatch *b = batch_for(R->job.psize);
+    return b ? b->run(R) : WV_ERR_HIP;
+  }
+};
+
+LloydBatcher *lloyd_batcher_create(int device) {
+  LloydBatcher *b = new LloydBatcher;
+  b->device = device;
+  return b;
+}
+
+void lloyd_batcher_destroy(LloydBatcher *b) { delete b; }
+
+void lloyd_batcher_stats(LloydBatcher *b, int64_t *windows, int64_t *run_windows) {
+  *windows = *run_windows = 0;
+  std::lock_guard<std::mutex> lk(b->mu);
+  for (LloydWindowBatch *c : b->cls) {
+    if (!c) continue;
+    std::lock_guard<std::mutex> lc(c->mu);
+    *windows += c->windows;
+    *run_windows += c->run_windows;
+  }
+}
+
+// the whole run: set-up, the windows (on this stream, or batched with the
+// other runs in flight), the codebook of the iteration that stopped the
+// loop, and the bins from it
+int lloyd_run(const float *coef_d, int64_t n, const wv_thresh_result *res, int bin_count,
+              int max_iters, void *ws_d, float *bin_boundaries, float *bin_values,
+              int *iters_out, hipStream_t s, LloydBatcher *batcher) {
+  LloydRun R;
+  const int rc = lloyd_setup(coef_d, n, res, bin_count, max_iters, ws_d, R, s);
+  if (rc != WV_OK) return rc;
+  if (!R.hs.done) {
+    const int lrc = batcher ? batcher->run(&R) : lloyd_loop(R, s);
+    if (lrc != WV_OK) return lrc;
+    if (batcher) {
+      LCHK(hipMemcpyAsync(&R.hs, R.job.S, sizeof R.hs, hipMemcpyDeviceToHost, s), "D2H state");
+      LCHK(hipStreamSynchronize(s), "sync");
+    }
+  }
+  const int cs = R.job.cs;
   // the codebook of the iteration that stopped the loop (the initial one
   // when the loop never ran)
-  const float *final_cb = hs.iters > 0 ? cbring + (int64_t)hs.final_slot * (cs + 1) : cb;
+  const float *final_cb =
+      R.hs.iters > 0 ? R.job.cbring + (int64_t)R.hs.final_slot * (cs + 1) : R.job.cb;
   std::vector<float> fin(cs);
   LCHK(hipMemcpyAsync(fin.data(), final_cb, cs * 4, hipMemcpyDeviceToHost, s), "D2H codebook");
   LCHK(hipStreamSynchronize(s), "sync");
-  if (iters_out) *iters_out = hs.iters;
+  if (iters_out) *iters_out = R.hs.iters;
   return bins_from_codebook(bin_values, bin_boundaries, bin_count, fin.data(), cs,
                             res->threshold, res->min_val, res->max_val);
+}
+
+}  // namespace wv
+
+extern "C" int wv_lloyd(const float *coef_d, int64_t n, const wv_thresh_result *res,
+                        int bin_count, int max_iters, void *ws_d,
+                        float *bin_boundaries, float *bin_values, int *iters_out,
+                        void *stream) {
+  if (!coef_d || !res || !ws_d || !bin_boundaries || !bin_values || n <= 0) {
+    set_error("bad arguments");
+    return WV_ERR_ARG;
+  }
+  return lloyd_run(coef_d, n, res, bin_count, max_iters, ws_d, bin_boundaries, bin_values,
+                   iters_out, (hipStream_t)stream, nullptr);
 }

@@ -18,6 +18,9 @@
 //              threshold, [Lloyd], quantize + histogram; the cubelet stays
 //              in HBM throughout) and the device Huffman coder; only the
 //              payload (and bins / counts when asked for) comes back.
+//              With lloyd_batch, the Lloyd windows of all requests in
+//              flight share launches: a run joins a batcher thread at a
+//              window boundary and its worker waits for it there.
 //              The device memory returns to the pool right after.
 //   encode     host threads write the .cube container around the payload,
 //              so the device worker moves on to the next request at once.
@@ -139,6 +142,7 @@ struct wv_server {
   bool stop = false;
   std::vector<std::thread> workers, encoders;
   int device = 0;
+  LloydBatcher *lloyd = nullptr;   // with cfg.lloyd_batch
 
   double now() const {
     return std::chrono::duration<double>(Clock::now() - t0).count();
@@ -220,8 +224,9 @@ struct wv_server {
                      hipSuccess ? WV_OK : WV_ERR_HIP;
     if (r.opt.quant_alg == WV_QUANT_LLOYD) q->codebook.resize(2 * r.opt.bin_count - 1);
     if (rc == WV_OK)
-      rc = wv_compress_device(din, r.dtype, r.orig, &r.opt, work, q->work_bytes,
-                              q->codebook.empty() ? nullptr : q->codebook.data(), &q->info, s);
+      rc = compress_device(din, r.dtype, r.orig, &r.opt, work, q->work_bytes,
+                           q->codebook.empty() ? nullptr : q->codebook.data(), &q->info, s,
+                           lloyd);
     if (rc == WV_OK) {
       wv_geom g;
       rc = wv_geom_init(&g, r.orig, r.opt.steps, r.opt.wavelet, r.dtype);
@@ -306,6 +311,13 @@ int wv_serve_create(const wv_serve_config *cfg, wv_server **out) {
   s->cfg = *cfg;
   s->t0 = Clock::now();
   (void)hipGetDevice(&s->device);
+  if (cfg->lloyd_batch) {
+    s->lloyd = lloyd_batcher_create(s->device);
+    if (!s->lloyd) {
+      delete s;
+      return WV_ERR_HIP;
+    }
+  }
   for (int i = 0; i < cfg->slots; i++) s->workers.emplace_back(&wv_server::device_worker, s);
   const int ne = cfg->encode ? (cfg->host_threads > 0 ? cfg->host_threads : cfg->slots) : 0;
   for (int i = 0; i < ne; i++) s->encoders.emplace_back(&wv_server::encoder, s);
@@ -380,7 +392,15 @@ int wv_serve_destroy(wv_server *s) {
   }
   for (auto &t : s->workers) t.join();
   for (auto &t : s->encoders) t.join();
+  if (s->lloyd) lloyd_batcher_destroy(s->lloyd);
   delete s;
+  return WV_OK;
+}
+
+int wv_serve_lloyd_stats(wv_server *s, int64_t *windows, int64_t *run_windows) {
+  if (!s || !windows || !run_windows) { set_error("bad arguments"); return WV_ERR_ARG; }
+  *windows = *run_windows = 0;
+  if (s->lloyd) lloyd_batcher_stats(s->lloyd, windows, run_windows);
   return WV_OK;
 }
 
