@@ -297,7 +297,8 @@ def single_cubelet(args, wv, torch, dev, steps, warmup):
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
             "traffic_source": "profiles/dwt_pass_traffic_512cube_cdf97.md (rocprofv3 FETCH_SIZE "
-                              "+ WRITE_SIZE, scaled by a 512 MiB copy)" if traffic else None,
+                              "+ WRITE_SIZE, scaled by what they read for a 1 GiB copy kernel)"
+                              if traffic else None,
             "kernel": "dwt_pass_kernel (fused axis pass) on the 512^3 cubelet",
             "pass_ms": [round(float(v), 4) for v in pass_ms],
             "algorithmic_bytes_per_launch": pass_bytes}
@@ -405,10 +406,9 @@ def decompress_leg(wv, torch, dev, file_bytes, out_bytes):
                 "one_call = wv_decompress from a host file image to a host volume (512 MB D2H)"}
 
 
-def mixed_stream(args, wv, torch, dev, steps, warmup):
-    """The batched launch on the mixed stream, and the plain path beside it."""
+def make_stream(args, wv, torch, dev):
+    """The mixed stream's cubelets, generated on the device."""
     from wavelets_amd import synth
-    L = wv.lib()
     stream = synth.mixed_stream(args.count, seed=4)
     t0 = time.time()
     cubes, in_bytes = [], 0
@@ -420,6 +420,28 @@ def mixed_stream(args, wv, torch, dev, steps, warmup):
     torch.cuda.synchronize()
     log("generated %d cubelets (%.2f GB) in %.1f s" % (len(cubes), in_bytes / 1e9,
                                                       time.time() - t0))
+    return cubes, in_bytes
+
+
+def stream_stage_bytes(cubes):
+    """Algorithmic bytes of one submit of the stream, per stage: transform
+    s_in + 4 + 8 (npass - 1) per sample (pass 1 reads the input and writes
+    4 B, each further pass reads and writes 4 B), select 4 (one read),
+    quantize + histogram 8."""
+    t = s = q = 0
+    for c in cubes:
+        n = c["vol"].numel()
+        npass = sum(1 for v in c["steps"] if v > 0)
+        t += n * (c["vol"].element_size() + 4 + 8 * (npass - 1))
+        s += n * 4
+        q += n * 8
+    return {"transform": t, "select": s, "quantize": q}
+
+
+def mixed_stream(args, wv, torch, dev, steps, warmup):
+    """The batched launch on the mixed stream, and the plain path beside it."""
+    L = wv.lib()
+    cubes, in_bytes = make_stream(args, wv, torch, dev)
     t0 = time.time()
     flags = 0 if args.no_autotune else wv.BATCH_AUTOTUNE
     b = wv.Batch(cubes, flags=flags)
@@ -498,11 +520,7 @@ def mixed_stream(args, wv, torch, dev, steps, warmup):
     runs = [b.stage_times() for _ in range(3)]
     st = {k: float(np.median([r[k] for r in runs])) for k in runs[0]}
     samples = int(sum(c["vol"].numel() for c in cubes))
-    t_bytes = 0
-    for c in cubes:
-        npass = sum(1 for v in c["steps"] if v > 0)
-        t_bytes += c["vol"].numel() * (c["vol"].element_size() + 4 + 8 * (npass - 1))
-    stage_bytes = {"transform": t_bytes, "select": 4 * samples, "quantize": 8 * samples}
+    stage_bytes = stream_stage_bytes(cubes)
     stages = {}
     for k in ("transform", "select", "quantize"):
         ms = st[k + "_ms"]
@@ -751,6 +769,9 @@ def main():
         t = batched["stages"]["transform"]
         roof = {"bound": "hbm", "achieved": t["gbps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": t["frac"], "traffic": None,
+                "traffic_note": "not measured on the stream; the fused pass's own HBM traffic "
+                                "per launch, against its algorithmic bytes, is measured on the "
+                                "512^3 cubelet (single_cubelet.roofline)",
                 "kernel": "dwt_pass_kernel: every transform pass of the 2000-cubelet stream "
                           "(%d launches over 30 groups, run serialised)" % t["launches"],
                 "algorithmic_bytes": t["algorithmic_bytes"], "ms": t["ms"],
