@@ -249,6 +249,21 @@ __device__ inline void load_regs(const DwtPass &P, const Tile &t, const size_t *
     }
     return;
   }
+  if (TT == 0 && P.rt_exact && t.nvalid == P.T && !line_major_load(P)) {
+    // the runtime-T form of the exact position-major loads: no clamping,
+    // a uniform base plus a 32-bit lane offset
+    const PosMap m = pos_map(P.ld, t);
+    const char *base = (const char *)(src + m.rowbase + t.a);
+    const int lg = P.log2T;
+    const unsigned voff =
+        (((unsigned)tid >> lg) * (unsigned)m.pstride + ((unsigned)tid & (unsigned)(P.T - 1))) *
+        (unsigned)sizeof(SRC);
+    const size_t ustep = ((size_t)NT >> lg) * m.pstride * sizeof(SRC);
+#pragma unroll
+    for (int u = 0; u < EPT; u++)
+      v[u] = to_f(ld_tile((const SRC *)(base + u * ustep + voff)));
+    return;
+  }
   if (line_major_load(P)) {
     const int pmax = P.ld.pcap;
 #pragma unroll
@@ -357,6 +372,21 @@ __device__ inline void store_tile(const DwtPass &P, const Tile &t, const float *
 #pragma unroll
       for (int u = 0; u < EPT; u++)
         st_tile((float *)(base + u * ustep + voff), b[u * PSTEP * LS]);
+    }
+    return;
+  }
+  if (TT == 0 && P.st_block && t.nvalid == P.T) {
+    // the tile's lines are adjacent in the output: one contiguous block of
+    // T*L floats (line l, position p at l*L + p), stored as 16-byte vectors
+    // of 4 positions of one line (L % 4 == 0)
+    float *base = t.dst + lb[0];
+#pragma unroll
+    for (int j = 0; j < EPT / 4; j++) {
+      const int e = (tid + j * NT) * 4;
+      const int l = (int)__umulhi((unsigned)e, P.magic_L);
+      const int p = e - l * P.L;
+      const float *r = tout + p * LS + l;
+      st_tile4((wv_vec4 *)(base + e), wv_vec4{r[0], r[LS], r[2 * LS], r[3 * LS]});
     }
     return;
   }
@@ -956,6 +986,10 @@ hipError_t launch_pass(const DwtPass &P, int ntiles, int wavelet, bool inverse,
                                      : ((uintptr_t)P.dst & 15) == 0;
   Q.vec_st = Q.exact && !P.st_line && P.T % 32 == 0 && P.st.sp % 4 == 0 &&
              P.st.sv % 4 == 0 && dst_aligned && !vec_store_off();
+  Q.rt_exact = (long long)P.L * P.T == (long long)P.nt * P.ept && P.T >= 8 && unpadded &&
+               ((long long)(P.nt / P.T) * stride + P.T) * 4 < (1ll << 31);
+  Q.st_block = Q.rt_exact && P.st_line && P.st.su == P.L && P.L % 4 == 0 &&
+               P.st.sv % 4 == 0 && dst_aligned && !vec_store_off();
   if (wavelet == WV_WAVELET_CDF97)
     return inverse ? launch_wave<WV_WAVELET_CDF97, true>(Q, src_dtype, clamp, ntiles, s)
                    : launch_wave<WV_WAVELET_CDF97, false>(Q, src_dtype, clamp, ntiles, s);

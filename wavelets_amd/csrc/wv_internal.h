@@ -56,6 +56,11 @@ struct DwtPass {
   // every dst_tab pointer is 16-byte aligned (batches; P.dst is checked
   // directly); vec_st: exact tiles store their segments as 16-byte vectors
   int dst_tab_aligned, vec_st;
+  // tiles of the runtime-T kernels (any L): rt_exact: a full tile holds
+  // exactly nt*ept elements and needs no clamping; st_block: such a tile's
+  // lines are adjacent in the output (line stride L), so it stores one
+  // contiguous block of T*L floats as 16-byte vectors
+  int rt_exact, st_block;
 };
 
 // Memory layouts of a cubelet, fastest axis first: x,y,z (the input and
