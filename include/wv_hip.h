@@ -609,9 +609,12 @@ int64_t wv_split_compress(const void *vol_d, int dtype, const int size[3], const
  * forward transform once, takes min / max / max |x| and sorts |x| on the
  * device; in_d (the original samples) must stay valid until destroy.  A
  * test point quantizes with threshold T and bin_count bins (uniform or
- * log, max = max |x|; bin_count <= 0: only |x| <= T -> 0), inverts, and
- * compares with the original over its extent (restore rules of
- * computeErrorRatesAfterDequant).  size_bytes is
+ * log, max = max |x|; Lloyd: the code book trained on the sorted |x| from
+ * findSorted(T), all but the largest, with no iteration cap, as
+ * computeLloydQuantization does (test_compress_cpu.cc:404-459, the
+ * reference's fixedPSNR sweep, optimize.cc:53-90); bin_count <= 0: only
+ * |x| <= T -> 0), inverts, and compares with the original over its extent
+ * (restore rules of computeErrorRatesAfterDequant).  size_bytes is
  * Huffman::totalEncodedLengthBytes of that quantization (huffman.cc:
  * 241-252), 0 without quantization.  Error sums are double, deterministic.
  * ------------------------------------------------------------------ */
@@ -621,6 +624,7 @@ typedef struct {
   float l1, l2, mse, psnr, rel;   /* getL1Error, getL2Error, getMeanSquaredError,
                                      getPeakSignalToNoiseRatio, getRelativeError */
   int zero_bin;                   /* quant(0); -1 without quantization */
+  int lloyd_iters;                /* Lloyd iterations of the test point's code book */
 } wv_opt_result;
 int wv_opt_create(const void *in_d, int dtype, const int orig[3], const int steps[3],
                   int wavelet, int max_possible, int compress_zeros, wv_opt **out,

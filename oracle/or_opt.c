@@ -48,7 +48,7 @@ int or_opt_sorted(const void *in, int dtype, const int orig[3], const int steps_
 
 int64_t or_opt_test(const void *in, int dtype, const int orig[3], const int steps_in[3],
                     int wavelet, int max_possible, int compress_zeros, float threshold,
-                    int bin_count, int quant_alg, or_error_acc *acc) {
+                    int bin_count, int quant_alg, or_error_acc *acc, int *lloyd_iters) {
   int steps[3], size[3];
   size_t n;
   float *data = forward(in, dtype, orig, steps_in, wavelet, steps, size, &n);
@@ -58,10 +58,39 @@ int64_t or_opt_test(const void *in, int dtype, const int orig[3], const int step
     for (size_t i = 0; i < n; i++)
       if (fabsf(data[i]) <= threshold) data[i] = 0;
   } else {
-    float mabs = 0;
-    for (size_t i = 0; i < n; i++) if (fabsf(data[i]) > mabs) mabs = fabsf(data[i]);
+    float mabs = 0, mn = 3.40282347e+38f, mx = 1.17549435e-38f;
+    for (size_t i = 0; i < n; i++) {
+      if (fabsf(data[i]) > mabs) mabs = fabsf(data[i]);
+      if (data[i] < mn) mn = data[i];
+      if (data[i] > mx) mx = data[i];
+    }
     or_quantizer q;
-    or_quant_init(&q, quant_alg, bin_count, threshold, mabs, NULL, NULL);
+    float *bounds = NULL, *values = NULL;
+    if (quant_alg == OR_QUANT_LLOYD) {
+      /* optimize_cpu.cc:90-96: the sorted |x| from findSorted(threshold)
+         (std::lower_bound, optimize_cpu.cc:16-18) to computeLloydQuantization,
+         with the transformed data's min and max (thresh_cpu.cc:67-68 starts
+         them at FLT_MAX / FLT_MIN) */
+      float *sorted = (float *)malloc(sizeof(float) * n);
+      for (size_t i = 0; i < n; i++) sorted[i] = fabsf(data[i]);
+      qsort(sorted, n, sizeof(float), cmp_float);
+      size_t lo = 0, len = n;
+      while (len > 0) {
+        size_t half = len >> 1;
+        if (sorted[lo + half] < threshold) { lo += half + 1; len -= half + 1; }
+        else len = half;
+      }
+      bounds = (float *)malloc(sizeof(float) * bin_count);
+      values = (float *)malloc(sizeof(float) * bin_count);
+      int iters = 0;
+      or_compute_lloyd_quantization(sorted + lo, (int)(n - lo), bin_count, mn, mx, threshold,
+                                    bounds, values, -1, &iters);
+      if (lloyd_iters) *lloyd_iters = iters;
+      free(sorted);
+      or_quant_init(&q, quant_alg, bin_count, threshold, 0, bounds, values);
+    } else {
+      or_quant_init(&q, quant_alg, bin_count, threshold, mabs, NULL, NULL);
+    }
     int zero_bin = or_quant(&q, 0);
     int32_t *bins = (int32_t *)malloc(sizeof(int32_t) * n);
     or_quantize(&q, data, bins, (int)n);
@@ -70,6 +99,8 @@ int64_t or_opt_test(const void *in, int dtype, const int orig[3], const int step
     or_huff_free(h);
     or_dequantize(&q, bins, data, (int)n);
     free(bins);
+    free(bounds);
+    free(values);
   }
   float *tmp = (float *)malloc(sizeof(float) * n);
   or_dwt3d(data, tmp, size, wavelet, steps, 1);

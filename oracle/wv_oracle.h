@@ -228,12 +228,14 @@ int or_split(const int size[3], const int cub[3], int overlap, int *boxes, int c
 /* one test point of the parameter optimisation (testParameters,
    optimize_gpu.cu:52-202, with the CPU path's double error sums): forward
    transform of the original, then bin_count <= 0: |x| <= threshold -> 0;
-   else quantize (uniform / log, max = max |x|), Huffman size
+   else quantize (uniform / log, max = max |x|; Lloyd: computeLloydQuantization
+   over the sorted |x| from findSorted(threshold), no iteration cap, its
+   iterations to *lloyd_iters when non-null), Huffman size
    (totalEncodedLengthBytes), dequantize; inverse; error over the original
    extent.  Returns the size in bytes (0 without quantization). */
 int64_t or_opt_test(const void *in, int dtype, const int orig[3], const int steps[3],
                     int wavelet, int max_possible, int compress_zeros, float threshold,
-                    int bin_count, int quant_alg, or_error_acc *acc);
+                    int bin_count, int quant_alg, or_error_acc *acc, int *lloyd_iters);
 /* the sorted |x| of the transformed data (getSorted's array), n entries */
 int or_opt_sorted(const void *in, int dtype, const int orig[3], const int steps[3],
                   int wavelet, float *sorted_out, int64_t cap);

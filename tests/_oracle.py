@@ -86,7 +86,8 @@ def _declare(L):
     L.or_set_steps.argtypes = [P, P]
     L.or_split.argtypes = [P, P, I, P, I, P, I, P]
     L.or_opt_test.restype = C.c_int64
-    L.or_opt_test.argtypes = [P, I, P, P, I, I, I, F, I, I, C.POINTER(ErrorAcc)]
+    L.or_opt_test.argtypes = [P, I, P, P, I, I, I, F, I, I, C.POINTER(ErrorAcc),
+                              C.POINTER(I)]
     L.or_opt_sorted.argtypes = [P, I, P, P, I, P, C.c_int64]
     L.or_pad_convert.argtypes = [P, I, P, P, P]
     for f in ("or_cdf97_fwd", "or_cdf97_inv", "or_haar_fwd", "or_haar_inv"):
@@ -312,15 +313,19 @@ def huff_decode(table, dup_key, dup_value, words, count):
     return out[:n]
 
 
-def opt_test(vol, steps, wavelet, max_possible, compress_zeros, threshold, bins, alg):
-    """One parameter-optimisation test point: (size bytes, mse, psnr, acc)."""
+def opt_test(vol, steps, wavelet, max_possible, compress_zeros, threshold, bins, alg,
+             want_iters=False):
+    """One parameter-optimisation test point: (size bytes, mse, psnr, acc)
+    (and the Lloyd iterations with want_iters)."""
     v = np.ascontiguousarray(vol)
     orig = [v.shape[2], v.shape[1], v.shape[0]]
     acc = ErrorAcc()
+    it = I(0)
     nb = lib().or_opt_test(_p(v), dtype_id(v), _p(i3(orig)), _p(i3(steps)), wavelet,
                            max_possible, int(compress_zeros), F(threshold), bins, alg,
-                           C.byref(acc))
-    return nb, lib().or_error_mse(C.byref(acc)), lib().or_error_psnr(C.byref(acc)), acc
+                           C.byref(acc), C.byref(it))
+    out = (nb, lib().or_error_mse(C.byref(acc)), lib().or_error_psnr(C.byref(acc)), acc)
+    return out + (it.value,) if want_iters else out
 
 
 def opt_sorted(vol, steps, wavelet):

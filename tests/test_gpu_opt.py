@@ -100,3 +100,37 @@ def test_plot_all_rows_match_oracle(wv):
         wsize, _, wpsnr, _ = O.opt_test(vol, [3, 3, 3], 2, 0, True, t, b, 2)
         assert size == wsize and _close(psnr, wpsnr), (frac, b)
     o.close()
+
+
+@pytest.mark.parametrize("shape,dtype,wave,steps,maxp", CASES)
+@pytest.mark.parametrize("frac,bins,zeros", [(0.9, 255, True), (0.6, 50, False),
+                                             (0.95, 1001, True), (0.8, 64, True)])
+def test_lloyd_point_matches_oracle(wv, shape, dtype, wave, steps, maxp, frac, bins, zeros):
+    """A Lloyd test point (the fixedPSNR sweep's quantizer): the code book
+    trained on the sorted |x| from findSorted(T), uncapped; iterations and
+    size exact, errors to 1e-6."""
+    vol = _vol(shape, dtype, 4)
+    o = wv.Opt(torch.from_numpy(vol).cuda(), steps, wave, maxp, compress_zeros=zeros)
+    t = o.sorted(int(np.float32(o.count()) * np.float32(frac)))
+    r = o.test(t, bins, wv.LLOYD)
+    ov = vol.astype(np.int32) if dtype == np.int16 else vol
+    size, mse, psnr, acc, iters = O.opt_test(ov, steps, wave, maxp, zeros, t, bins, 3,
+                                             want_iters=True)
+    assert r.lloyd_iters == iters and iters > 0
+    assert r.size_bytes == size
+    assert _close(r.mse, mse) and _close(r.psnr, psnr)
+    o.close()
+
+
+def test_fixed_psnr_sweep_matches_oracle(wv):
+    """fixedPSNR (optimize.cc:53-90) on a small image: every point it tests
+    against the oracle's test point."""
+    vol = _vol((1, 64, 96), np.uint8, 6)
+    o = wv.Opt(torch.from_numpy(vol).cuda(), [4, 4, 0], 2)
+    rows = wv.opt_fixed_psnr(o, 40.0)
+    assert rows and rows[0][1] == 0
+    for frac, b, size, psnr in rows[:6] + rows[-3:]:
+        t = o.sorted(int(np.float32(o.count()) * np.float32(1 - np.float32(frac))))
+        wsize, _, wpsnr, _ = O.opt_test(vol, [4, 4, 0], 2, 0, True, t, b, 3)
+        assert size == wsize and _close(psnr, wpsnr), (frac, b)
+    o.close()

@@ -97,7 +97,7 @@ class SplitBox(C.Structure):
 class OptResult(C.Structure):
     _fields_ = [("size_bytes", C.c_int64), ("l1", C.c_float), ("l2", C.c_float),
                 ("mse", C.c_float), ("psnr", C.c_float), ("rel", C.c_float),
-                ("zero_bin", C.c_int)]
+                ("zero_bin", C.c_int), ("lloyd_iters", C.c_int)]
 
 
 class SplitInfo(C.Structure):
@@ -687,7 +687,9 @@ class Opt:
         return check_size(lib().wv_opt_find_sorted(self.h, C.c_float(value)))
 
     def test(self, threshold, bins, alg=LOG, stream=None):
-        """testParameters: OptResult (size_bytes, l1, l2, mse, psnr, rel)."""
+        """testParameters: OptResult (size_bytes, l1, l2, mse, psnr, rel,
+        zero_bin, lloyd_iters); alg LLOYD trains the code book on the sorted
+        |x| above the threshold."""
         r = OptResult()
         check(lib().wv_opt_test(self.h, C.c_float(threshold), int(bins), int(alg), C.byref(r),
                                 _stream(stream)))
@@ -721,6 +723,32 @@ def opt_plot_all(o, alg=LOG, stream=None):
             rows.append((float(frac), b, int(r.size_bytes), float(r.psnr)))
             b = int(b * 1.1)
         frac = np.float32(frac + np.float32(0.01))
+    return rows
+
+
+def opt_fixed_psnr(o, target_psnr, stream=None):
+    """fixedPSNR (optimize.cc:53-90), the reference's Lloyd sweep: for kept
+    fractions .40, .35, ... (float steps, above .04) the threshold-only
+    PSNR at threshold getSorted(count * (1 - frac)); where that reaches the
+    target, Lloyd bin counts 50, 55, 60, ... (x1.1 truncated, up to 30000)
+    until one exceeds it.  Rows (kept fraction, bins (0: threshold only),
+    size, psnr) in the order tested."""
+    rows = []
+    n = o.count()
+    frac = np.float32(0.40)
+    while frac > np.float32(0.04):
+        t = o.sorted(int(np.float32(n) * np.float32(np.float32(1) - frac)))
+        r = o.test(t, 0, LLOYD, stream)
+        rows.append((float(frac), 0, int(r.size_bytes), float(r.psnr)))
+        if r.psnr >= target_psnr:
+            b = 50
+            while b <= 30000:
+                r = o.test(t, b, LLOYD, stream)
+                rows.append((float(frac), b, int(r.size_bytes), float(r.psnr)))
+                if r.psnr > target_psnr:
+                    break
+                b = int(b * 1.1)
+        frac = np.float32(np.float64(frac) - 0.05)   # float -= double literal
     return rows
 
 

@@ -115,6 +115,11 @@ void lloyd_batcher_stats(LloydBatcher *b, int64_t *windows, int64_t *run_windows
 int lloyd_run(const float *coef_d, int64_t n, const wv_thresh_result *res, int bin_count,
               int max_iters, void *ws_d, float *bin_boundaries, float *bin_values,
               int *iters_out, hipStream_t s, LloydBatcher *batcher);
+// Lloyd over points already sorted on the device (the optimiser's test
+// points): nonzero[0 .. count-1] ascending, ws of wv_lloyd_ws_bytes(n_ws, bins)
+int lloyd_run_sorted(const float *nonzero, int64_t count, float threshold, int bin_count,
+                     int max_iters, void *ws_d, int64_t n_ws, float min_val, float max_val,
+                     float *bin_boundaries, float *bin_values, int *iters_out, hipStream_t s);
 // wv_compress_device with an optional Lloyd batcher (wv_compress.cpp)
 int compress_device(const void *in_d, int dtype, const int orig[3], const wv_options *opt,
                     void *work_d, int64_t work_bytes, float *codebook_out,

@@ -653,6 +653,10 @@ struct LloydRun {
   LloydState hs;
 };
 
+static int lloyd_setup_points(const float *nonzero, int64_t count, float threshold,
+                              int bin_count, int max_iters, void *ws_d, int64_t n_ws,
+                              LloydRun &R, hipStream_t s);
+
 // points, initial codebook, initial assignment and distortion; the state
 // comes back to R.hs (R.hs.done: the loop never runs)
 static int lloyd_setup(const float *coef_d, int64_t n, const wv_thresh_result *res,
@@ -689,6 +693,31 @@ static int lloyd_setup(const float *coef_d, int64_t n, const wv_thresh_result *r
                        res->threshold_key);
   if (ngt > 0)
     LCHK(hipMemcpyAsync(tail + ndup, keys, ngt * 4, hipMemcpyDeviceToDevice, s), "tail copy");
+  return lloyd_setup_points((const float *)tail, nz, res->threshold, bin_count, max_iters,
+                            ws_d, n, R, s);
+}
+
+// The set-up from sorted points: nonzero[0 .. count-1] ascending, the run
+// over nonzero[0 .. count-2] (computeLloydQuantization,
+// test_compress_cpu.cc:404-459: the largest value is left out), initial
+// codebook log-spaced from the threshold to nonzero[count-2], initial
+// assignment and distortion; the state comes back to R.hs.  ws_d is laid
+// out for n_ws samples (n_ws >= count).
+static int lloyd_setup_points(const float *nonzero, int64_t count, float threshold,
+                              int bin_count, int max_iters, void *ws_d, int64_t n_ws,
+                              LloydRun &R, hipStream_t s) {
+  const int cs = (bin_count - 1) / 2;
+  const int64_t psize = count - 1;
+  if (cs < 2 || cs > LL_MAX_CS || psize < 2 || psize > 0x7fffffff) {
+    set_error("lloyd: %lld points / %d codebook entries unsupported", (long long)psize, cs);
+    return WV_ERR_UNSUPPORTED;
+  }
+  const LWs L = lloyd_layout(n_ws, bin_count);
+  char *w = (char *)ws_d;
+  LloydState *S = (LloydState *)(w + L.state);
+  hipLaunchKernelGGL(ll_state_init, dim3(1), dim3(1), 0, s, S, (int)psize, cs, max_iters);
+  const float *tail = nonzero;
+  const int64_t nz = count;
 
   LJob &J = R.job;
   J = LJob{};
@@ -720,7 +749,7 @@ static int lloyd_setup(const float *coef_d, int64_t n, const wv_thresh_result *r
   LCHK(hipMemcpyAsync(&max2, tail + (nz - 2), 4, hipMemcpyDeviceToHost, s), "D2H max");
   LCHK(hipStreamSynchronize(s), "sync");
   std::vector<float> hcb(cs);
-  initial_codebook(hcb.data(), cs, res->threshold, max2);
+  initial_codebook(hcb.data(), cs, threshold, max2);
   LCHK(hipMemcpyAsync(J.cb, hcb.data(), cs * 4, hipMemcpyHostToDevice, s), "H2D codebook");
   LCHK(hipMemcpyAsync(R.job_d, &J, sizeof J, hipMemcpyHostToDevice, s), "H2D run table");
 
@@ -1007,6 +1036,32 @@ int lloyd_run(const float *coef_d, int64_t n, const wv_thresh_result *res, int b
   if (iters_out) *iters_out = R.hs.iters;
   return bins_from_codebook(bin_values, bin_boundaries, bin_count, fin.data(), cs,
                             res->threshold, res->min_val, res->max_val);
+}
+
+// Lloyd over points already sorted on the device: nonzero[0 .. count-1]
+// ascending, the run over all but the largest (the optimiser's test points,
+// which take nonzero = the sorted |x| from findSorted(threshold)).  ws_d:
+// wv_lloyd_ws_bytes(n_ws, bin_count) with n_ws >= count.
+int lloyd_run_sorted(const float *nonzero, int64_t count, float threshold, int bin_count,
+                     int max_iters, void *ws_d, int64_t n_ws, float min_val, float max_val,
+                     float *bin_boundaries, float *bin_values, int *iters_out, hipStream_t s) {
+  LloydRun R;
+  const int rc = lloyd_setup_points(nonzero, count, threshold, bin_count, max_iters, ws_d, n_ws,
+                                    R, s);
+  if (rc != WV_OK) return rc;
+  if (!R.hs.done) {
+    const int lrc = lloyd_loop(R, s);
+    if (lrc != WV_OK) return lrc;
+  }
+  const int cs = R.job.cs;
+  const float *final_cb =
+      R.hs.iters > 0 ? R.job.cbring + (int64_t)R.hs.final_slot * (cs + 1) : R.job.cb;
+  std::vector<float> fin(cs);
+  LCHK(hipMemcpyAsync(fin.data(), final_cb, cs * 4, hipMemcpyDeviceToHost, s), "D2H codebook");
+  LCHK(hipStreamSynchronize(s), "sync");
+  if (iters_out) *iters_out = R.hs.iters;
+  return bins_from_codebook(bin_values, bin_boundaries, bin_count, fin.data(), cs, threshold,
+                            min_val, max_val);
 }
 
 }  // namespace wv

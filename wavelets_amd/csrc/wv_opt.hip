@@ -128,6 +128,8 @@ struct wv_opt {
   int64_t enc_cap = 0;   // enc_ws: its own buffer, grown to what a test needs
   double *partial;
   double *sums_h = nullptr;   // pinned: the four error sums
+  char *lloyd_ws = nullptr;   // Lloyd test points: wv_lloyd_ws_bytes(n, bins), grown
+  int64_t lloyd_cap = 0;
   size_t sort_bytes;
   int64_t table_cap;
   float min_val, max_val, max_abs;
@@ -238,6 +240,7 @@ extern "C" int wv_opt_create(const void *in_d, int dtype, const int orig[3], con
       hipHostMalloc((void **)&o->sums_h, 4 * sizeof(double), 0) != hipSuccess) {
     if (o->arena) (void)hipFree(o->arena);
     if (o->enc_ws) (void)hipFree(o->enc_ws);
+  if (o->lloyd_ws) (void)hipFree(o->lloyd_ws);
     if (o->sums_h) (void)hipHostFree(o->sums_h);
     delete o;
     set_error("wv_opt_create: %lld device bytes", (long long)off);
@@ -332,7 +335,8 @@ extern "C" int64_t wv_opt_find_sorted(const wv_opt *o, float value) {
 }
 
 // testParameters (optimize_gpu.cu:52-202) for the uniform and log
-// quantizers (max = max |x|, as compress does), or no quantization when
+// quantizers (max = max |x|, as compress does), the Lloyd code book trained
+// on the sorted |x| above the threshold, or no quantization when
 // bin_count <= 0
 extern "C" int wv_opt_test(wv_opt *o, float threshold, int bin_count, int quant_alg,
                            wv_opt_result *r, void *stream) {
@@ -353,8 +357,9 @@ extern "C" int wv_opt_test(wv_opt *o, float threshold, int bin_count, int quant_
     OCHK(error_sums(o, s));
     OHIP(hipStreamSynchronize(s), "wv_opt_test: sync");
   } else {
-    if (quant_alg != WV_QUANT_UNIFORM && quant_alg != WV_QUANT_LOG) {
-      set_error("wv_opt_test: quantizer %d (uniform or log)", quant_alg);
+    if (quant_alg != WV_QUANT_UNIFORM && quant_alg != WV_QUANT_LOG &&
+        quant_alg != WV_QUANT_LLOYD) {
+      set_error("wv_opt_test: quantizer %d", quant_alg);
       return WV_ERR_UNSUPPORTED;
     }
     if (bin_count > WV_MAX_BINS) {
@@ -362,8 +367,35 @@ extern "C" int wv_opt_test(wv_opt *o, float threshold, int bin_count, int quant_
       return WV_ERR_ARG;
     }
     wv_quant_params p;
-    OCHK(wv_quant_params_init(&p, quant_alg, bin_count, threshold, o->max_abs, nullptr,
-                              nullptr));
+    if (quant_alg == WV_QUANT_LLOYD) {
+      // computeLloydQuantization over the sorted |x| from findSorted(T)
+      // (optimize_cpu.cc:90-96): all but the largest, no iteration cap
+      const int64_t first = wv_opt_find_sorted(o, threshold);
+      const int64_t need = wv_lloyd_ws_bytes(n, bin_count);
+      if (need > o->lloyd_cap) {
+        if (hipStreamSynchronize(s) != hipSuccess) {
+          set_error("wv_opt_test: sync");
+          return WV_ERR_HIP;
+        }
+        (void)hipFree(o->lloyd_ws);
+        o->lloyd_ws = nullptr;
+        o->lloyd_cap = 0;
+        if (hipMalloc((void **)&o->lloyd_ws, (size_t)need) != hipSuccess) {
+          set_error("wv_opt_test: Lloyd workspace");
+          return WV_ERR_NOMEM;
+        }
+        o->lloyd_cap = need;
+      }
+      std::vector<float> bounds((size_t)bin_count - 1), values((size_t)bin_count);
+      OCHK(lloyd_run_sorted((const float *)(o->sorted + first), n - first, threshold, bin_count,
+                            -1, o->lloyd_ws, n, o->min_val, o->max_val, bounds.data(),
+                            values.data(), &r->lloyd_iters, s));
+      OCHK(wv_quant_params_init(&p, quant_alg, bin_count, threshold, 0.f, bounds.data(),
+                                values.data()));
+    } else {
+      OCHK(wv_quant_params_init(&p, quant_alg, bin_count, threshold, o->max_abs, nullptr,
+                                nullptr));
+    }
     int rc = wv_quant_table_bytes(&p) <= o->table_cap ? WV_OK : WV_ERR_ARG;
     if (rc == WV_OK) rc = wv_quant_params_upload(&p, o->table, stream);
     if (rc == WV_OK) rc = wv_quantize_hist(o->coef, n, &p, o->bins, o->counts, stream);
@@ -426,6 +458,7 @@ extern "C" int wv_opt_destroy(wv_opt *o) {
   if (!o) return WV_OK;
   if (o->arena) (void)hipFree(o->arena);
   if (o->enc_ws) (void)hipFree(o->enc_ws);
+  if (o->lloyd_ws) (void)hipFree(o->lloyd_ws);
   if (o->sums_h) (void)hipHostFree(o->sums_h);
   delete o;
   return WV_OK;
