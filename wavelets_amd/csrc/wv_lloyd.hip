@@ -250,12 +250,21 @@ __device__ inline bool ll_skip(const LJob &j, int k) {
   return j.S->done || (k >= 0 && k >= j.nw);
 }
 
-// assignment of every point (ring: a 16-bit copy of the nearest groups,
-// kept for the distortion chain of window iteration k, which runs later)
+// Assignment of every point and its run boundaries in one pass (ring: a
+// 16-bit copy of the nearest groups, kept for the distortion chain of
+// window iteration k, which runs later).  A block takes 256 consecutive
+// points at a time; a point starts a run of its group when the point
+// before it is in another group (or it is the first point) and ends one
+// when the point after it is: the neighbours' groups come from LDS, and at
+// the chunk's two ends they are computed again from the points themselves.
+// A group whose members are contiguous has exactly one run.  The run
+// arrays must be cleared first (ll_clear_runs, or the update kernels).
+#define LL_NOGROUP (-2147483647 - 1)
 template <bool NEAREST>
 __global__ void __launch_bounds__(WV_BLOCK)
 ll_assign(const LJob *J, int k) {
   extern __shared__ float cb[];
+  __shared__ int sg[WV_BLOCK + 2];
   const LJob j = J[blockIdx.y];
   if (ll_skip(j, k)) return;
   const int cs = j.cs;
@@ -263,36 +272,45 @@ ll_assign(const LJob *J, int k) {
   __syncthreads();
   uint16_t *ring = NEAREST && k >= 0 ? j.ring + (int64_t)k * j.ring_stride : nullptr;
   const int psize = j.psize;
-  for (int i = blockIdx.x * WV_BLOCK + threadIdx.x; i < psize; i += gridDim.x * WV_BLOCK) {
+  auto group = [&](int i) {
     const float p = j.pts[i];
-    const int g = NEAREST ? nearest(cb, cs, p) : between(cb, cs, p);
-    j.grp[i] = g;
-    if (ring) ring[i] = (uint16_t)g;
+    return NEAREST ? nearest(cb, cs, p) : between(cb, cs, p);
+  };
+  const int tid = threadIdx.x;
+  for (int base = blockIdx.x * WV_BLOCK; base < psize; base += gridDim.x * WV_BLOCK) {
+    const int i = base + tid;
+    int g = LL_NOGROUP;
+    if (i < psize) {
+      g = group(i);
+      j.grp[i] = g;
+      if (ring) ring[i] = (uint16_t)g;
+    }
+    sg[tid + 1] = g;
+    if (tid == 0) sg[0] = base > 0 ? group(base - 1) : LL_NOGROUP;
+    if (tid == WV_BLOCK - 1) sg[WV_BLOCK + 1] = base + WV_BLOCK < psize ? group(base + WV_BLOCK)
+                                                                     : LL_NOGROUP;
+    __syncthreads();
+    if (i < psize && g >= 0) {
+      if (sg[tid] != g) { j.first[g] = i; atomicAdd(&j.nruns[g], 1); }
+      if (sg[tid + 2] != g) j.last[g] = i;
+    }
+    __syncthreads();
   }
 }
 
-// run boundaries: a group whose members are contiguous has exactly one run
-__global__ void __launch_bounds__(WV_BLOCK)
-ll_runs(const LJob *J, int k) {
-  const LJob j = J[blockIdx.y];
-  if (ll_skip(j, k)) return;
-  const int psize = j.psize;
-  for (int i = blockIdx.x * WV_BLOCK + threadIdx.x; i < psize; i += gridDim.x * WV_BLOCK) {
-    const int g = j.grp[i];
-    if (g < 0) continue;
-    if (i == 0 || j.grp[i - 1] != g) { j.first[g] = i; atomicAdd(&j.nruns[g], 1); }
-    if (i == psize - 1 || j.grp[i + 1] != g) j.last[g] = i;
+// the run arrays the next assignment fills
+__device__ inline void clear_runs(const LJob &j) {
+  for (int t = threadIdx.x; t < j.cs; t += blockDim.x) {
+    j.nruns[t] = 0;
+    j.first[t] = 0;
+    j.last[t] = -1;
   }
 }
 
 __global__ void ll_clear_runs(const LJob *J, int k) {
   const LJob j = J[blockIdx.y];
   if (ll_skip(j, k)) return;
-  for (int t = threadIdx.x; t < j.cs; t += blockDim.x) {
-    j.nruns[t] = 0;
-    j.first[t] = 0;
-    j.last[t] = -1;
-  }
+  clear_runs(j);
 }
 
 // one wavefront per chain: chains 0..cs-1 are the cells, chain cs the
@@ -476,24 +494,34 @@ __global__ void ll_init_finish(const LJob *J) {
   }
 }
 
-// loop head + codebook update (lloyds.cpp:69-87)
+// loop head + codebook update (lloyds.cpp:69-87), window iteration k >= 0;
+// clears the runs for the partition pass that follows
 __global__ void ll_update_codebook(const LJob *J, int k) {
   const LJob j = J[blockIdx.y];
   if (ll_skip(j, k)) return;
+  clear_runs(j);
   const LloydState *S = j.S;
   const int cs = j.cs;
+  // the iteration's codebook also goes to its window slot (nothing changes
+  // it later in the iteration)
+  float *snap = j.cbring + (int64_t)k * (cs + 1);
   for (int i = threadIdx.x; i < cs; i += blockDim.x) {
-    if (j.count[i] != 0) j.cb[i] = j.sum[i] / (float)j.count[i];
-    else if (i == 0) j.cb[i] = (j.part[0] + S->pmin) / 2;
-    else if (i == cs - 1) j.cb[i] = (j.part[i - 1] + S->pmax) / 2;
-    else j.cb[i] = (j.part[i - 1] + j.part[i]) / 2;
+    float c;
+    if (j.count[i] != 0) c = j.sum[i] / (float)j.count[i];
+    else if (i == 0) c = (j.part[0] + S->pmin) / 2;
+    else if (i == cs - 1) c = (j.part[i - 1] + S->pmax) / 2;
+    else c = (j.part[i - 1] + j.part[i]) / 2;
+    j.cb[i] = c;
+    snap[i] = c;
   }
 }
 
-// partition update (lloyds.cpp:102-111)
+// partition update (lloyds.cpp:102-111); clears the runs for the nearest
+// pass that follows (the cell sums it reads were taken already)
 __global__ void ll_update_partition(const LJob *J, int k) {
   const LJob j = J[blockIdx.y];
   if (ll_skip(j, k)) return;
+  clear_runs(j);
   const int cs = j.cs;
   for (int i = threadIdx.x; i < cs - 1; i += blockDim.x) {
     if (j.count[i] != 0) j.part[i] = j.sum[i] / (float)j.count[i];
@@ -511,12 +539,6 @@ __global__ void ll_update_partition(const LJob *J, int k) {
 // one thread applies the loop test to them in order: the first iteration
 // that stops the loop is the result, the later ones of the window are
 // dropped.
-__global__ void ll_snapshot(const LJob *J, int k) {
-  const LJob j = J[blockIdx.y];
-  if (ll_skip(j, k)) return;
-  float *dst = j.cbring + (int64_t)k * (j.cs + 1);
-  for (int i = threadIdx.x; i < j.cs; i += blockDim.x) dst[i] = j.cb[i];
-}
 
 __global__ void __launch_bounds__(64)
 ll_window_dist(const LJob *J) {
@@ -756,9 +778,8 @@ static int lloyd_setup_points(const float *nonzero, int64_t count, float thresho
   // 3. initial assignment (lloyds.cpp:35-67): nearest groups, their cell
   // sums and the distortion
   const size_t cb_lds = (size_t)cs * 4;
-  hipLaunchKernelGGL(ll_assign<true>, dim3(R.ag), dim3(WV_BLOCK), cb_lds, s, R.job_d, -1);
   hipLaunchKernelGGL(ll_clear_runs, dim3(1), dim3(256), 0, s, R.job_d, -1);
-  hipLaunchKernelGGL(ll_runs, dim3(R.ag), dim3(WV_BLOCK), 0, s, R.job_d, -1);
+  hipLaunchKernelGGL(ll_assign<true>, dim3(R.ag), dim3(WV_BLOCK), cb_lds, s, R.job_d, -1);
   hipLaunchKernelGGL(ll_chains<true>, dim3(cs + 1), dim3(64), cb_lds, s, R.job_d, -1);
   hipLaunchKernelGGL(ll_init_finish, dim3(1), dim3(256), 0, s, R.job_d);
   LCHK(hipMemcpyAsync(&R.hs, S, sizeof R.hs, hipMemcpyDeviceToHost, s), "D2H state");
@@ -779,12 +800,11 @@ static void enqueue_window(const LJob *J, int njobs, int max_cs, int max_ag, int
                            hipStream_t s) {
   const size_t cb_lds = (size_t)max_cs * 4;
   auto assign_and_chains = [&](bool nearest_pass, int k) {
+    // the update kernel before each assignment has cleared its runs
     if (nearest_pass)
       hipLaunchKernelGGL(ll_assign<true>, dim3(max_ag, njobs), dim3(WV_BLOCK), cb_lds, s, J, k);
     else
       hipLaunchKernelGGL(ll_assign<false>, dim3(max_ag, njobs), dim3(WV_BLOCK), cb_lds, s, J, k);
-    hipLaunchKernelGGL(ll_clear_runs, dim3(1, njobs), dim3(256), 0, s, J, k);
-    hipLaunchKernelGGL(ll_runs, dim3(max_ag, njobs), dim3(WV_BLOCK), 0, s, J, k);
     hipLaunchKernelGGL(ll_chains<false>, dim3(max_cs, njobs), dim3(64), 0, s, J, k);
   };
   for (int k = 0; k < W; k++) {
@@ -792,7 +812,6 @@ static void enqueue_window(const LJob *J, int njobs, int max_cs, int max_ag, int
     assign_and_chains(false, k);
     hipLaunchKernelGGL(ll_update_partition, dim3(1, njobs), dim3(256), 0, s, J, k);
     assign_and_chains(true, k);
-    hipLaunchKernelGGL(ll_snapshot, dim3(1, njobs), dim3(256), 0, s, J, k);
   }
   hipLaunchKernelGGL(ll_window_dist, dim3(W, njobs), dim3(64), cb_lds, s, J);
   hipLaunchKernelGGL(ll_window_eval, dim3(1, njobs), dim3(1), 0, s, J);
