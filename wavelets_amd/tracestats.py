@@ -5,6 +5,7 @@ the profile notes quote).
     python3 -m wavelets_amd.tracestats submit   <kernel_trace.csv>
     python3 -m wavelets_amd.tracestats busy     <kernel_trace.csv> <seconds>
     python3 -m wavelets_amd.tracestats counters <fetch_dir> <write_dir> [copy_bytes]
+    python3 -m wavelets_amd.tracestats pmc      <counter_dir>
 
 kernels   per-kernel calls, total and average duration, share of the total;
           the select's radix rounds split by round (launch order mod 3);
@@ -15,7 +16,9 @@ busy      the last <seconds> of the trace: time with at least one kernel
           running, kernels in flight while busy, the busiest kernels;
 counters  FETCH_SIZE / WRITE_SIZE per dispatch (two --pmc runs), scaled by
           what they read for the library's own copy kernel (wv_copy_stream)
-          of copy_bytes bytes, and the DWT passes' traffic per launch.
+          of copy_bytes bytes, and the DWT passes' traffic per launch;
+pmc       every counter of one --pmc run, mean per dispatch, for the DWT
+          passes, the collect, the quantize and the copy kernel.
 """
 import collections
 import csv
@@ -147,6 +150,24 @@ def counters(fetch_dir, write_dir, copy_bytes=1 << 30):
     return "\n".join(out)
 
 
+def pmc(counter_dir):
+    f = glob.glob(os.path.join(counter_dir, "*counter_collection.csv"))[0]
+    agg = collections.defaultdict(lambda: collections.defaultdict(list))
+    for r in csv.DictReader(open(f)):
+        agg[r["Kernel_Name"].split("(")[0]][r["Counter_Name"]].append(float(r["Counter_Value"]))
+    names = sorted({c for d in agg.values() for c in d})
+    out = ["| kernel | dispatches | " + " | ".join(names) + " |",
+           "|---|---|" + "---|" * len(names)]
+    keep = ("dwt_pass", "copy_kernel", "quant_hist", "sel_collect")
+    for k in sorted(agg):
+        if any(t in k for t in keep):
+            d = agg[k]
+            out.append("| `%s` | %d | %s |" % (
+                k[:80], max(len(v) for v in d.values()),
+                " | ".join("%.0f" % (sum(d[c]) / len(d[c])) if d.get(c) else "-" for c in names)))
+    return "\n".join(out)
+
+
 def main(argv):
     if len(argv) < 3:
         print(__doc__)
@@ -158,6 +179,8 @@ def main(argv):
         print(submit(argv[2]))
     elif cmd == "busy":
         print(busy(argv[2], float(argv[3])))
+    elif cmd == "pmc":
+        print(pmc(argv[2]))
     elif cmd == "counters":
         print(counters(argv[2], argv[3], int(argv[4]) if len(argv) > 4 else 1 << 30))
     else:
