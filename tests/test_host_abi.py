@@ -49,6 +49,18 @@ def test_geometry_matches_oracle(wv):
         assert list(g.out_size) == s
 
 
+def test_forward_pass_plan(wv):
+    # planes of at most 32 x 32 with x and y levels: one plane pass (x and y
+    # in LDS) plus the z pass; otherwise one pass per axis, a depth-1 plane
+    # needs no z pass, and no levels at all is one conversion pass
+    for orig, steps, n in [([32, 32, 32], [3, 3, 3], 2), ([33, 20, 17], [3, 3, 3], 2),
+                           ([32, 32, 1], [3, 3, 0], 1), ([32, 32, 32], [0, 3, 3], 2),
+                           ([64, 64, 64], [3, 3, 3], 3), ([512, 512, 16], [3, 3, 3], 3),
+                           ([2048, 2048, 1], [-1, -1, -1], 2), ([16, 16, 16], [0, 0, 0], 1)]:
+        g = wv.geom(orig, steps, wv.CDF97, wv.FLOAT32)
+        assert wv.lib().wv_dwt3d_num_passes(C.byref(g)) == n, (orig, steps)
+
+
 @pytest.mark.parametrize("alg", [1, 2])
 @pytest.mark.parametrize("bins,thr,mx", [(256, 1.161753893, 421519.75), (1024, 31.5, 40000.0),
                                          (255, 0.5, 10.0), (7, 1.0, 9.0), (2, 0.25, 4.0)])

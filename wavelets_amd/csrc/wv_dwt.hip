@@ -34,9 +34,14 @@
 // chunk then reads plain rows -- no per-element mirroring -- and the final
 // coefficients sit at row p (p < final low length) or p + 4.
 //
+// Small planes (at most 32 x 32 samples) take a plane pass instead of the x
+// and y passes: dwt_plane_kernel below, x and y levels of whole xy-planes in
+// LDS.
+//
 // Arithmetic follows the reference CPU code operation by operation with
 // contraction disabled (-ffp-contract=off), so coefficients are
 // bit-identical to cdf97 / haar (dwt_cpu.cc:139-203,683-799).
+#include <algorithm>
 #include <atomic>
 #include <cstdlib>
 #include "wv_device.h"
@@ -701,6 +706,274 @@ dwt_pass_kernel(DwtPass P, int ntiles) {
   store_tile<TT, EPT, NT>(P, t, tout, lb_st);
 }
 
+// ---- the plane pass ---------------------------------------------------------
+// x and y levels of PP consecutive xy-planes in one pass (planes of at most
+// 128 x 128; taken for planes of at most WV_DWT_PLANE_MAX samples, 32 x 32
+// by default: on larger planes the three axis passes are faster, since the
+// plane tile's two sets of levels make the pass latency-bound): the planes
+// are loaded into LDS (converted from the source
+// type, coordinates clamped into the unpadded source), every row is
+// transformed along x, then every column along y, and the planes are stored
+// back in x,y,z (the z pass then writes z,x,y).  Two HBM round trips for the
+// three axes instead of three.  Plane p's row y sits at LDS row
+// p*(Y+8) + 4 + y, its column x at 4 + x, with four spare rows above and
+// below each plane and four spare columns left and right of each row; the
+// row stride X + 9 is odd, so a wave's accesses along a row (y levels:
+// neighbouring lines are neighbouring columns) and across rows (x levels:
+// neighbouring lines are neighbouring rows) are both bank-conflict free.
+// The levels are forward97_levels / forward_haar_levels with a line base
+// and a position stride in place of the tile's column and row stride: the
+// same arithmetic, the same gap layout of the 9/7 bands.
+#define WV_PL_EPT 32   // tile elements per thread: PP*X*Y <= 512*32
+
+template <typename Base>
+__device__ void plane97_levels(float *lds, int nl, Base base, int ps, int n0, int levels) {
+  const int tid = opaque_tid();
+  const int line = tid % nl;
+  const int k = tid / nl;
+  const int tpl = WV_DBLOCK / nl;
+  float *col = lds + base(line);
+  int n = n0;
+  for (int s = 0; s < levels; s++) {
+    const int pairs = n >> 1;
+    const int cpt = (pairs + tpl - 1) / tpl;
+    const int j0 = k * cpt;
+    const int cnt = min(cpt, pairs - j0);
+    float lo[WV_PL_EPT / 2], hi[WV_PL_EPT / 2];
+    if (cnt > 0) {
+      const int c = 2 * j0;
+      // mirrored halo positions this chunk reads (MirroredArray,
+      // dwt_cpu.h:162-176)
+      if (c < 4) {
+#pragma unroll
+        for (int h = 1; h <= 4; h++) col[-h * ps] = col[wv_mirror(-h, n) * ps];
+      }
+      if (c + 2 * cnt + 2 >= n) {
+#pragma unroll
+        for (int h = 0; h < 3; h++) col[(n + h) * ps] = col[wv_mirror(n + h, n) * ps];
+      }
+      const float *cc = col + c * ps;
+      float w0 = cc[-4 * ps], w1 = cc[-3 * ps], w2 = cc[-2 * ps], w3 = cc[-ps],
+            w4 = cc[0], w5 = cc[ps], w6 = cc[2 * ps], w7 = cc[3 * ps], w8 = cc[4 * ps];
+#pragma unroll
+      for (int q = 0; q < (WV_PL_EPT / 2); q++) {
+        if (q < cnt) {
+          if (q > 0) {
+            w0 = w2; w1 = w3; w2 = w4; w3 = w5; w4 = w6; w5 = w7; w6 = w8;
+            w7 = cc[(2 * q + 3) * ps];
+            w8 = cc[(2 * q + 4) * ps];
+          }
+          // cdf97_tmpl dwt_cpu.cc:709-723
+          float sl = F0 * w4;
+          sl += F1 * (w5 + w3);
+          sl += F2 * (w6 + w2);
+          sl += F3 * (w7 + w1);
+          sl += F4 * (w8 + w0);
+          float sh = G0 * w5;
+          sh += G1 * (w6 + w4);
+          sh += G2 * (w7 + w3);
+          sh += G3 * (w8 + w2);
+          lo[q] = sl;
+          hi[q] = sh;
+        }
+      }
+    }
+    __syncthreads();
+    if (cnt > 0) {
+      const int lshift = s == levels - 1 ? WV_HALO : 0;
+      float *lw = col + (j0 + lshift) * ps, *hw = col + (pairs + WV_HALO + j0) * ps;
+#pragma unroll
+      for (int q = 0; q < (WV_PL_EPT / 2); q++) {
+        if (q < cnt) {
+          lw[q * ps] = lo[q];
+          hw[q * ps] = hi[q];
+        }
+      }
+    }
+    __syncthreads();
+    n = pairs;
+  }
+}
+
+template <typename Base>
+__device__ void plane_haar_levels(float *lds, int nl, Base base, int ps, int n0, int levels) {
+  const int tid = opaque_tid();
+  const int line = tid % nl;
+  const int k = tid / nl;
+  const int tpl = WV_DBLOCK / nl;
+  float *col = lds + base(line);
+  int n = n0;
+  for (int s = 0; s < levels; s++) {
+    const int pairs = n >> 1;
+    const int cpt = (pairs + tpl - 1) / tpl;
+    const int j0 = k * cpt;
+    const int cnt = min(cpt, pairs - j0);
+    float lo[WV_PL_EPT / 2], hi[WV_PL_EPT / 2];
+    if (cnt > 0) {
+      const float *cc = col + 2 * j0 * ps;
+#pragma unroll
+      for (int q = 0; q < (WV_PL_EPT / 2); q++) {
+        if (q < cnt) {
+          const float a = cc[(2 * q) * ps], b = cc[(2 * q + 1) * ps];
+          // haar_internal dwt_cpu.cc:159-162
+          hi[q] = (a - b) * INV_SQRT2;
+          lo[q] = (a + b) * INV_SQRT2;
+        }
+      }
+    }
+    __syncthreads();
+    if (cnt > 0) {
+      float *lw = col + j0 * ps, *hw = col + (pairs + j0) * ps;
+#pragma unroll
+      for (int q = 0; q < (WV_PL_EPT / 2); q++) {
+        if (q < cnt) {
+          lw[q * ps] = lo[q];
+          hw[q * ps] = hi[q];
+        }
+      }
+    }
+    __syncthreads();
+    n = pairs;
+  }
+}
+
+// tile -> (cubelet source, destination, first plane)
+struct PlTile {
+  const void *src;
+  float *dst;
+  int z0, np;   // first plane, planes in this tile
+};
+
+__device__ inline PlTile plane_tile(const DwtPass &P, int tile) {
+  PlTile t;
+  if (P.src_tab) {
+    const int n1 = opaque(P.ntiles1);
+    const int c = tile / n1;
+    tile -= c * n1;
+    t.src = P.src_tab[c];
+    t.dst = P.dst_tab[c];
+  } else {
+    t.src = P.src;
+    t.dst = P.dst;
+  }
+  t.z0 = tile * P.T;
+  t.np = min(P.T, P.Z - t.z0);
+  return t;
+}
+
+// element i of thread tid: flat index tid + i*NT of the tile's planes
+// (x fastest); division by X and X*Y through exact reciprocals
+__device__ inline void plane_xyz(const DwtPass &P, int idx, int &p, int &y, int &x) {
+  p = (int)__umulhi((unsigned)idx, P.magic_XY);
+  const int r = idx - p * opaque(P.X * P.Y);
+  y = (int)__umulhi((unsigned)r, P.magic_L);
+  x = r - y * opaque(P.X);
+}
+
+template <typename SRC>
+__device__ inline void plane_load(const DwtPass &P, const PlTile &t, float (&v)[WV_PL_EPT]) {
+  const WV_GLOBAL SRC *src = gptr((const SRC *)t.src);
+  const int64_t sy = P.src_sy > 0 ? P.src_sy : (int64_t)P.ox;
+  const int64_t sz = P.src_sz > 0 ? P.src_sz : (int64_t)P.ox * P.oy;
+  const int ne = t.np * P.X * P.Y;
+  const int tid = opaque_tid();
+#pragma unroll
+  for (int i = 0; i < WV_PL_EPT; i++) {
+    const int idx = tid + i * WV_DBLOCK;
+    if (idx < ne) {
+      int p, y, x;
+      plane_xyz(P, idx, p, y, x);
+      const int zc = min(t.z0 + p, P.oz - 1), yc = min(y, P.oy - 1), xc = min(x, P.ox - 1);
+      v[i] = to_f(src[zc * sz + yc * sy + xc]);
+    }
+  }
+}
+
+__device__ inline void plane_to_lds(const DwtPass &P, const PlTile &t, const float (&v)[WV_PL_EPT],
+                                    float *lds) {
+  const int ne = t.np * P.X * P.Y;
+  const int RS = P.X + 2 * WV_HALO + 1, PR = P.Y + 2 * WV_HALO;
+  const int tid = opaque_tid();
+#pragma unroll
+  for (int i = 0; i < WV_PL_EPT; i++) {
+    const int idx = tid + i * WV_DBLOCK;
+    if (idx < ne) {
+      int p, y, x;
+      plane_xyz(P, idx, p, y, x);
+      lds[(p * PR + WV_HALO + y) * RS + WV_HALO + x] = v[i];
+    }
+  }
+}
+
+__device__ inline void plane_store(const DwtPass &P, const PlTile &t, const float *lds, int xs,
+                                   int ys) {
+  WV_GLOBAL float *dst = gptr(t.dst);
+  const int XY = P.X * P.Y;
+  const int ne = t.np * XY;
+  const int RS = P.X + 2 * WV_HALO + 1, PR = P.Y + 2 * WV_HALO;
+  const int tid = opaque_tid();
+  const int64_t base = (int64_t)t.z0 * XY;
+#pragma unroll
+  for (int i = 0; i < WV_PL_EPT; i++) {
+    const int idx = tid + i * WV_DBLOCK;
+    if (idx < ne) {
+      int p, y, x;
+      plane_xyz(P, idx, p, y, x);
+      dst[base + idx] = lds[(p * PR + WV_HALO + y + ys) * RS + WV_HALO + x + xs];
+    }
+  }
+}
+
+// P.T planes per tile, P.levels x levels, P.levels2 y levels; persistent,
+// the next tile's loads in flight during this tile's levels and stores.
+template <int WAVE, typename SRC>
+__global__ void __launch_bounds__(WV_DBLOCK, WV_DWAVES)
+dwt_plane_kernel(DwtPass P, int ntiles) {
+  extern __shared__ float smem[];
+  const int X = P.X, Y = P.Y;
+  const int RS = X + 2 * WV_HALO + 1, PR = Y + 2 * WV_HALO;
+  const int xs = WAVE == WV_WAVELET_CDF97 && P.levels > 0 ? WV_HALO : 0;
+  const int ys = WAVE == WV_WAVELET_CDF97 && P.levels2 > 0 ? WV_HALO : 0;
+  // x levels: line l is row l % Y of plane l / Y; y levels: line l is
+  // column l % X of plane l / X (after the x levels' shift)
+  auto xbase = [&](int l) {
+    const int p = l / Y;
+    return (p * PR + WV_HALO + (l - p * Y)) * RS + WV_HALO;
+  };
+  auto ybase = [&](int l) {
+    const int p = l / X;
+    return (p * PR + WV_HALO) * RS + WV_HALO + xs + (l - p * X);
+  };
+  int g = blockIdx.x;
+  const int G = gridDim.x;
+  if (g >= ntiles) return;
+  PlTile t = plane_tile(P, g);
+  float v[WV_PL_EPT];
+  plane_load<SRC>(P, t, v);
+  for (;;) {
+    plane_to_lds(P, t, v, smem);
+    __syncthreads();
+    const int gn = g + G;
+    PlTile tn = t;
+    if (gn < ntiles) {
+      tn = plane_tile(P, gn);
+      plane_load<SRC>(P, tn, v);
+    }
+    if (WAVE == WV_WAVELET_CDF97) {
+      plane97_levels(smem, P.T * Y, xbase, 1, X, P.levels);
+      plane97_levels(smem, P.T * X, ybase, RS, Y, P.levels2);
+    } else {
+      plane_haar_levels(smem, P.T * Y, xbase, 1, X, P.levels);
+      plane_haar_levels(smem, P.T * X, ybase, RS, Y, P.levels2);
+    }
+    plane_store(P, t, smem, xs, ys);
+    __syncthreads();
+    if (gn >= ntiles) break;
+    g = gn;
+    t = tn;
+  }
+}
+
 // ---- host side ------------------------------------------------------------
 // WV_DWT_VEC_ST=0 keeps the segment stores scalar (for comparison)
 static bool vec_store_off() {
@@ -844,12 +1117,54 @@ static void layout_strides(int layout, int64_t X, int64_t Y, int64_t Z, int64_t 
   else { st[0] = 1; st[1] = X; st[2] = X * Y; }
 }
 
+// WV_DWT_PLANE=0 turns the plane pass off (for comparison)
+static bool plane_off() {
+  static const bool v = [] {
+    const char *e = getenv("WV_DWT_PLANE");
+    return e && e[0] == '0';
+  }();
+  return v;
+}
+
+// autotuning hooks: WV_DWT_PLANE_MAX (largest X*Y taking the plane pass),
+// WV_DWT_PLANE_PP (planes per tile, clamped to what fits)
+static int env_int(const char *name, int dflt) {
+  const char *e = getenv(name);
+  return e ? atoi(e) : dflt;
+}
+
+// Planes per tile of the plane pass for X x Y planes, or 0 when the planes
+// do not take it: at most WV_PL_EPT elements per thread, at most one thread
+// per line and at most 16 pairs per thread at the first level in either
+// direction, and the tile within one block's LDS.
+static int plane_count(int X, int Y, int Z) {
+  static const int max_xy = env_int("WV_DWT_PLANE_MAX", 1024);
+  static const int want = env_int("WV_DWT_PLANE_PP", 0);
+  if (X < 8 || Y < 8 || X > 128 || Y > 128 || X * Y > max_xy) return 0;
+  const long plane_lds = (long)(Y + 2 * WV_HALO) * (X + 2 * WV_HALO + 1) * 4;
+  int pp = (int)std::min<long>({(long)WV_DBLOCK * WV_PL_EPT / (X * Y), (long)WV_DBLOCK / std::max(X, Y),
+                                (want > 0 ? 150 : 76) * 1024 / plane_lds, (long)Z});
+  if (want > 0 && want < pp) pp = want;
+  for (; pp >= 1; pp--) {
+    const int tx = WV_DBLOCK / (pp * Y), ty = WV_DBLOCK / (pp * X);   // threads per line
+    if (tx >= 1 && ty >= 1 && (X / 2 + tx - 1) / tx <= WV_PL_EPT / 2 &&
+        (Y / 2 + ty - 1) / ty <= WV_PL_EPT / 2)
+      return pp;
+  }
+  return 0;
+}
+
 int forward_specs(const int padded[3], const int steps[3], PassSpec s[3]) {
   const int *S = steps;
   const int Z = padded[2];
   int n = 0;
   if (!S[0] && !S[1] && !S[2]) {            // no transform, no rotation
     s[n++] = {0, 0, WV_LAYOUT_XYZ, WV_LAYOUT_XYZ};
+    return n;
+  }
+  if (S[0] && S[1] && !plane_off() && plane_count(padded[0], padded[1], Z) > 0) {
+    s[n++] = {WV_PASS_PLANE, S[0], WV_LAYOUT_XYZ, WV_LAYOUT_XYZ, S[1]};
+    if (S[2] || Z > 1) s[n++] = {2, S[2], WV_LAYOUT_XYZ, WV_LAYOUT_ZXY};
     return n;
   }
   if (S[0] && Z >= 32) {
@@ -899,6 +1214,23 @@ void pass_buffers(const PassSpec *s, int n, int dst_out[3]) {
 // blocks per CU); the others keep EPT = 32, whose segments across lines
 // are twice as long.
 int plan_pass(DwtPass &P, const PassSpec &s, bool clamp) {
+  if (s.axis == WV_PASS_PLANE) {
+    const int pp = plane_count(P.X, P.Y, P.Z);
+    if (pp <= 0) return -1;
+    P.plane = 1;
+    P.levels = s.levels;
+    P.levels2 = s.levels2;
+    P.L = P.X;
+    P.T = pp;
+    P.nt = WV_DBLOCK;
+    P.ept = WV_PL_EPT;
+    P.magic_L = (unsigned)(((1ull << 32) + (unsigned)P.X - 1) / (unsigned)P.X);
+    P.magic_XY = (unsigned)(((1ull << 32) + (unsigned)(P.X * P.Y) - 1) / (unsigned)(P.X * P.Y));
+    P.ntiles1 = (P.Z + pp - 1) / pp;
+    (void)clamp;
+    const long long total = (long long)P.ntiles1 * (P.src_tab ? (P.ncub > 0 ? P.ncub : 0) : 1);
+    return total > 0x7fffffff ? -1 : (int)total;
+  }
   const int dims[3] = {P.X, P.Y, P.Z};
   const int odims[3] = {P.ox, P.oy, P.oz};
   const int a = s.axis;
@@ -965,10 +1297,39 @@ int plan_pass(DwtPass &P, const PassSpec &s, bool clamp) {
   return total > 0x7fffffff ? -1 : (int)total;
 }
 
+template <int WAVE, typename SRC>
+static hipError_t launch_plane_one(const DwtPass &P, int ntiles, hipStream_t s) {
+  auto k = dwt_plane_kernel<WAVE, SRC>;
+  const size_t lds = (size_t)P.T * (P.Y + 2 * WV_HALO) * (P.X + 2 * WV_HALO + 1) * 4;
+  static const hipError_t attr = hipFuncSetAttribute(
+      (const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 152 * 1024);
+  if (attr != hipSuccess) return attr;
+  int occ = 0;
+  hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k, WV_DBLOCK, lds);
+  if (e != hipSuccess) return e;
+  const long cap = (long)(occ > 0 ? occ : 1) * cu_count();
+  const int grid = (int)(ntiles < cap ? ntiles : cap);
+  hipLaunchKernelGGL(k, dim3(grid), dim3(WV_DBLOCK), lds, s, P, ntiles);
+  return hipGetLastError();
+}
+
+template <int WAVE>
+static hipError_t launch_plane(const DwtPass &P, int src_dtype, int ntiles, hipStream_t s) {
+  if (src_dtype == WV_DTYPE_UINT8) return launch_plane_one<WAVE, uint8_t>(P, ntiles, s);
+  if (src_dtype == WV_DTYPE_INT16) return launch_plane_one<WAVE, int16_t>(P, ntiles, s);
+  if (src_dtype == WV_DTYPE_INT32) return launch_plane_one<WAVE, int32_t>(P, ntiles, s);
+  return launch_plane_one<WAVE, float>(P, ntiles, s);
+}
+
 hipError_t launch_pass(const DwtPass &P, int ntiles, int wavelet, bool inverse,
                        int src_dtype, bool clamp, bool stats, hipStream_t s) {
   (void)stats;
   if (ntiles <= 0) return hipSuccess;
+  if (P.plane) {
+    if (inverse) return hipErrorInvalidValue;
+    return wavelet == WV_WAVELET_CDF97 ? launch_plane<WV_WAVELET_CDF97>(P, src_dtype, ntiles, s)
+                                       : launch_plane<WV_WAVELET_HAAR>(P, src_dtype, ntiles, s);
+  }
   DwtPass Q = P;
   // exact tiles: a tile holds exactly nt*ept elements, the source
   // needs no clamping, and lane offsets (< nt/T rows of the widest

@@ -61,6 +61,11 @@ struct DwtPass {
   // lines are adjacent in the output (line stride L), so it stores one
   // contiguous block of T*L floats as 16-byte vectors
   int rt_exact, st_block;
+  // the plane pass (x and y levels of T planes per tile, x,y,z in place):
+  // plane set, levels2 = y levels (levels = x levels), magic_XY = exact
+  // reciprocal of X*Y (magic_L then holds X's)
+  int plane, levels2;
+  unsigned magic_XY;
 };
 
 // Memory layouts of a cubelet, fastest axis first: x,y,z (the input and
@@ -68,17 +73,23 @@ struct DwtPass {
 enum { WV_LAYOUT_XYZ = 0, WV_LAYOUT_YXZ = 1, WV_LAYOUT_ZXY = 2 };
 
 // One axis pass: levels along axis, reading layout lin, writing lout.
+// axis WV_PASS_PLANE: x levels (levels) and y levels (levels2) of whole
+// xy-planes in one pass, x,y,z in place.
+enum { WV_PASS_PLANE = 3 };
 struct PassSpec {
   int axis, levels, lin, lout;
+  int levels2;
 };
 
 // The forward pass sequence; returns the number of passes.  The reference
 // visits x, rotates, y, rotates, z (cdf97_3d dwt_cpu.cc:943-965); here only
-// the data layouts change between passes.  With x and y levels and depth
+// the data layouts change between passes.  With x and y levels on planes
+// of at most 32 x 32 (WV_DWT_PLANE_MAX), one plane pass runs both in x,y,z
+// and the z pass writes z,x,y.  Otherwise, with x and y levels and depth
 // >= 32 the x pass writes y,x,z and the y pass z,x,y, so every pass reads
-// whole lines; otherwise x and y stay in x,y,z and the z pass writes z,x,y
-// (with depth 1 the two orders coincide and no z pass runs unless it has
-// levels).  No levels at all: one conversion pass in x,y,z.
+// whole lines; else x and y stay in x,y,z and the z pass writes z,x,y (with
+// depth 1 the two orders coincide and no z pass runs unless it has levels).
+// No levels at all: one conversion pass in x,y,z.
 int forward_specs(const int padded[3], const int steps[3], PassSpec s[3]);
 // inverse: z (from z,x,y into x,y,z), then y, then x
 int inverse_specs(const int padded[3], const int steps[3], PassSpec s[3]);
