@@ -73,7 +73,8 @@ def submit(path):
     starts = [prev_end, bsel_f[0]]
     first = None
     for p in range(2):
-        idx = [i for i in range(starts[p] + 1, bsel_s[p]) if "dwt_pass_kernel" in names[i]]
+        idx = [i for i in range(starts[p] + 1, bsel_s[p])
+               if "dwt_pass_kernel" in names[i] or "dwt_plane_kernel" in names[i]]
         first = idx[0] if first is None else first
         span = (max(E(i) for i in idx) - min(S(i) for i in idx)) / 1e6
         sel = (E(bsel_f[p]) - S(bsel_s[p])) / 1e6
@@ -158,7 +159,7 @@ def pmc(counter_dir):
     names = sorted({c for d in agg.values() for c in d})
     out = ["| kernel | dispatches | " + " | ".join(names) + " |",
            "|---|---|" + "---|" * len(names)]
-    keep = ("dwt_pass", "copy_kernel", "quant_hist", "sel_collect")
+    keep = ("dwt_pass", "dwt_plane", "copy_kernel", "quant_hist", "sel_collect")
     for k in sorted(agg):
         if any(t in k for t in keep):
             d = agg[k]
