@@ -877,15 +877,16 @@ __device__ inline void plane_load(const DwtPass &P, const PlTile &t, float (&v)[
   const int64_t sz = P.src_sz > 0 ? P.src_sz : (int64_t)P.ox * P.oy;
   const int ne = t.np * P.X * P.Y;
   const int tid = opaque_tid();
+  // unconditional loads (elements past the tile re-read its last one): a
+  // load under a branch would make the branch join wait for it, and the
+  // loads would no longer be in flight together
 #pragma unroll
   for (int i = 0; i < WV_PL_EPT; i++) {
-    const int idx = tid + i * WV_DBLOCK;
-    if (idx < ne) {
-      int p, y, x;
-      plane_xyz(P, idx, p, y, x);
-      const int zc = min(t.z0 + p, P.oz - 1), yc = min(y, P.oy - 1), xc = min(x, P.ox - 1);
-      v[i] = to_f(src[zc * sz + yc * sy + xc]);
-    }
+    const int idx = min(tid + i * WV_DBLOCK, ne - 1);
+    int p, y, x;
+    plane_xyz(P, idx, p, y, x);
+    const int zc = min(t.z0 + p, P.oz - 1), yc = min(y, P.oy - 1), xc = min(x, P.ox - 1);
+    v[i] = to_f(src[zc * sz + yc * sy + xc]);
   }
 }
 
@@ -913,14 +914,20 @@ __device__ inline void plane_store(const DwtPass &P, const PlTile &t, const floa
   const int RS = P.X + 2 * WV_HALO + 1, PR = P.Y + 2 * WV_HALO;
   const int tid = opaque_tid();
   const int64_t base = (int64_t)t.z0 * XY;
+  // every LDS read first (unconditional, so they issue together), then the
+  // stores of the elements inside the tile
+  float o[WV_PL_EPT];
+#pragma unroll
+  for (int i = 0; i < WV_PL_EPT; i++) {
+    const int idx = min(tid + i * WV_DBLOCK, ne - 1);
+    int p, y, x;
+    plane_xyz(P, idx, p, y, x);
+    o[i] = lds[(p * PR + WV_HALO + y + ys) * RS + WV_HALO + x + xs];
+  }
 #pragma unroll
   for (int i = 0; i < WV_PL_EPT; i++) {
     const int idx = tid + i * WV_DBLOCK;
-    if (idx < ne) {
-      int p, y, x;
-      plane_xyz(P, idx, p, y, x);
-      dst[base + idx] = lds[(p * PR + WV_HALO + y + ys) * RS + WV_HALO + x + xs];
-    }
+    if (idx < ne) dst[base + idx] = o[i];
   }
 }
 
