@@ -707,15 +707,15 @@ dwt_pass_kernel(DwtPass P, int ntiles) {
 }
 
 // ---- the plane pass ---------------------------------------------------------
-// x and y levels of PP consecutive xy-planes in one pass (planes of at most
-// 128 x 128; taken for planes of at most WV_DWT_PLANE_MAX samples, 32 x 32
-// by default: on larger planes the three axis passes are faster, since the
-// plane tile's two sets of levels make the pass latency-bound): the planes
-// are loaded into LDS (converted from the source
-// type, coordinates clamped into the unpadded source), every row is
-// transformed along x, then every column along y, and the planes are stored
-// back in x,y,z (the z pass then writes z,x,y).  Two HBM round trips for the
-// three axes instead of three.  Plane p's row y sits at LDS row
+// x and y levels of PP consecutive xy-planes in one pass.  The kernel takes
+// planes of up to 128 x 128; forward_specs uses it for planes of at most
+// WV_DWT_PLANE_MAX samples (32 x 32 by default: on larger planes the three
+// axis passes are faster, since the plane tile's two sets of levels make
+// the pass latency-bound).  The planes are loaded into LDS (converted from
+// the source type, coordinates clamped into the unpadded source), every row
+// is transformed along x, then every column along y, and the planes are
+// stored back in x,y,z (the z pass then writes z,x,y): two HBM round trips
+// for the three axes instead of three.  Plane p's row y sits at LDS row
 // p*(Y+8) + 4 + y, its column x at 4 + x, with four spare rows above and
 // below each plane and four spare columns left and right of each row; the
 // row stride X + 9 is odd, so a wave's accesses along a row (y levels:
